@@ -1,0 +1,153 @@
+/*
+ * CPU oracle for the CDL-ADMM and PWM-scan hot path of CDLmotif.jl.
+ *
+ * TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+ * bench.py load this library as the checker.  The product (libcdlm) never links it.
+ *
+ * CDL part: oracle_impl.h, instantiated for float (_f32) and double (_f64).
+ * PWM part (below): plain C restatements of
+ *   greedy_search!  src/SEARCH/scan.jl:223-239  (position-major FP32 sum, threshold)
+ *   scan_w_gpu!     src/SEARCH/scan.jl:176-221  (argmax over positions, 1 -> "no hit")
+ *   scan_n          src/SEARCH/scan.jl:10-21    (nucleotide-major FP32 sum, first max)
+ *   overlapping_scan! src/SEARCH/scan.jl:89-121 (both strands, forward wins ties)
+ *   pwm_complement  src/SEARCH/helpers.jl:16    (reverse in both dimensions)
+ *   best_score      src/SEARCH/PWM_Touzet.jl:9  (sum of column maxima)
+ * Sequences are given as base codes 0..3 (A,C,G,T), the information content of the
+ * reference's one-hot columns (src/LOAD/to_dummy.jl:2-9; TYPE_setup.jl:33-36).
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define REAL float
+#define SFX _f32
+#include "oracle_impl.h"
+#undef REAL
+#undef SFX
+
+#define REAL double
+#define SFX _f64
+#include "oracle_impl.h"
+#undef REAL
+#undef SFX
+
+/* PWMs are packed (K, 4, maxlen) row-major: pwm[(k*4 + a)*maxlen + j] (zero padded). */
+
+/* Position-major sequential FP32 score of window starting at base l0 (0-based),
+ * complement selects the reverse-complement PWM. */
+static inline float pm_score(const uint8_t *seq, const float *pwm, int maxlen, int m, int l0, int comp) {
+    float acc = 0.0f;
+    for (int j = 0; j < m; j++) {
+        int b = seq[l0 + j];
+        float w = comp ? pwm[(3 - b) * maxlen + (m - 1 - j)] : pwm[b * maxlen + j];
+        acc = acc + w;
+    }
+    return acc;
+}
+
+/* Hit list: every window with score > thresh[k], both strands when both_strands != 0.
+ * Output records (k, n, pos0, strand) in (k, n, pos, strand) order.  Returns the number of
+ * hits; writes at most cap records. */
+long orc_pwm_hits(const uint8_t *seqs, int L, long N, const float *pwms, const int *lens, const float *thresh,
+                  int K, int maxlen, int both_strands, int32_t *out_k, int64_t *out_n, int32_t *out_pos,
+                  int8_t *out_strand, float *out_score, long cap) {
+    long cnt = 0;
+    for (int k = 0; k < K; k++) {
+        const float *pw = pwms + (long)k * 4 * maxlen;
+        int m = lens[k];
+        for (long n = 0; n < N; n++) {
+            const uint8_t *seq = seqs + n * (long)L;
+            for (int l0 = 0; l0 + m <= L; l0++)
+                for (int st = 0; st < (both_strands ? 2 : 1); st++) {
+                    float s = pm_score(seq, pw, maxlen, m, l0, st);
+                    if (s > thresh[k]) {
+                        if (cnt < cap) {
+                            out_k[cnt] = k; out_n[cnt] = n; out_pos[cnt] = l0; out_strand[cnt] = (int8_t)st;
+                            if (out_score) out_score[cnt] = s;
+                        }
+                        cnt++;
+                    }
+                }
+        }
+    }
+    return cnt;
+}
+
+/* scan_w_gpu!: pos_scores[k,n,l] = (s > thresh ? s : 0) for l <= L-m+1, else 0;
+ * argmax over l (first max); index 1 (1-based) reported as 0 = no hit.
+ * out_pos[(k*N + n)] is the 1-based position or 0. */
+void orc_pwm_gpu_argmax(const uint8_t *seqs, int L, long N, const float *pwms, const int *lens, const float *thresh,
+                        int K, int maxlen, int32_t *out_pos) {
+    for (int k = 0; k < K; k++) {
+        const float *pw = pwms + (long)k * 4 * maxlen;
+        int m = lens[k];
+        for (long n = 0; n < N; n++) {
+            const uint8_t *seq = seqs + n * (long)L;
+            float best = 0.0f; int bi = 0;
+            for (int l0 = 0; l0 < L; l0++) {
+                float v = 0.0f;
+                if (l0 + m <= L) {
+                    float s = pm_score(seq, pw, maxlen, m, l0, 0);
+                    v = (s > thresh[k]) ? s : 0.0f;
+                }
+                if (l0 == 0) { best = v; bi = 0; }
+                else if (v > best) { best = v; bi = l0; }
+            }
+            out_pos[(long)k * N + n] = (bi == 0) ? 0 : bi + 1;
+        }
+    }
+}
+
+/* scan_n with nucleotide-major order; returns the 1-based start or -1, score via *out. */
+static int scan_n_nm(const uint8_t *seq, int L, const float *pw, int maxlen, int m, int comp, float *out) {
+    int bi = -1; float best = -INFINITY;
+    for (int i = 0; i + m <= L; i++) {
+        float score = 0.0f;
+        for (int a = 0; a < 4; a++)
+            for (int j = 0; j < m; j++) {
+                if (seq[i + j] != a) continue;          /* product with a 0 entry adds a signed zero */
+                float w = comp ? pw[(3 - a) * maxlen + (m - 1 - j)] : pw[a * maxlen + j];
+                score += w;
+            }
+        if (score > best) { best = score; bi = i + 1; }
+    }
+    *out = best;
+    return bi;
+}
+
+/* overlapping_scan!: per (k,n) best forward and best reverse-complement windows; forward
+ * wins ties (score >= score_c); a hit needs score > thresh.  out_pos 1-based or 0,
+ * out_comp 0/1, out_score. */
+void orc_pwm_best(const uint8_t *seqs, int L, long N, const float *pwms, const int *lens, const float *thresh, int K,
+                  int maxlen, int32_t *out_pos, int8_t *out_comp, float *out_score) {
+    for (int k = 0; k < K; k++) {
+        const float *pw = pwms + (long)k * 4 * maxlen;
+        int m = lens[k];
+        for (long n = 0; n < N; n++) {
+            const uint8_t *seq = seqs + n * (long)L;
+            float s, sc;
+            int p = scan_n_nm(seq, L, pw, maxlen, m, 0, &s);
+            int pc = scan_n_nm(seq, L, pw, maxlen, m, 1, &sc);
+            long o = (long)k * N + n;
+            out_pos[o] = 0; out_comp[o] = 0; out_score[o] = 0.0f;
+            if (s >= sc) {
+                if (s > thresh[k]) { out_pos[o] = p; out_comp[o] = 0; out_score[o] = s; }
+            } else {
+                if (sc > thresh[k]) { out_pos[o] = pc; out_comp[o] = 1; out_score[o] = sc; }
+            }
+        }
+    }
+}
+
+/* best_score: sum over columns (sequential, FP32) of the column maximum. */
+float orc_best_score(const float *pwm, int maxlen, int m) {
+    float acc = 0.0f;
+    for (int j = 0; j < m; j++) {
+        float mx = pwm[j];
+        for (int a = 1; a < 4; a++) if (pwm[a * maxlen + j] > mx) mx = pwm[a * maxlen + j];
+        acc += mx;
+    }
+    return acc;
+}
