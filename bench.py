@@ -1,0 +1,166 @@
+"""Benchmark of the CDL-ADMM hot path on one MI355X.
+
+Workload (the single-dataset configuration): N=20000 sequences, L=200 bases (4L=800),
+M=64 filters of flattened length 20 (Lg=840), lambda=0.05, rho=1, sigma=100; one step is
+one full training of cdl_optimize (src/WRAPPER/wrap.jl:5-12): initialisation, then
+10 x (20 CSC iterations + update_best + 1 dictionary iteration) = 200 ADMM iterations, then
+the code threshold.  Metric: CSC-ADMM seq*iters/s = N * 200 / seconds per step.
+
+Inputs are resident in HBM before timing: the data matrix is uploaded once; each step
+re-initialises the state on the device (D0 from the host generator, X0 ~ N(0,1) drawn on
+the device), as the reference's constructor does.  Every library call synchronises its
+stream before returning, so the wall clock brackets complete device work.
+
+Printed: one JSON line with the metric, a roofline object for the dominant kernel (the
+fused CSC iteration, HBM-bound; per-launch average from HIP events on the library's
+stream over the timed steps) and a cpu_baseline object (the CPU oracle timed on a bounded
+slice of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cdlmotif.jl_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+N_SEQ, L_BASES, M_FILT, FLEN = 20000, 200, 64, 20
+ITER_NUM, CSC_ITER, CDL_ITER = 10, 20, 1
+LAM, RHO, SIGMA = 0.05, 1.0, 100.0
+HBM_PEAK_GBS = 8000.0
+
+
+def csc_bytes(N, Lg, M):
+    """Algorithmic HBM bytes of one CSC iteration: X, y2, y3 read and written (6M) plus
+    y1, Y1, DX read and written and the data read (7), per (l, n), 4 bytes each."""
+    return 4.0 * N * Lg * (6 * M + 7)
+
+
+def cpu_baseline(sample_n: int):
+    """The oracle (reference-structured FP32 restatement, OpenMP over columns) on a
+    sample_n-sequence slice of the workload for one outer iteration (20 CSC iterations,
+    update_best, 1 dictionary iteration): the same work per sequence-iteration as a step."""
+    import numpy as np
+    import oracle as O
+    from cdlm import synth
+    bases, _ = synth.c2_dataset(seed=2, N=sample_n, L=L_BASES)
+    s = synth.onehot(bases)
+    Lg = 4 * L_BASES + 2 * FLEN
+    fl = [FLEN] * M_FILT
+    D0 = synth.rand_probvec_filters(7, M_FILT, Lg, fl)
+    X0 = np.random.default_rng(8).standard_normal((sample_n, M_FILT, Lg)).astype(np.float32)
+    oc = O.OracleCDL(s, D0, X0, fl, RHO, SIGMA, LAM, "f32")
+    t0 = time.perf_counter()
+    oc.code_update(CSC_ITER)
+    oc.update_best()
+    oc.dict_update(CDL_ITER)
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": sample_n * CSC_ITER / dt, "unit": "seq*iters/s", "cores": threads, "kind": "port",
+            "sample": "oracle f32 reference-structured CDL on %d sequences of the same shape (L=%d, M=%d, "
+                      "filter_len=%d), one outer iteration (%d CSC + update_best + %d dict) in %.1f s"
+                      % (sample_n, L_BASES, M_FILT, FLEN, CSC_ITER, CDL_ITER, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=N_SEQ, help="sequences (default: the workload's 20000)")
+    ap.add_argument("--cpu-sample", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path; "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import cdlm
+    from cdlm import synth
+
+    N = args.n
+    Lg = 4 * L_BASES + 2 * FLEN
+    fl = [FLEN] * M_FILT
+    bases, _ = synth.c2_dataset(seed=2, N=N, L=L_BASES)
+    s = synth.onehot(bases)
+    D0 = synth.rand_probvec_filters(7, M_FILT, Lg, fl)
+    print("bench: building problem N=%d L=%d M=%d filter_len=%d" % (N, L_BASES, M_FILT, FLEN), file=sys.stderr,
+          flush=True)
+    c = cdlm.CDL(s, fl, RHO, SIGMA, LAM, D0=D0, X0=None, seed=1234)
+
+    def step():
+        c.reset()
+        c.optimize(ITER_NUM, CSC_ITER, CDL_ITER)
+        return c.code_thresh(1.0)
+
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        step()
+        print("bench: warmup step %d %.3f s" % (i, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    c.timing(True)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        th, cnt = step()
+    dt = time.perf_counter() - t0
+    csc_ms, csc_n = c.timing_read(0)
+    dict_ms, dict_n = c.timing_read(1)
+    best_ms, best_n = c.timing_read(2)
+    red_ms, red_n = c.timing_read(3)
+    c.timing(False)
+    ms_step = 1000.0 * dt / args.steps
+    iters = ITER_NUM * CSC_ITER
+    value = N * iters / (dt / args.steps)
+    avg_csc = csc_ms / max(csc_n, 1)
+    achieved = csc_bytes(N, Lg, M_FILT) / (avg_csc * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "csc_c2_traffic.json")
+    if os.path.exists(tpath) and N == N_SEQ:
+        try:
+            traffic = json.load(open(tpath)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    print("bench: %d steps %.3f s/step; CSC %.3f ms x %d, dict %.3f ms x %d, best %.3f ms x %d, reduce %.3f ms x %d;"
+          " code threshold %.5f (%d positive codes)"
+          % (args.steps, dt / args.steps, avg_csc, csc_n, dict_ms / max(dict_n, 1), dict_n,
+             best_ms / max(best_n, 1), best_n, red_ms / max(red_n, 1), red_n, th, cnt), file=sys.stderr, flush=True)
+    out = {
+        "metric": "CSC-ADMM seq*iters/s at N=20k,L=200,K=64",
+        "value": value,
+        "unit": "seq*iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic planted-motif sequences (5 Dirichlet(0.5) motifs, 20% occupancy), random init",
+        "config": {"workload": "single-dataset CDL: N=%d L=%d M=%d filter_len=%d Lg=%d lambda=%g rho=%g sigma=%g, "
+                               "%dx(%d CSC + update_best + %d dict) = %d ADMM iterations per step"
+                               % (N, L_BASES, M_FILT, FLEN, Lg, LAM, RHO, SIGMA, ITER_NUM, CSC_ITER, CDL_ITER, iters),
+                   "N": N, "L": L_BASES, "M": M_FILT, "filter_len": FLEN, "iterations": iters},
+        "roofline": {"bound": "hbm", "kernel": "k_csc (fused CSC iteration)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": csc_bytes(N, Lg, M_FILT), "avg_launch_ms": avg_csc,
+                     "launches": csc_n},
+        "kernels_ms": {"csc": avg_csc, "dict": dict_ms / max(dict_n, 1), "best": best_ms / max(best_n, 1),
+                       "dict_reduce": red_ms / max(red_n, 1)},
+    }
+    c.close()
+    if not args.no_cpu:
+        print("bench: CPU baseline on a %d-sequence slice" % args.cpu_sample, file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_sample)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

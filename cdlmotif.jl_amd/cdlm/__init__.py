@@ -1,0 +1,14 @@
+"""cdlm -- host side of the MI355X CDL-ADMM / PWM-scan library.
+
+Mirrors the reference's Julia entry points for the hot path of CDLmotif.jl:
+  CDL_setup, init_cdl, code_update!, dict_update!, update_best!, objective, cdl_optimize
+  (src/SETUP/WRAPPER_setup.jl, src/SETUP/CDL_setup.jl, src/CDL/train.jl, src/WRAPPER/wrap.jl)
+  scan_w_gpu!, overlapping_scan!, pfm2pwm, pwm_complement, best_score
+  (src/SEARCH/scan.jl, src/SEARCH/helpers.jl, src/SEARCH/PWM_Touzet.jl)
+All compute runs in libcdlm.so on a gfx950 device.
+"""
+from ._lib import CdlmError, CdlmOOM, load  # noqa: F401
+from .device import Context, default_context  # noqa: F401
+from .cdl import CDL, CDL_setup, init_cdl, code_update, dict_update, update_best, objective, cdl_optimize  # noqa: F401
+from .scan import (pfm2pwm, pwm_complement, best_score, pack_pwms, scan_hits, scan_w_gpu,  # noqa: F401
+                   overlapping_scan, onehot_to_bases)

@@ -1,0 +1,206 @@
+"""Host mirror of the reference's CDL entry points over libcdlm.
+
+Reference (CDLmotif.jl):
+  CDL_setup            src/SETUP/WRAPPER_setup.jl:1-13
+  cdl / init_cdl       src/SETUP/CDL_setup.jl:70-283
+  code_update!         src/CDL/train.jl:1-44
+  dict_update!         src/CDL/train.jl:46-89
+  update_best!         src/CDL/train.jl:91-98
+  objective            src/SETUP/CDL_setup.jl:286
+  cdl_optimize         src/WRAPPER/wrap.jl:1-32 (one training; the code threshold of :12)
+
+Array conventions: the device keeps the reference's Julia column-major layouts; numpy
+views use the same memory with reversed axes, so a Julia (Lg, M, N) array is a numpy
+(N, M, Lg) array, and the data matrix (4L, N) is numpy (N, 4L).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from ._lib import check, cdlm_params, load
+from .device import Context, default_context
+
+FIELD_IDS = {"X": 0, "DX": 1, "d": 2, "G1": 3, "G2": 4, "G3": 5, "Xbest_slice": 6, "Dbest": 7, "Y1": 8, "y1": 11,
+             "y2": 12, "y3": 13, "g1": 14, "g2": 15, "g3": 16, "Td": 17}
+
+
+@dataclass
+class CDL_setup:
+    """Defaults of CDL_setup (src/SETUP/WRAPPER_setup.jl:1-13)."""
+    M: int = 10
+    filter_size: int = 4
+    rho: float = 100.0
+    sigma: float = 100.0
+    lam: float = 0.01
+    count_threshold: int = 50
+    csc_iter_num: int = 15
+    cdl_iter_num: int = 1
+    iter_num: int = 10
+    std_threshold_multiplier: float = 1.0
+    num_train: int = 2
+    filter_len: Optional[Sequence[int]] = field(default=None)   # per-filter override (flattened lengths)
+
+    def filter_lens(self):
+        if self.filter_len is not None:
+            return [int(x) for x in self.filter_len]
+        return [4 * self.filter_size] * self.M
+
+
+def _f32c(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class CDL:
+    """Device-resident state of one CDL problem (the reference's mutable struct `cdl`)."""
+
+    def __init__(self, data: np.ndarray, filter_len: Sequence[int], rho: float, sigma: float, lam: float,
+                 D0: Optional[np.ndarray] = None, X0: Optional[np.ndarray] = None, seed: int = 0,
+                 sherman_csc: float = 2.0, sherman_cdl: float = 2.0, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        L = load()
+        data = _f32c(data)                     # (N, 4L)
+        N, L4 = data.shape
+        fl = np.ascontiguousarray(filter_len, dtype=np.int32)
+        self._fl = fl
+        prm = cdlm_params(len(fl), fl.ctypes.data_as(C.POINTER(C.c_int32)), rho, sigma, lam, sherman_csc, sherman_cdl)
+        g = int(fl.max())
+        Lg = L4 + 2 * g
+        d0 = None if D0 is None else _f32c(D0)
+        x0 = None if X0 is None else _f32c(X0)
+        if d0 is not None and d0.shape != (len(fl), Lg):
+            raise ValueError("D0 must be (M, Lg) = %s, got %s" % ((len(fl), Lg), d0.shape))
+        if x0 is not None and x0.shape != (N, len(fl), Lg):
+            raise ValueError("X0 must be (N, M, Lg) = %s, got %s" % ((N, len(fl), Lg), x0.shape))
+        h = C.c_void_p()
+        st = L.cdlm_problem_create(self.ctx.h, C.byref(prm), data.ctypes.data, L4, N,
+                                   None if d0 is None else d0.ctypes.data, None if x0 is None else x0.ctypes.data,
+                                   seed & 0xFFFFFFFFFFFFFFFF, C.byref(h))
+        check(st, self.ctx.h)
+        self.h = h
+        dims = [C.c_int32() for _ in range(5)]
+        check(L.cdlm_problem_dims(h, *[C.byref(x) for x in dims]), self.ctx.h)
+        self.Lg, self.gamma, self.L, self.N, self.M = [x.value for x in dims]
+        self.lam = lam
+
+    # -- lifecycle
+    def close(self):
+        if getattr(self, "h", None):
+            load().cdlm_problem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self):
+        check(load().cdlm_problem_reset(self.h), self.ctx.h)
+
+    @property
+    def device_bytes(self) -> int:
+        return int(load().cdlm_problem_bytes(self.h))
+
+    # -- reference methods
+    def code_update(self, num_iter: int):
+        check(load().cdlm_code_update(self.h, int(num_iter)), self.ctx.h)
+
+    def dict_update(self, num_iter: int):
+        check(load().cdlm_dict_update(self.h, int(num_iter)), self.ctx.h)
+
+    def update_best(self):
+        obj, imp = C.c_double(), C.c_int32()
+        check(load().cdlm_update_best(self.h, C.byref(obj), C.byref(imp)), self.ctx.h)
+        return obj.value, bool(imp.value)
+
+    def objective(self) -> float:
+        obj = C.c_double()
+        check(load().cdlm_objective(self.h, C.byref(obj)), self.ctx.h)
+        return obj.value
+
+    def obj_best(self) -> float:
+        v = C.c_float()
+        check(load().cdlm_obj_best(self.h, C.byref(v)), self.ctx.h)
+        return v.value
+
+    def optimize(self, iter_num: int, csc_iter: int, cdl_iter: int) -> np.ndarray:
+        tr = np.zeros(max(iter_num, 1), np.float64)
+        check(load().cdlm_optimize(self.h, iter_num, csc_iter, cdl_iter, tr.ctypes.data), self.ctx.h)
+        return tr[:iter_num]
+
+    def code_thresh(self, mult: float = 1.0):
+        t, cnt = C.c_double(), C.c_int64()
+        check(load().cdlm_code_thresh(self.h, mult, C.byref(t), C.byref(cnt)), self.ctx.h)
+        return t.value, cnt.value
+
+    def get(self, name: str) -> np.ndarray:
+        fid = FIELD_IDS[name]
+        Lg, M, N, L = self.Lg, self.M, self.N, self.L
+        shape = {"X": (N, M, Lg), "d": (N, M, Lg), "y2": (N, M, Lg), "y3": (N, M, Lg), "g1": (N, M, Lg),
+                 "g3": (N, M, Lg), "DX": (N, Lg), "G2": (N, Lg), "Y1": (N, Lg), "y1": (N, Lg), "g2": (N, Lg),
+                 "Td": (N, Lg), "G1": (M, Lg), "G3": (M, Lg), "Dbest": (M, Lg), "Xbest_slice": (N, M, L)}[name]
+        out = np.empty(shape, np.float32)
+        check(load().cdlm_get(self.h, fid, out.ctypes.data, out.nbytes), self.ctx.h)
+        return out
+
+    def timing(self, enable: bool):
+        """Start (True) or stop (False) per-launch HIP-event timing; starting clears totals."""
+        check(load().cdlm_timing_enable(self.h, 1 if enable else 0), self.ctx.h)
+
+    def timing_read(self, kernel: int):
+        """(total device ms, launches) of kernel 0=CSC, 1=dict, 2=best, 3=dict mean reduction."""
+        t, n = C.c_double(), C.c_int64()
+        check(load().cdlm_timing_read(self.h, kernel, C.byref(t), C.byref(n)), self.ctx.h)
+        return t.value, n.value
+
+    def codes(self, mult: float = 1.0):
+        """(codes, threshold): X_best[gamma+1:4:gamma+4L, :, :] and mult*std(X_best[X_best .> 0])
+        (src/WRAPPER/wrap.jl:12; src/SEARCH/store.jl:15-17)."""
+        th, _ = self.code_thresh(mult)
+        return self.get("Xbest_slice"), th
+
+
+# ---- reference-named free functions ---------------------------------------------------
+def init_cdl(setup: CDL_setup, data: np.ndarray, D0=None, X0=None, seed: int = 0, ctx: Optional[Context] = None) -> CDL:
+    """init_cdl(cdl_setup, dna_dataset) (src/SETUP/CDL_setup.jl:273-283)."""
+    return CDL(data, setup.filter_lens(), setup.rho, setup.sigma, setup.lam, D0=D0, X0=X0, seed=seed, ctx=ctx)
+
+
+def code_update(c: CDL, num_iter: int, sherman_const_csc: float = 2.0):
+    """code_update!(c, num_iter, ComplexF32) (src/CDL/train.jl:1-44); the Sherman-Morrison
+    constant is a problem parameter fixed at creation."""
+    c.code_update(num_iter)
+
+
+def dict_update(c: CDL, num_iter: int, sherman_const_cdl: float = 2.0):
+    """dict_update!(c, num_iter, ComplexF32) (src/CDL/train.jl:46-89)."""
+    c.dict_update(num_iter)
+
+
+def update_best(c: CDL):
+    """update_best!(c, dat) (src/CDL/train.jl:91-98) -> (objective, improved)."""
+    return c.update_best()
+
+
+def objective(c: CDL) -> float:
+    """objective(c, signals) (src/SETUP/CDL_setup.jl:286)."""
+    return c.objective()
+
+
+def cdl_optimize(setup: CDL_setup, data: np.ndarray, D0=None, X0=None, seed: int = 0, ctx: Optional[Context] = None,
+                 trainings: Optional[int] = None):
+    """The CDL part of cdl_optimize (src/WRAPPER/wrap.jl:4-12): for each training, init,
+    iter_num x (csc_iter_num code updates, update_best, cdl_iter_num dict updates), then the
+    code threshold.  Returns a list of (codes (N, M, L), threshold, objective trace)."""
+    out = []
+    for t in range(trainings if trainings is not None else setup.num_train):
+        c = init_cdl(setup, data, D0=D0, X0=X0, seed=seed + t, ctx=ctx)
+        trace = c.optimize(setup.iter_num, setup.csc_iter_num, setup.cdl_iter_num)
+        codes, th = c.codes(setup.std_threshold_multiplier)
+        out.append((codes, th, trace))
+        c.close()
+    return out

@@ -1,0 +1,173 @@
+// cdlm_internal.h -- shared host/device helpers of libcdlm (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/cdlm.h"
+
+struct cdlm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    int num_cu = 0;
+};
+
+// Records an error message on the context and returns the status.
+cdlm_status cdlm_fail(cdlm_ctx *ctx, cdlm_status st, const char *fmt, ...);
+
+#define CDLM_HIP(ctx, expr)                                                                        \
+    do {                                                                                           \
+        hipError_t _e = (expr);                                                                    \
+        if (_e != hipSuccess) {                                                                    \
+            if (_e == hipErrorOutOfMemory || _e == hipErrorMemoryAllocation)                       \
+                return cdlm_fail((ctx), CDLM_ERR_OOM, "%s: %s", #expr, hipGetErrorString(_e));     \
+            return cdlm_fail((ctx), CDLM_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e));         \
+        }                                                                                          \
+    } while (0)
+
+// ------------------------------------------------------------------ FFT plan
+// Stockham autosort, mixed radix.  Stage st uses radix[st]; the twiddle table holds
+// W_N^t = exp(-2 pi i t / N) for the table length N; a transform of length n = N / twstride
+// reads entry t * twstride for W_n^t.
+constexpr int kMaxStages = 24;
+struct FftPlan {
+    int n = 0;
+    int nst = 0;
+    int twstride = 1;
+    int radix[kMaxStages] = {0};
+};
+
+inline bool make_fft_plan(int n, int twstride, FftPlan *p) {
+    p->n = n;
+    p->twstride = twstride;
+    p->nst = 0;
+    int m = n;
+    while (m % 4 == 0) { if (p->nst >= kMaxStages) return false; p->radix[p->nst++] = 4; m /= 4; }
+    while (m % 2 == 0) { if (p->nst >= kMaxStages) return false; p->radix[p->nst++] = 2; m /= 2; }
+    for (int q = 3; m > 1; q += 2)
+        while (m % q == 0) { if (p->nst >= kMaxStages) return false; p->radix[p->nst++] = q; m /= q; }
+    return true;
+}
+
+// ------------------------------------------------------------------ device side
+#if defined(__HIPCC__)
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+
+template <bool INV>
+__device__ __forceinline__ float2 twid(const float2 *tw, int idx) {
+    float2 w = tw[idx];
+    if (INV) w.y = -w.y;
+    return w;
+}
+
+// Makes this wave's LDS writes visible to its own lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One radix-R butterfly j of a Stockham stage with current sub-length Ns.
+template <int R, bool INV>
+__device__ __forceinline__ void fft_bfly(const float2 *__restrict__ a, float2 *__restrict__ b, int j, int T, int Ns,
+                                         int n, const float2 *__restrict__ tw, int tws) {
+    const int k = j % Ns;
+    const int ob = (j - k) * R + k;
+    const int st = (n / (Ns * R)) * tws;
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        float2 x = a[j + r * T];
+        v[r] = (r == 0) ? x : cmul(x, twid<INV>(tw, r * k * st));
+    }
+    if constexpr (R == 2) {
+        b[ob] = cadd(v[0], v[1]);
+        b[ob + Ns] = csub(v[0], v[1]);
+    } else if constexpr (R == 4) {
+        float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        float2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+        // forward: W_4 = -i ; d13 * (-i) = (d13.y, -d13.x)
+        float2 rot = INV ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);
+        b[ob] = cadd(s02, s13);
+        b[ob + Ns] = cadd(d02, rot);
+        b[ob + 2 * Ns] = csub(s02, s13);
+        b[ob + 3 * Ns] = csub(d02, rot);
+    } else {
+        float2 wr[R];
+        const int rs = (n / R) * tws;
+#pragma unroll
+        for (int t = 0; t < R; ++t) wr[t] = twid<INV>(tw, t * rs);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            float2 acc = v[0];
+#pragma unroll
+            for (int r = 1; r < R; ++r) acc = cadd(acc, cmul(v[r], wr[(r * q) % R]));
+            b[ob + q * Ns] = acc;
+        }
+    }
+}
+
+// Team-cooperative complex FFT of length P.n from a into b (ping-pong).  The team is
+// threads [tid0, tid0+nthr) that all call this; SYNC is the team barrier.  Returns the
+// buffer holding the (unscaled) result.  Forward: exp(-2 pi i kl/n); INV: exp(+...).
+template <bool INV, typename Sync>
+__device__ float2 *team_fft(float2 *a, float2 *b, const FftPlan &P, const float2 *__restrict__ tw, int tid, int nthr,
+                            Sync sync) {
+    const int n = P.n;
+    const int tws = P.twstride;
+    int Ns = 1;
+    for (int st = 0; st < P.nst; ++st) {
+        const int R = P.radix[st];
+        const int T = n / R;
+        switch (R) {
+        case 2: for (int j = tid; j < T; j += nthr) fft_bfly<2, INV>(a, b, j, T, Ns, n, tw, tws); break;
+        case 3: for (int j = tid; j < T; j += nthr) fft_bfly<3, INV>(a, b, j, T, Ns, n, tw, tws); break;
+        case 4: for (int j = tid; j < T; j += nthr) fft_bfly<4, INV>(a, b, j, T, Ns, n, tw, tws); break;
+        case 5: for (int j = tid; j < T; j += nthr) fft_bfly<5, INV>(a, b, j, T, Ns, n, tw, tws); break;
+        case 7: for (int j = tid; j < T; j += nthr) fft_bfly<7, INV>(a, b, j, T, Ns, n, tw, tws); break;
+        default: {
+            // Large prime radix: one output (j, q) per work item,
+            // y = sum_r a[j + rT] * W_{Ns R}^{r (k + q Ns)}.
+            const int NsR = Ns * R;
+            const int st2 = (n / NsR) * tws;
+            for (int idx = tid; idx < n; idx += nthr) {
+                const int j = idx % T, q = idx / T;
+                const int k = j % Ns;
+                const int kk = k + q * Ns;
+                float2 acc = a[j];
+                int e = 0;
+                for (int r = 1; r < R; ++r) {
+                    e += kk;
+                    if (e >= NsR) e -= NsR;
+                    acc = cadd(acc, cmul(a[j + r * T], twid<INV>(tw, e * st2)));
+                }
+                b[(j - k) * R + kk] = acc;
+            }
+        } break;
+        }
+        sync();
+        float2 *t = a; a = b; b = t;
+        Ns *= R;
+    }
+    return a;
+}
+
+struct BlockSync {
+    __device__ __forceinline__ void operator()() const { __syncthreads(); }
+};
+struct WaveSync {
+    __device__ __forceinline__ void operator()() const { wave_lds_sync(); }
+};
+
+#endif  // __HIPCC__

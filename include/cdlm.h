@@ -1,0 +1,182 @@
+/*
+ * cdlm.h -- C ABI of the MI355X (gfx950) CDL-ADMM and PWM-scan library (libcdlm.so).
+ *
+ * This ABI is the drop-in boundary for the hot path of CDLmotif.jl.  The reference has
+ * no FFI; its hot path is a set of Julia methods on the mutable struct `cdl`
+ * (src/SETUP/CDL_setup.jl:70-164) and on `good_stuff` (src/SETUP/WRAPPER_setup.jl:47-62).
+ * Each entry point below names the reference method it replaces.  A Julia host binds them
+ * with `ccall` (see the binding in INTEGRATION.md); Python binds them with ctypes.
+ *
+ * Conventions
+ *  - Arrays cross the ABI as plain pointers in Julia column-major order (first index
+ *    fastest), FP32, exactly the reference's layouts:
+ *      s   (4L, N)        one-hot data matrix (src/LOAD/to_dummy.jl:29-38)
+ *      D0  (Lg, M)        initial filters, Lg = 4L + 2*max(filter_len)
+ *      X0  (Lg, M, N)     initial codes
+ *  - Sizes are int32 (the reference's int_t, src/SETUP/TYPE_setup.jl:4); N may be int64
+ *    for the PWM scan.  Positions returned are 0-based (the Julia side adds 1).
+ *  - The library owns all device memory; host buffers are borrowed for the call only.
+ *  - Every call is synchronous for the caller (results are in host memory when it
+ *    returns) unless its comment says otherwise.  A handle is not thread-safe.
+ *  - Errors: a cdlm_status code plus cdlm_last_error(ctx).  Out-of-memory is its own code
+ *    (CDLM_ERR_OOM), never a crash.
+ */
+#ifndef CDLM_H
+#define CDLM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    CDLM_OK = 0,
+    CDLM_ERR_ARG = 1,          /* bad argument (shape, null pointer, value out of range) */
+    CDLM_ERR_HIP = 2,          /* HIP runtime error */
+    CDLM_ERR_OOM = 3,          /* device memory exhausted or admission budget exceeded */
+    CDLM_ERR_UNSUPPORTED = 4,  /* shape outside what the kernels support (e.g. filter_len > 32) */
+    CDLM_ERR_STATE = 5         /* call not valid in the current state */
+} cdlm_status;
+
+typedef struct cdlm_ctx cdlm_ctx;
+typedef struct cdlm_problem cdlm_problem;
+
+/* Hyper-parameters of one CDL problem: the fields of CDL_setup that reach the `cdl`
+ * constructor (src/SETUP/WRAPPER_setup.jl:1-13; CDL_setup.jl:165-172,273-283). */
+typedef struct {
+    int32_t M;                  /* number of filters */
+    const int32_t *filter_len;  /* M flattened filter lengths (4*filter_size), multiples of 4, <= 32 */
+    float rho;                  /* CSC penalty (used by update_Y1!, csc_kernels.jl:91) */
+    float sigma;                /* CDL penalty (update_g2!, cdl_kernels.jl:73) */
+    float lambda;               /* sparsity weight (shrink!, csc_kernels.jl:61) */
+    float sherman_csc;          /* sherman_const_csc, 2 in the reference (train.jl:4) */
+    float sherman_cdl;          /* sherman_const_cdl, 2 in the reference (train.jl:49) */
+} cdlm_params;
+
+/* ---- context ---------------------------------------------------------------------- */
+/* One context per device; owns the HIP stream the problems run on. */
+cdlm_status cdlm_create(cdlm_ctx **out, int32_t device);
+void cdlm_destroy(cdlm_ctx *ctx);
+const char *cdlm_last_error(const cdlm_ctx *ctx);
+/* Library version string, and a device-memory query (free, total bytes). */
+const char *cdlm_version(void);
+cdlm_status cdlm_mem_info(cdlm_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
+
+/* ---- CDL ADMM ----------------------------------------------------------------------- */
+/* Replaces init_cdl / the cdl{T,S,Q} constructor (src/SETUP/CDL_setup.jl:165-283).
+ * s: (L4, N) data.  D0: (Lg, M) filters before masking (NULL: drawn on the host from
+ * seed as masked rand_probvec groups, CDL_setup.jl:59-68,183).  X0: (Lg, M, N) codes
+ * before the stride-4 projection (NULL: N(0,1) drawn on the device from seed, the
+ * CUDA.randn of CDL_setup.jl:191).  The problem state stays resident on the device. */
+cdlm_status cdlm_problem_create(cdlm_ctx *ctx, const cdlm_params *params, const float *s, int32_t L4, int32_t N,
+                                const float *D0, const float *X0, uint64_t seed, cdlm_problem **out);
+cdlm_status cdlm_problem_destroy(cdlm_problem *p);
+/* Re-runs the initialisation from the same D0 / X0 / seed without re-uploading s. */
+cdlm_status cdlm_problem_reset(cdlm_problem *p);
+/* Dimensions: Lg (padded length), gamma (max filter length), L (bases), N, M. */
+cdlm_status cdlm_problem_dims(const cdlm_problem *p, int32_t *Lg, int32_t *gamma, int32_t *L, int32_t *N,
+                              int32_t *M);
+/* Device bytes held by the problem. */
+size_t cdlm_problem_bytes(const cdlm_problem *p);
+/* Device bytes a problem of this shape would hold (admission by HBM footprint). */
+size_t cdlm_footprint_bytes(int32_t L4, int32_t N, int32_t M, int32_t gamma, int32_t keep_x0);
+
+/* code_update!(c, num_iter, ComplexF32; sherman_const_csc)  (src/CDL/train.jl:1-44) */
+cdlm_status cdlm_code_update(cdlm_problem *p, int32_t num_iter);
+/* dict_update!(c, num_iter, ComplexF32; sherman_const_cdl)  (src/CDL/train.jl:46-89) */
+cdlm_status cdlm_dict_update(cdlm_problem *p, int32_t num_iter);
+/* update_best!(c, dat)  (src/CDL/train.jl:91-98).  obj: the objective (Float64, as the
+ * reference's objective() returns); improved: 1 when X_best/D_best were replaced. */
+cdlm_status cdlm_update_best(cdlm_problem *p, double *obj, int32_t *improved);
+/* objective(c, signals)  (src/SETUP/CDL_setup.jl:286) */
+cdlm_status cdlm_objective(cdlm_problem *p, double *obj);
+/* The schedule of cdl_optimize for one training (src/WRAPPER/wrap.jl:5-11):
+ * iter_num x { csc_iter x code_update; update_best; cdl_iter x dict_update }.
+ * obj_trace (may be NULL) receives the iter_num objectives seen by update_best.
+ * Runs without host synchronisation until the end. */
+cdlm_status cdlm_optimize(cdlm_problem *p, int32_t iter_num, int32_t csc_iter, int32_t cdl_iter, double *obj_trace);
+/* mult * std(X_best[X_best .> 0])  (src/WRAPPER/wrap.jl:12), corrected std; count of
+ * positive entries in *count. */
+cdlm_status cdlm_code_thresh(cdlm_problem *p, double mult, double *thresh, int64_t *count);
+/* The reference's obj_best field (Float32). */
+cdlm_status cdlm_obj_best(cdlm_problem *p, float *obj_best);
+
+/* Field read-out (copies into a caller buffer of exactly the field's size in bytes). */
+typedef enum {
+    CDLM_F_X = 0,        /* (Lg, M, N) codes */
+    CDLM_F_DX = 1,       /* (Lg, N) */
+    CDLM_F_D = 2,        /* (Lg, M, N) consensus filters d */
+    CDLM_F_G1 = 3,       /* (Lg, M) */
+    CDLM_F_G2 = 4,       /* (Lg, N) */
+    CDLM_F_G3 = 5,       /* (Lg, M) */
+    CDLM_F_XBEST_SLICE = 6, /* (L, M, N) = X_best[gamma+1:4:gamma+4L, :, :] (src/SEARCH/store.jl:15) */
+    CDLM_F_DBEST = 7,    /* (Lg, M) */
+    CDLM_F_Y1 = 8,       /* (Lg, N) */
+    CDLM_F_y1 = 11,      /* (Lg, N) */
+    CDLM_F_y2 = 12,      /* (Lg, M, N) */
+    CDLM_F_y3 = 13,      /* (Lg, M, N) */
+    CDLM_F_g1 = 14,      /* (Lg, M, N) */
+    CDLM_F_g2 = 15,      /* (Lg, N) */
+    CDLM_F_g3 = 16,      /* (Lg, M, N) */
+    CDLM_F_TD = 17       /* (Lg, N) */
+} cdlm_field;
+cdlm_status cdlm_get(cdlm_problem *p, int32_t field, void *dst, size_t bytes);
+
+/* Kernel timing with HIP events recorded on the library's stream around each launch (the
+ * reference only prints @time wall clocks, src/WRAPPER/wrap.jl:30-137).  enable != 0 starts
+ * a fresh accumulation; read returns the summed device time and launch count of one
+ * kernel: 0 = fused CSC iteration, 1 = dictionary iteration (per-sequence kernel),
+ * 2 = best/objective pass, 3 = dictionary mean reduction. */
+cdlm_status cdlm_timing_enable(cdlm_problem *p, int32_t enable);
+cdlm_status cdlm_timing_read(cdlm_problem *p, int32_t kernel, double *total_ms, int64_t *launches);
+/* Device copy of `bytes` bytes, `reps` times, on the context stream; average ms per copy.
+ * A plain streaming kernel of known traffic, used to calibrate HBM byte counters. */
+cdlm_status cdlm_copy_bench(cdlm_ctx *ctx, size_t bytes, int32_t reps, double *ms_per_copy);
+
+/* ---- PWM scan ----------------------------------------------------------------------- */
+/* PWMs are packed (K, 4, maxlen) row-major, pwm[(k*4 + a)*maxlen + j], zero padded past
+ * lens[k]; rows a = A,C,G,T.  Sequences are base codes 0..3, (N, L) row-major, one byte
+ * each (the information content of the reference's one-hot columns).  Scores are FP32. */
+
+/* All windows with score > thresh[k]: the window score is the position-major sequential
+ * FP32 sum of greedy_search! (src/SEARCH/scan.jl:223-239); strand 1 uses the reverse
+ * complement PWM (pwm_complement = reverse in both dimensions, src/SEARCH/helpers.jl:16).
+ * Hit records (k, n, pos, strand); order unspecified (sort on the host for a canonical
+ * order).  On CDLM_OK *n_hits is the total; if it exceeds cap, only cap records were
+ * written and the call returns CDLM_ERR_OOM (rerun with cap >= *n_hits). */
+cdlm_status cdlm_pwm_hits(cdlm_ctx *ctx, const uint8_t *bases, int32_t L, int64_t N, const float *pwms,
+                          const int32_t *lens, const float *thresh, int32_t K, int32_t maxlen, int32_t both_strands,
+                          int32_t *hit_k, int64_t *hit_n, int32_t *hit_pos, int8_t *hit_strand, int64_t cap,
+                          int64_t *n_hits);
+/* scan_w_gpu! (src/SEARCH/scan.jl:176-221): forward strand, per (k, n) the first argmax
+ * over positions of (score > thresh ? score : 0) with positions past L-m+1 at 0; a best
+ * position of 1 (1-based) is "no hit".  out_pos (K, N) row-major: 1-based position or 0. */
+cdlm_status cdlm_pwm_argmax(cdlm_ctx *ctx, const uint8_t *bases, int32_t L, int64_t N, const float *pwms,
+                            const int32_t *lens, const float *thresh, int32_t K, int32_t maxlen, int32_t *out_pos);
+/* overlapping_scan! (src/SEARCH/scan.jl:89-121 with scan_n :10-21): nucleotide-major FP32
+ * sums, best window per strand (first maximum), forward wins ties, hit iff > thresh.
+ * Outputs (K, N) row-major: 1-based position or 0, strand (1 = reverse complement),
+ * score (0 when no hit). */
+cdlm_status cdlm_pwm_best(cdlm_ctx *ctx, const uint8_t *bases, int32_t L, int64_t N, const float *pwms,
+                          const int32_t *lens, const float *thresh, int32_t K, int32_t maxlen, int32_t *out_pos,
+                          int8_t *out_strand, float *out_score);
+/* One-hot (4L, N) FP32 columns to base codes (N, L); a column with no 1 in a 4-group
+ * gets code 4 (never matches a PWM row).  Replaces the device side of data_2_dummy. */
+cdlm_status cdlm_onehot_to_bases(cdlm_ctx *ctx, const float *onehot, int32_t L4, int64_t N, uint8_t *bases);
+
+/* ---- Device-resident variants (inputs already in HBM, for benchmarks and hosts that
+ * keep data on the device).  Pointers are device pointers; outputs stay on the device
+ * and the call returns after enqueueing unless noted. ------------------------------- */
+/* Hit list into device buffers (hit_k, hit_n, hit_pos, hit_strand), counter in
+ * d_count (device int64, zeroed by the call).  Synchronous: *n_hits is valid on return. */
+cdlm_status cdlm_pwm_hits_device(cdlm_ctx *ctx, const uint8_t *d_bases, int32_t L, int64_t N, const float *d_pwms,
+                                 const int32_t *lens, const float *thresh, int32_t K, int32_t maxlen,
+                                 int32_t both_strands, int32_t *d_hit_k, int64_t *d_hit_n, int32_t *d_hit_pos,
+                                 int8_t *d_hit_strand, int64_t cap, int64_t *n_hits);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CDLM_H */

@@ -41,6 +41,7 @@ static inline float pm_score(const uint8_t *seq, const float *pwm, int maxlen, i
     float acc = 0.0f;
     for (int j = 0; j < m; j++) {
         int b = seq[l0 + j];
+        if (b > 3) { acc = acc + 0.0f; continue; }    /* all-zero one-hot column adds +0 */
         float w = comp ? pwm[(3 - b) * maxlen + (m - 1 - j)] : pwm[b * maxlen + j];
         acc = acc + w;
     }

@@ -1,0 +1,163 @@
+"""GPU parity of the CDL ADMM against the CPU oracle (reference-structured FP32 and FP64)
+and the committed golden fixtures.  Tolerance: relative norm error <= 1e-4 for every
+compared array (|a-b|_2 / |b|_2), FP32 GPU vs FP32 oracle; FP64 error is printed."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gpu_util import flush, rel
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-4
+
+cdlm = pytest.importorskip("cdlm")
+
+
+def _synth(seed, N, L, M, fl):
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    from make_golden import make_cdl_inputs
+    return make_cdl_inputs(seed, N, L, M, fl)
+
+
+def _both(s, D0, X0, fl, rho, sigma, lam):
+    g = cdlm.CDL(s, fl, rho, sigma, lam, D0=D0, X0=X0)
+    o32 = O.OracleCDL(s, D0, X0, fl, rho, sigma, lam, "f32")
+    return g, o32
+
+
+@pytest.mark.parametrize("case", ["cdl_tiny", "cdl_small", "cdl_mixedfl", "cdl_long"])
+def test_golden_schedule(case, golden_dir):
+    flush("case %s" % case)
+    z = np.load(os.path.join(golden_dir, case + ".npz"))
+    rho, sigma, lam = [float(x) for x in z["params"]]
+    g = cdlm.CDL(z["s"], z["fl"], rho, sigma, lam, D0=z["D0"], X0=z["X0"])
+    outer, ncsc, ndict = [int(x) for x in z["sched"]]
+    objs = []
+    for _ in range(outer):
+        g.code_update(ncsc)
+        o, _imp = g.update_best()
+        objs.append(o)
+        g.dict_update(ndict)
+    np.testing.assert_allclose(objs, z["obj"], rtol=TOL)
+    for f in ("X", "DX", "d", "G1", "G3", "y1", "y2", "y3", "Y1", "g1", "g2", "g3", "Td", "Dbest"):
+        e = rel(g.get(f), z[f])
+        assert e < TOL, (f, e)
+    L = g.L
+    xs = g.get("Xbest_slice")
+    np.testing.assert_allclose(xs, z["Xbest"][:, :, g.gamma:g.gamma + 4 * L:4], rtol=0, atol=TOL * np.abs(z["Xbest"]).max())
+    th, cnt = g.code_thresh(1.0)
+    assert abs(th - float(z["thresh"])) <= TOL * float(z["thresh"])
+
+
+def test_single_steps_match_oracle():
+    fl = [16] * 5
+    s, D0, X0, _ = _synth(21, 12, 25, 5, fl)
+    g, o = _both(s, D0, X0, fl, 1.0, 100.0, 0.05)
+    assert abs(g.objective() - o.objective()) <= TOL * abs(o.objective())
+    for f in ("X", "DX", "d", "G1"):
+        assert rel(g.get(f), o.get(f)) < 1e-6, f
+    g.code_update(1); o.code_update(1)
+    for f in ("X", "DX", "y1", "y2", "y3", "Y1"):
+        assert rel(g.get(f), o.get(f)) < TOL, f
+    ob_g, imp_g = g.update_best(); ob_o, imp_o = o.update_best()
+    assert abs(ob_g - ob_o) <= TOL * abs(ob_o) and imp_g == imp_o
+    g.dict_update(1); o.dict_update(1)
+    for f in ("d", "g1", "g2", "g3", "G1", "G3", "G2", "Td", "Dbest"):
+        assert rel(g.get(f), o.get(f)) < TOL, f
+
+
+def test_c1_like_full_schedule_vs_oracle():
+    """Configuration-1 shape (L=100, M=10, filter_len=16, lambda=0.05, rho=1) with N=60,
+    10 outer x (10 CSC + best + 1 dict) = 100 ADMM iterations."""
+    fl = [16] * 10
+    s, D0, X0, _ = _synth(1, 60, 100, 10, fl)
+    g, o = _both(s, D0, X0, fl, 1.0, 100.0, 0.05)
+    o64 = O.OracleCDL(s, D0, X0, fl, 1.0, 100.0, 0.05, "f64")
+    tg, to, t64 = [], [], []
+    for _ in range(10):
+        g.code_update(10); o.code_update(10); o64.code_update(10)
+        tg.append(g.update_best()[0]); to.append(o.update_best()[0]); t64.append(o64.update_best()[0])
+        g.dict_update(1); o.dict_update(1); o64.dict_update(1)
+    np.testing.assert_allclose(tg, to, rtol=TOL)
+    for f in ("X", "DX", "d", "G1", "G3"):
+        e32, e64 = rel(g.get(f), o.get(f)), rel(g.get(f), o64.get(f))
+        flush("C1-like %s: rel err vs oracle f32 %.2e, vs f64 %.2e (oracle f32 vs f64 %.2e)"
+              % (f, e32, e64, rel(o.get(f), o64.get(f))))
+        assert e32 < TOL, f
+    tg_, cg = g.code_thresh(1.0)
+    to_, co = o.code_thresh(1.0)
+    assert abs(tg_ - to_) <= TOL * to_
+    assert abs(cg - co) <= max(2, 1e-3 * co)
+
+
+def test_c2_shape_slice_vs_oracle():
+    """Configuration-2 shape per sequence (L=200, M=64, filter_len=20, Lg=840) on a
+    24-sequence slice, 10 outer x (20 CSC + best + 1 dict) = 200 ADMM iterations."""
+    fl = [20] * 64
+    s, D0, X0, _ = _synth(2, 24, 200, 64, fl)
+    g, o = _both(s, D0, X0, fl, 1.0, 100.0, 0.05)
+    tg = g.optimize(10, 20, 1)
+    to = []
+    for _ in range(10):
+        o.code_update(20)
+        to.append(o.update_best()[0])
+        o.dict_update(1)
+    flush("C2-slice obj trace gpu %s" % np.array2string(np.asarray(tg), precision=6))
+    np.testing.assert_allclose(tg, to, rtol=TOL)
+    for f in ("X", "DX", "d", "G1", "G3"):
+        e = rel(g.get(f), o.get(f))
+        flush("C2-slice %s rel err %.2e" % (f, e))
+        assert e < TOL, f
+
+
+def test_csc_is_per_sequence_exact():
+    """Size-independent property: a CSC iteration touches each sequence alone, so sequences
+    5..8 of a 40-sequence problem equal the same 4 sequences run as their own problem, bit
+    for bit (the dictionary is not updated)."""
+    fl = [20] * 8
+    s, D0, X0, _ = _synth(3, 40, 50, 8, fl)
+    a = cdlm.CDL(s, fl, 1.0, 100.0, 0.05, D0=D0, X0=X0)
+    b = cdlm.CDL(s[5:9], fl, 1.0, 100.0, 0.05, D0=D0, X0=X0[5:9])
+    a.code_update(7); b.code_update(7)
+    for f in ("X", "DX", "y2", "y3", "Y1", "y1"):
+        np.testing.assert_array_equal(a.get(f)[5:9], b.get(f), err_msg=f)
+
+
+def test_projection_invariants():
+    """y3 is exactly 0 on the stride-4 support after a CSC iteration (y3 += X - Pi4(X + y3)),
+    G1 and G3 vanish off the filter support, G3 lies in [0, 1]."""
+    fl = [12, 8, 16]
+    s, D0, X0, _ = _synth(4, 10, 30, 3, fl)
+    g = cdlm.CDL(s, fl, 1.0, 100.0, 0.05, D0=D0, X0=X0)
+    g.code_update(3)
+    y3 = g.get("y3")
+    assert np.all(y3[:, :, 0::4] == 0)
+    g.update_best(); g.dict_update(2)
+    G1, G3 = g.get("G1"), g.get("G3")
+    for m, f in enumerate(fl):
+        assert np.all(G1[m, f:] == 0) and np.all(G3[m, f:] == 0)
+    assert G3.min() >= 0 and G3.max() <= 1
+
+
+def test_reset_reproduces():
+    fl = [16] * 4
+    s, D0, X0, _ = _synth(5, 8, 20, 4, fl)
+    g = cdlm.CDL(s, fl, 1.0, 100.0, 0.05, D0=D0, X0=X0)
+    t1 = g.optimize(3, 4, 1)
+    x1 = g.get("X")
+    g.reset()
+    t2 = g.optimize(3, 4, 1)
+    np.testing.assert_array_equal(t1, t2)
+    np.testing.assert_array_equal(x1, g.get("X"))
+
+
+def test_bad_arguments_raise():
+    with pytest.raises(cdlm.CdlmError):
+        cdlm.CDL(np.zeros((3, 10), np.float32), [16], 1.0, 1.0, 0.1)     # 4L not a multiple of 4
+    with pytest.raises(cdlm.CdlmError):
+        cdlm.CDL(np.zeros((3, 16), np.float32), [18], 1.0, 1.0, 0.1)     # filter_len not a multiple of 4
+    with pytest.raises(cdlm.CdlmError):
+        cdlm.CDL(np.zeros((3, 16), np.float32), [36], 1.0, 1.0, 0.1)     # beyond 32 taps

@@ -1,0 +1,101 @@
+"""GPU PWM scans: bit-exact against the CPU oracle and the golden fixture."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gpu_util import flush
+
+pytestmark = pytest.mark.gpu
+cdlm = pytest.importorskip("cdlm")
+
+
+def _fixture(golden_dir):
+    return np.load(os.path.join(golden_dir, "pwm_small.npz"))
+
+
+def test_hits_golden(golden_dir):
+    z = _fixture(golden_dir)
+    h = cdlm.scan_hits(z["bases"], z["pwms"], z["lens"], z["thresh"], both_strands=True)
+    np.testing.assert_array_equal(h, z["hits"])
+
+
+def test_argmax_golden(golden_dir):
+    z = _fixture(golden_dir)
+    np.testing.assert_array_equal(cdlm.scan_w_gpu(z["bases"], z["pwms"], z["lens"], z["thresh"]), z["argmax_pos"])
+
+
+def test_best_golden(golden_dir):
+    z = _fixture(golden_dir)
+    pos, comp, sc = cdlm.overlapping_scan(z["bases"], z["pwms"], z["lens"], z["thresh"])
+    np.testing.assert_array_equal(pos, z["best_pos"])
+    np.testing.assert_array_equal(comp.astype(np.int8), z["best_comp"])
+    np.testing.assert_array_equal(sc.view(np.uint32), z["best_score"].view(np.uint32))
+
+
+def _random_case(seed, N, L, K, lmin=8, lmax=24):
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(lmin, lmax + 1, K).astype(np.int32)
+    maxlen = int(lens.max())
+    pwms = np.zeros((K, 4, maxlen), np.float32)
+    for k in range(K):
+        p = rng.dirichlet(0.5 * np.ones(4), lens[k])
+        p = (p + 0.01) / 1.04
+        pwms[k, :, :lens[k]] = np.log2(p / 0.25).T.astype(np.float32)
+    bases = rng.choice(4, size=(N, L), p=[0.29, 0.21, 0.21, 0.29]).astype(np.uint8)
+    thresh = np.array([np.float32(0.8) * O.best_score(pwms[k, :, :lens[k]]) for k in range(K)], np.float32)
+    return bases, pwms, lens, thresh
+
+
+def test_hits_random_vs_oracle():
+    bases, pwms, lens, thresh = _random_case(5, 300, 500, 24)
+    k, n, p, st, _ = O.pwm_hits(bases, pwms, lens, thresh, both_strands=True)
+    exp = np.stack([k, n, p, st], 1).astype(np.int64)
+    got = cdlm.scan_hits(bases, pwms, lens, thresh, both_strands=True)
+    flush("random hits: %d" % len(exp))
+    assert len(exp) > 0
+    np.testing.assert_array_equal(got, exp)
+
+
+def test_argmax_and_best_random_vs_oracle():
+    bases, pwms, lens, thresh = _random_case(6, 200, 300, 16)
+    np.testing.assert_array_equal(cdlm.scan_w_gpu(bases, pwms, lens, thresh),
+                                  O.pwm_gpu_argmax(bases, pwms, lens, thresh))
+    pos, comp, sc = cdlm.overlapping_scan(bases, pwms, lens, thresh)
+    opos, ocomp, osc = O.pwm_best(bases, pwms, lens, thresh)
+    np.testing.assert_array_equal(pos, opos)
+    np.testing.assert_array_equal(comp.astype(np.int8), ocomp)
+    np.testing.assert_array_equal(sc.view(np.uint32), osc.view(np.uint32))
+
+
+def test_edge_cases():
+    rng = np.random.default_rng(9)
+    pwms = np.zeros((2, 4, 12), np.float32)
+    pwms[0, :, :12] = rng.standard_normal((4, 12))
+    pwms[1, :, :3] = rng.standard_normal((4, 3))
+    lens = np.array([12, 3], np.int32)
+    thresh = np.array([-100.0, -100.0], np.float32)
+    # sequences shorter than the PWM: no window for k=0
+    bases = rng.integers(0, 4, (5, 8)).astype(np.uint8)
+    h = cdlm.scan_hits(bases, pwms, lens, thresh)
+    assert np.all(h[:, 0] == 1) and len(h) == 5 * 6 * 2
+    pos, comp, sc = cdlm.overlapping_scan(bases, pwms, lens, thresh)
+    assert np.all(pos[0] == 0)
+    np.testing.assert_array_equal(cdlm.scan_w_gpu(bases, pwms, lens, thresh), O.pwm_gpu_argmax(bases, pwms, lens, thresh))
+    # an unknown base (code 4, an all-zero one-hot column) contributes 0
+    bases2 = bases.copy(); bases2[:, 2] = 4
+    k, n, p, st, _ = O.pwm_hits(bases2, pwms, lens, thresh)
+    got = cdlm.scan_hits(bases2, pwms, lens, thresh)
+    np.testing.assert_array_equal(got, np.stack([k, n, p, st], 1).astype(np.int64))
+
+
+def test_onehot_to_bases_roundtrip():
+    rng = np.random.default_rng(3)
+    b = rng.integers(0, 4, (7, 33)).astype(np.uint8)
+    oh = np.zeros((7, 4 * 33), np.float32)
+    oh[np.arange(7)[:, None], 4 * np.arange(33)[None, :] + b] = 1
+    oh[2, 4 * 5:4 * 6] = 0
+    got = cdlm.onehot_to_bases(oh)
+    exp = b.copy(); exp[2, 5] = 4
+    np.testing.assert_array_equal(got, exp)
