@@ -12,7 +12,9 @@
 //
 // Sequences: base codes 0..3, one byte each, (N, L) row-major.
 
+#include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "cdlm_internal.h"
 
@@ -21,43 +23,120 @@ namespace {
 constexpr int kPwmThreads = 256;
 constexpr int kMaxPwmLen = 64;
 
-// ---------------------------------------------------------------------------- hit list
-// Block (n-tile of one sequence, k): one thread per window.  Records (k, n, pos, strand).
-__global__ __launch_bounds__(kPwmThreads) void k_pwm_hits(const uint8_t *__restrict__ bases, int L, long N,
-                                                          const float *__restrict__ pwms, const int *__restrict__ lens,
-                                                          const float *__restrict__ thresh, int K, int maxlen,
-                                                          int both, int *hk, long long *hn, int *hp, signed char *hs,
-                                                          long long cap, unsigned long long *count) {
-    __shared__ float pw[4 * kMaxPwmLen];
-    const long n = blockIdx.x;
-    const int k = blockIdx.y;
-    const int m = lens[k];
-    for (int i = threadIdx.x; i < 4 * maxlen; i += kPwmThreads) pw[i] = pwms[(size_t)k * 4 * maxlen + i];
-    __syncthreads();
-    const uint8_t *seq = bases + n * (long)L;
-    const float th = thresh[k];
-    for (int l0 = threadIdx.x; l0 + m <= L; l0 += kPwmThreads) {
-        float f = 0.f;
-        for (int j = 0; j < m; ++j) {
-            const int b = seq[l0 + j];
-            f = f + (b < 4 ? pw[b * maxlen + j] : 0.f);
-        }
-        if (f > th) {
-            unsigned long long slot = atomicAdd(count, 1ULL);
-            if ((long long)slot < cap) { hk[slot] = k; hn[slot] = n; hp[slot] = l0; hs[slot] = 0; }
-        }
-        if (both) {
-            float r = 0.f;
-            for (int j = 0; j < m; ++j) {
-                const int b = seq[l0 + j];
-                r = r + (b < 4 ? pw[(3 - b) * maxlen + (m - 1 - j)] : 0.f);
-            }
-            if (r > th) {
-                unsigned long long slot = atomicAdd(count, 1ULL);
-                if ((long long)slot < cap) { hk[slot] = k; hn[slot] = n; hp[slot] = l0; hs[slot] = 1; }
-            }
-        }
+// ---------------------------------------------------------------------------- hit list on MFMA
+// One-hot im2col x PWM contraction on v_mfma_f32_16x16x4_f32.  A tile of 16 windows x 16
+// strand-PWMs accumulates one motif column j per MFMA: A[i][a] = [base(w0+i+j) == a],
+// B[a][c] = pwm_c[a][j].  Each K-block holds exactly one non-zero product per output (the
+// base's row), so every MFMA step is the single FP32 addition acc + pwm[b][j] of the
+// position-major sequential sum (greedy_search!, scan.jl:231-234); columns past a PWM's
+// length are zero and add +0.
+// Operand layout (16x16x4, one block): lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// result register r of lane l is D[4*(l/16)+r][l%16].
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+struct PwmTiles {
+    int T;                 // tiles of 16 strand-PWMs, sorted by length
+    int groups;            // tile groups (G tiles per wave)
+    const float *B;        // per tile: [maxlen_t][64] lane-ordered B operands
+    const int *boff;       // per tile: float offset into B
+    const int *tlen;       // per tile: max length (steps)
+    const int *slot_k;     // per slot (T*16): PWM index, -1 for an empty slot
+    const int *slot_m;     // length
+    const int *slot_st;    // strand
+    const float *slot_th;  // threshold
+};
+
+template <int G, int MAXJ>
+__global__ __launch_bounds__(256) void k_pwm_hits_mfma(const uint8_t *__restrict__ bases, int L, long N, PwmTiles P,
+                                                       int *hk, long long *hn, int *hp, signed char *hs, long long cap,
+                                                       unsigned long long *count) {
+    extern __shared__ uint8_t seqs_lds[];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int Lpad = ((L + 15) & ~15) + MAXJ + 16;
+    uint8_t *sq = seqs_lds + w * Lpad;
+    const int g = blockIdx.x;
+    const int t0 = g * G;
+    float Bv[G][MAXJ];
+    int sk[G], sm[G], sst[G];
+    float sth[G];
+    int gmax = 0;
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+        const int t = t0 + gi;
+        const bool ok = t < P.T;
+        const int len = ok ? P.tlen[t] : 0;
+        gmax = len > gmax ? len : gmax;
+#pragma unroll
+        for (int j = 0; j < MAXJ; ++j) Bv[gi][j] = (ok && j < len) ? P.B[P.boff[t] + j * 64 + lane] : 0.f;
+        const int sl = t * 16 + (lane & 15);
+        sk[gi] = ok ? P.slot_k[sl] : -1;
+        sm[gi] = ok ? P.slot_m[sl] : 0;
+        sst[gi] = ok ? P.slot_st[sl] : 0;
+        sth[gi] = ok ? P.slot_th[sl] : 0.f;
     }
+    const int arow = lane & 15, abase = lane >> 4;
+    const long nstride = (long)gridDim.y * 4;
+    for (long n = (long)blockIdx.y * 4 + w; n < N; n += nstride) {
+        const uint8_t *src = bases + n * (long)L;
+        for (int i = lane; i < Lpad; i += 64) sq[i] = i < L ? src[i] : (uint8_t)4;
+        wave_lds_sync();
+        for (int w0 = 0; w0 < L; w0 += 16) {
+            floatx4 acc[G];
+#pragma unroll
+            for (int gi = 0; gi < G; ++gi) acc[gi] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int j = 0; j < MAXJ; ++j) {
+                if (j < gmax) {
+                    const float a = (sq[w0 + arow + j] == abase) ? 1.0f : 0.0f;
+#pragma unroll
+                    for (int gi = 0; gi < G; ++gi) acc[gi] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bv[gi][j], acc[gi], 0, 0, 0);
+                }
+            }
+            int myc = 0;
+            unsigned hitmask = 0;
+#pragma unroll
+            for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int l0 = w0 + 4 * abase + r;
+                    const bool h = sk[gi] >= 0 && l0 + sm[gi] <= L && acc[gi][r] > sth[gi];
+                    if (h) { hitmask |= 1u << (gi * 4 + r); ++myc; }
+                }
+            if (__ballot(myc > 0)) {
+                int incl = myc;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int t = __shfl_up(incl, o);
+                    if (lane >= o) incl += t;
+                }
+                const int total = __shfl(incl, 63);
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(count, (unsigned long long)total);
+                base = __shfl(base, 0);
+                unsigned long long slot = base + (unsigned long long)(incl - myc);
+#pragma unroll
+                for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (hitmask & (1u << (gi * 4 + r))) {
+                            if ((long long)slot < cap) {
+                                hk[slot] = sk[gi]; hn[slot] = n; hp[slot] = w0 + 4 * abase + r; hs[slot] = (signed char)sst[gi];
+                            }
+                            ++slot;
+                        }
+            }
+        }
+        wave_lds_sync();
+    }
+}
+
+// Lane layout and rounding probe of v_mfma_f32_16x16x4_f32: lane l supplies a[l] and b[l];
+// c holds the 4 accumulator inputs of each lane; d receives the 4 results of each lane.
+__global__ void k_mfma_probe(const float *a, const float *b, const float *c, float *d) {
+    const int l = threadIdx.x;
+    floatx4 acc = {c[4 * l], c[4 * l + 1], c[4 * l + 2], c[4 * l + 3]};
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[l], b[l], acc, 0, 0, 0);
+    for (int r = 0; r < 4; ++r) d[4 * l + r] = acc[r];
 }
 
 // ---------------------------------------------------------------------------- GPU argmax
@@ -213,21 +292,107 @@ void release(cdlm_ctx *ctx, DevPwm &d, bool own_bases) {
     if (d.lens) (void)hipFree(d.lens);
 }
 
-cdlm_status launch_hits(cdlm_ctx *ctx, const uint8_t *d_bases, int L, long N, const DevPwm &d, int K, int maxlen,
-                        int both, int *hk, long long *hn, int *hp, signed char *hs, long long cap,
-                        unsigned long long *d_count) {
+// Strand-PWM tiles for the MFMA kernel: (k, strand) pairs sorted by length (longest
+// first, then k, strand), 16 per tile.
+struct HostTiles {
+    std::vector<float> B;
+    std::vector<int> boff, tlen, sk, sm, sst;
+    std::vector<float> sth;
+    int T = 0;
+};
+
+void build_tiles(const float *pwms, const int *lens, const float *thresh, int K, int maxlen, int both, HostTiles &ht) {
+    struct SP { int k, st, m; };
+    std::vector<SP> v;
+    for (int k = 0; k < K; ++k)
+        for (int st = 0; st < (both ? 2 : 1); ++st) v.push_back({k, st, lens[k]});
+    std::stable_sort(v.begin(), v.end(), [](const SP &x, const SP &y) { return x.m > y.m; });
+    ht.T = (int)((v.size() + 15) / 16);
+    ht.sk.assign(ht.T * 16, -1); ht.sm.assign(ht.T * 16, 0); ht.sst.assign(ht.T * 16, 0); ht.sth.assign(ht.T * 16, 0.f);
+    for (int t = 0; t < ht.T; ++t) {
+        int tl = 0;
+        for (int c = 0; c < 16; ++c) {
+            const size_t i = (size_t)t * 16 + c;
+            if (i < v.size()) tl = v[i].m > tl ? v[i].m : tl;
+        }
+        ht.tlen.push_back(tl);
+        ht.boff.push_back((int)ht.B.size());
+        for (int j = 0; j < tl; ++j)
+            for (int lane = 0; lane < 64; ++lane) {
+                const int a = lane >> 4, c = lane & 15;
+                const size_t i = (size_t)t * 16 + c;
+                float val = 0.f;
+                if (i < v.size() && j < v[i].m) {
+                    const int k = v[i].k, m = v[i].m;
+                    const float *pw = pwms + (size_t)k * 4 * maxlen;
+                    val = v[i].st ? pw[(3 - a) * maxlen + (m - 1 - j)] : pw[a * maxlen + j];
+                }
+                ht.B.push_back(val);
+            }
+        for (int c = 0; c < 16; ++c) {
+            const size_t i = (size_t)t * 16 + c;
+            if (i >= v.size()) continue;
+            ht.sk[i] = v[i].k; ht.sm[i] = v[i].m; ht.sst[i] = v[i].st; ht.sth[i] = thresh[v[i].k];
+        }
+    }
+}
+
+struct DevTiles {
+    float *B = nullptr, *sth = nullptr;
+    int *boff = nullptr, *tlen = nullptr, *sk = nullptr, *sm = nullptr, *sst = nullptr;
+    void release() {
+        for (void *q : {(void *)B, (void *)sth, (void *)boff, (void *)tlen, (void *)sk, (void *)sm, (void *)sst})
+            if (q) (void)hipFree(q);
+    }
+};
+
+template <typename T>
+cdlm_status up(cdlm_ctx *ctx, T **dst, const std::vector<T> &src) {
+    CDLM_HIP(ctx, hipMalloc(dst, sizeof(T) * (src.size() ? src.size() : 1)));
+    if (!src.empty()) CDLM_HIP(ctx, hipMemcpyAsync(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, ctx->stream));
+    return CDLM_OK;
+}
+
+cdlm_status launch_hits(cdlm_ctx *ctx, const uint8_t *d_bases, int L, long N, const float *pwms, const int *lens,
+                        const float *thresh, int K, int maxlen, int both, int *hk, long long *hn, int *hp,
+                        signed char *hs, long long cap, unsigned long long *d_count) {
     hipStream_t st = ctx->stream;
     CDLM_HIP(ctx, hipMemsetAsync(d_count, 0, sizeof(unsigned long long), st));
-    if (N > 0) {
-        // grid.x * block.x must stay below 2^32 work-items
-        if ((unsigned long long)N * kPwmThreads >= (1ULL << 32))
-            return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "N=%ld too large for one launch", N);
-        hipLaunchKernelGGL(k_pwm_hits, dim3((unsigned)N, K), dim3(kPwmThreads), 0, st, d_bases, L, N, d.pwms, d.lens,
-                           d.thresh, K, maxlen, both, hk, hn, hp, hs, cap, d_count);
+    if (N == 0) return CDLM_OK;
+    HostTiles ht;
+    build_tiles(pwms, lens, thresh, K, maxlen, both, ht);
+    DevTiles dt;
+    cdlm_status s = CDLM_OK;
+    if ((s = up(ctx, &dt.B, ht.B)) == CDLM_OK && (s = up(ctx, &dt.boff, ht.boff)) == CDLM_OK &&
+        (s = up(ctx, &dt.tlen, ht.tlen)) == CDLM_OK && (s = up(ctx, &dt.sk, ht.sk)) == CDLM_OK &&
+        (s = up(ctx, &dt.sm, ht.sm)) == CDLM_OK && (s = up(ctx, &dt.sst, ht.sst)) == CDLM_OK &&
+        (s = up(ctx, &dt.sth, ht.sth)) == CDLM_OK) {
+        PwmTiles P;
+        P.T = ht.T; P.B = dt.B; P.boff = dt.boff; P.tlen = dt.tlen;
+        P.slot_k = dt.sk; P.slot_m = dt.sm; P.slot_st = dt.sst; P.slot_th = dt.sth;
+        const int G = maxlen <= 32 ? 4 : 2;
+        const int MAXJ = maxlen <= 32 ? 32 : 64;
+        P.groups = (ht.T + G - 1) / G;
+        const int Lpad = ((L + 15) & ~15) + MAXJ + 16;
+        const size_t lds = 4 * (size_t)Lpad;
+        long yb = (N + 3) / 4;
+        const long target = (long)ctx->num_cu * 16 / (P.groups > 0 ? P.groups : 1) + 1;
+        if (yb > target) yb = target;
+        if (yb > 65535) yb = 65535;
+        if (yb < 1) yb = 1;
+        if (lds > 64 * 1024) s = cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "sequence length %d too long for the scan", L);
+        else if (G == 4)
+            hipLaunchKernelGGL((k_pwm_hits_mfma<4, 32>), dim3(P.groups, (unsigned)yb), dim3(256), lds, st, d_bases, L, N,
+                               P, hk, hn, hp, hs, cap, d_count);
+        else
+            hipLaunchKernelGGL((k_pwm_hits_mfma<2, 64>), dim3(P.groups, (unsigned)yb), dim3(256), lds, st, d_bases, L, N,
+                               P, hk, hn, hp, hs, cap, d_count);
+        hipError_t e = hipGetLastError();
+        if (s == CDLM_OK && e != hipSuccess) s = cdlm_fail(ctx, CDLM_ERR_HIP, "k_pwm_hits_mfma: %s", hipGetErrorString(e));
     }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return cdlm_fail(ctx, CDLM_ERR_HIP, "k_pwm_hits: %s", hipGetErrorString(e));
-    return CDLM_OK;
+    (void)hipStreamSynchronize(st);
+    dt.release();
+    return s;
 }
 
 }  // namespace
@@ -262,7 +427,7 @@ extern "C" cdlm_status cdlm_pwm_hits(cdlm_ctx *ctx, const uint8_t *bases, int32_
         (void)hipGetLastError();
         s = cdlm_fail(ctx, CDLM_ERR_OOM, "no device memory for %lld hit records", (long long)cap);
     }
-    if (s == CDLM_OK) s = launch_hits(ctx, d.bases, L, N, d, K, maxlen, both_strands, hk, hn, hp, hs, cap, cnt);
+    if (s == CDLM_OK) s = launch_hits(ctx, d.bases, L, N, pwms, lens, thresh, K, maxlen, both_strands, hk, hn, hp, hs, cap, cnt);
     unsigned long long total = 0;
     if (s == CDLM_OK) {
         hipStream_t st = ctx->stream;
@@ -302,8 +467,8 @@ extern "C" cdlm_status cdlm_pwm_hits_device(cdlm_ctx *ctx, const uint8_t *d_base
         s = cdlm_fail(ctx, CDLM_ERR_OOM, "counter");
     }
     if (s == CDLM_OK)
-        s = launch_hits(ctx, d_bases, L, N, d, K, maxlen, both_strands, d_hit_k, (long long *)d_hit_n, d_hit_pos,
-                        (signed char *)d_hit_strand, cap, cnt);
+        s = launch_hits(ctx, d_bases, L, N, pwms, lens, thresh, K, maxlen, both_strands, d_hit_k, (long long *)d_hit_n,
+                        d_hit_pos, (signed char *)d_hit_strand, cap, cnt);
     unsigned long long total = 0;
     if (s == CDLM_OK && (hipMemcpyAsync(&total, cnt, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
                          hipStreamSynchronize(ctx->stream) != hipSuccess))
@@ -407,5 +572,26 @@ extern "C" cdlm_status cdlm_onehot_to_bases(cdlm_ctx *ctx, const float *onehot, 
     }
     if (d_oh) (void)hipFree(d_oh);
     if (d_b) (void)hipFree(d_b);
+    return s;
+}
+
+extern "C" cdlm_status cdlm_selftest_mfma(cdlm_ctx *ctx, const float *a64, const float *b64, const float *c256,
+                                          float *d256) {
+    if (!ctx || !a64 || !b64 || !c256 || !d256) return CDLM_ERR_ARG;
+    float *buf = nullptr;
+    CDLM_HIP(ctx, hipMalloc(&buf, sizeof(float) * (64 + 64 + 256 + 256)));
+    hipStream_t st = ctx->stream;
+    cdlm_status s = CDLM_OK;
+    if (hipMemcpyAsync(buf, a64, 256, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(buf + 64, b64, 256, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(buf + 128, c256, 1024, hipMemcpyHostToDevice, st) != hipSuccess)
+        s = cdlm_fail(ctx, CDLM_ERR_HIP, "probe upload");
+    if (s == CDLM_OK) {
+        hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, st, buf, buf + 64, buf + 128, buf + 384);
+        if (hipGetLastError() != hipSuccess || hipMemcpyAsync(d256, buf + 384, 1024, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            s = cdlm_fail(ctx, CDLM_ERR_HIP, "probe failed");
+    }
+    (void)hipFree(buf);
     return s;
 }

@@ -166,6 +166,11 @@ cdlm_status cdlm_pwm_best(cdlm_ctx *ctx, const uint8_t *bases, int32_t L, int64_
  * gets code 4 (never matches a PWM row).  Replaces the device side of data_2_dummy. */
 cdlm_status cdlm_onehot_to_bases(cdlm_ctx *ctx, const float *onehot, int32_t L4, int64_t N, uint8_t *bases);
 
+/* Self-test of the matrix-core operand layout and rounding the PWM hit-list kernel relies
+ * on: one v_mfma_f32_16x16x4_f32 with lane l supplying a64[l] and b64[l] and accumulator
+ * inputs c256[4l..4l+3]; d256 receives each lane's 4 results (no reference counterpart). */
+cdlm_status cdlm_selftest_mfma(cdlm_ctx *ctx, const float *a64, const float *b64, const float *c256, float *d256);
+
 /* ---- Device-resident variants (inputs already in HBM, for benchmarks and hosts that
  * keep data on the device).  Pointers are device pointers; outputs stay on the device
  * and the call returns after enqueueing unless noted. ------------------------------- */

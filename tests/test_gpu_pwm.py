@@ -99,3 +99,42 @@ def test_onehot_to_bases_roundtrip():
     got = cdlm.onehot_to_bases(oh)
     exp = b.copy(); exp[2, 5] = 4
     np.testing.assert_array_equal(got, exp)
+
+
+def test_mfma_layout_and_rounding_probe():
+    """The operand layout the hit-list kernel assumes for v_mfma_f32_16x16x4_f32, and that
+    an accumulation with one non-zero product per K-block rounds like one FP32 addition."""
+    import ctypes as C
+    from cdlm import _lib
+    ctx = cdlm.default_context()
+    rng = np.random.default_rng(11)
+    a = rng.standard_normal(64).astype(np.float32)
+    b = rng.standard_normal(64).astype(np.float32)
+    c = rng.standard_normal(256).astype(np.float32)
+    d = np.empty(256, np.float32)
+    _lib.check(_lib.load().cdlm_selftest_mfma(ctx.h, a.ctypes.data, b.ctypes.data, c.ctypes.data, d.ctypes.data), ctx.h)
+    A = np.zeros((16, 4)); B = np.zeros((4, 16)); Cm = np.zeros((16, 16))
+    for l in range(64):
+        A[l % 16, l // 16] = a[l]
+        B[l // 16, l % 16] = b[l]
+        for r in range(4):
+            Cm[4 * (l // 16) + r, l % 16] = c[4 * l + r]
+    ref = A @ B + Cm
+    got = np.zeros((16, 16))
+    for l in range(64):
+        for r in range(4):
+            got[4 * (l // 16) + r, l % 16] = d[4 * l + r]
+    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-5)
+    # one-hot A: exactly one non-zero product per output -> result must be fl32(c + p)
+    a1 = np.zeros(64, np.float32)
+    for l in range(64):
+        a1[l] = 1.0 if (l // 16) == (l % 16) % 4 else 0.0     # row i selects k = i % 4
+    b1 = (rng.standard_normal(64) * 3).astype(np.float32)
+    c1 = (rng.standard_normal(256) * 1e3).astype(np.float32)
+    _lib.check(_lib.load().cdlm_selftest_mfma(ctx.h, a1.ctypes.data, b1.ctypes.data, c1.ctypes.data, d.ctypes.data), ctx.h)
+    for l in range(64):
+        for r in range(4):
+            i, j = 4 * (l // 16) + r, l % 16
+            p = b1[(i % 4) * 16 + j]                  # B[k=i%4][j]
+            exp = np.float32(np.float32(c1[4 * l + r]) + p)
+            assert d[4 * l + r].view(np.uint32) == exp.view(np.uint32), (l, r)
