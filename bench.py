@@ -74,6 +74,9 @@ def main():
     ap.add_argument("--n", type=int, default=N_SEQ, help="sequences (default: the workload's 20000)")
     ap.add_argument("--cpu-sample", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--outer", type=int, default=ITER_NUM, help="outer iterations per step (profiling runs only)")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
+                    help="only run a plain device copy of GIB GiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path; "
@@ -83,6 +86,17 @@ def main():
     import numpy as np
     import cdlm
     from cdlm import synth
+
+    if args.calibrate_copy > 0:
+        import ctypes as C
+        from cdlm import _lib
+        ctx = cdlm.default_context()
+        nbytes = int(args.calibrate_copy * (1 << 30))
+        ms = C.c_double()
+        _lib.check(_lib.load().cdlm_copy_bench(ctx.h, nbytes, 5, C.byref(ms)), ctx.h)
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "ms_per_copy": ms.value,
+                          "GBps": 2 * nbytes / (ms.value * 1e-3) / 1e9, "reps": 5, "warmup_launches": 1}))
+        return
 
     N = args.n
     Lg = 4 * L_BASES + 2 * FLEN
@@ -96,7 +110,7 @@ def main():
 
     def step():
         c.reset()
-        c.optimize(ITER_NUM, CSC_ITER, CDL_ITER)
+        c.optimize(args.outer, CSC_ITER, CDL_ITER)
         return c.code_thresh(1.0)
 
     for i in range(args.warmup):
@@ -114,7 +128,7 @@ def main():
     red_ms, red_n = c.timing_read(3)
     c.timing(False)
     ms_step = 1000.0 * dt / args.steps
-    iters = ITER_NUM * CSC_ITER
+    iters = args.outer * CSC_ITER
     value = N * iters / (dt / args.steps)
     avg_csc = csc_ms / max(csc_n, 1)
     achieved = csc_bytes(N, Lg, M_FILT) / (avg_csc * 1e-3) / 1e9

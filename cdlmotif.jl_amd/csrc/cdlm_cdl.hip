@@ -69,14 +69,14 @@ __device__ __forceinline__ void st4(float *p, int s, float4 v) { reinterpret_cas
 __device__ __forceinline__ float f4(const float4 &v, int q) { return q == 0 ? v.x : q == 1 ? v.y : q == 2 ? v.z : v.w; }
 __device__ __forceinline__ float4 mk4(const float *a) { return make_float4(a[0], a[1], a[2], a[3]); }
 
-// X on the 4 samples of slot s: Xt + sum_j g[j] * h[l + j] (circular; hb holds h plus a
-// wrapped tail of TAPS+4 samples).
+// X on the 4 samples of slot s: Xt + sum_j g[j] * h[l + j] (circular; hb4 holds h plus a
+// wrapped tail of TAPS+4 samples, as float4).
 template <int TAPS>
-__device__ __forceinline__ void recon4(const float4 xt, const float *hb, int s, const float *g, float *X) {
+__device__ __forceinline__ void recon4(const float4 xt, const float4 *hb4, int s, const float *g, float *X) {
     float hw[TAPS + 4];
 #pragma unroll
     for (int u = 0; u <= TAPS / 4; ++u) {
-        float4 t = reinterpret_cast<const float4 *>(hb)[s + u];
+        float4 t = hb4[s + u];
         hw[4 * u] = t.x; hw[4 * u + 1] = t.y; hw[4 * u + 2] = t.z; hw[4 * u + 3] = t.w;
     }
 #pragma unroll
@@ -88,17 +88,26 @@ __device__ __forceinline__ void recon4(const float4 xt, const float *hb, int s, 
     }
 }
 
-// z[4s..4s+3] += sum_j g[j] * v[(l - j) mod Lg]; vb holds v at offset TAPS with the
-// wrapped head of TAPS samples in front.
+// X on the first sample of slot s only (the stride-4 support of Y3).
 template <int TAPS>
-__device__ __forceinline__ void conv_acc4(const float *vb, int s, const float *g, float *zacc) {
+__device__ __forceinline__ float recon1(float xt0, const float *hb, int s, const float *g) {
+    float acc = xt0;
+#pragma unroll
+    for (int j = 0; j < TAPS; ++j) acc = fmaf(g[j], hb[4 * s + j], acc);
+    return acc;
+}
+
+// z[4s..4s+3] += sum_j g[j] * v[(l - j) mod Lg]; vb4 holds v at float4 offset TAPS/4 with
+// the wrapped head of TAPS samples in front.
+template <int TAPS>
+__device__ __forceinline__ void conv_acc4(const float4 *vb4, int s, const float *g, float4 *zacc4) {
     float vw[TAPS + 4];
 #pragma unroll
     for (int u = 0; u <= TAPS / 4; ++u) {
-        float4 t = reinterpret_cast<const float4 *>(vb)[s + u];
+        float4 t = vb4[s + u];
         vw[4 * u] = t.x; vw[4 * u + 1] = t.y; vw[4 * u + 2] = t.z; vw[4 * u + 3] = t.w;
     }
-    float4 z = reinterpret_cast<float4 *>(zacc)[s];
+    float4 z = zacc4[s];
     float o[4] = {z.x, z.y, z.z, z.w};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -107,7 +116,7 @@ __device__ __forceinline__ void conv_acc4(const float *vb, int s, const float *g
         for (int j = 0; j < TAPS; ++j) acc = fmaf(g[j], vw[TAPS + q - j], acc);
         o[q] = acc;
     }
-    reinterpret_cast<float4 *>(zacc)[s] = mk4(o);
+    zacc4[s] = mk4(o);
 }
 
 template <int TAPS>
@@ -144,8 +153,8 @@ __global__ __launch_bounds__(kThreads) void k_init(CdlDev P, const float *__rest
     float *zbase = vbase + kWaves * vbLen;
     for (int i = tid; i < kWaves * Lg; i += kThreads) zbase[i] = 0.f;
     __syncthreads();
-    float *vb = vbase + w * vbLen;
-    float *zacc = zbase + w * Lg;
+    float4 *vb4 = smem4 + (w * vbLen) / 4;
+    float4 *zacc4 = smem4 + (kWaves * vbLen + w * Lg) / 4;
     const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int m = w; m < M; m += kWaves) {
         const int mm = __builtin_amdgcn_readfirstlane(m);
@@ -163,13 +172,12 @@ __global__ __launch_bounds__(kThreads) void k_init(CdlDev P, const float *__rest
             st4(P.g1 + col, s, zero4);
             st4(P.g3 + col, s, zero4);
             st4(P.d + col, s, ld4(P.G1cur + (size_t)mm * Lg, s));
-            reinterpret_cast<float4 *>(vb)[TAPS / 4 + s] = xp;
+            vb4[TAPS / 4 + s] = xp;
         }
         wave_lds_sync();
-        if (lane < TAPS / 4)
-            reinterpret_cast<float4 *>(vb)[TAPS / 4 - 1 - lane] = reinterpret_cast<float4 *>(vb)[TAPS / 4 + nslot - 1 - lane];
+        if (lane < TAPS / 4) vb4[TAPS / 4 - 1 - lane] = vb4[TAPS / 4 + nslot - 1 - lane];
         wave_lds_sync();
-        for (int s = lane; s < nslot; s += 64) conv_acc4<TAPS>(vb, s, g, zacc);
+        for (int s = lane; s < nslot; s += 64) conv_acc4<TAPS>(vb4, s, g, zacc4);
         wave_lds_sync();
     }
     __syncthreads();
@@ -189,65 +197,96 @@ __global__ __launch_bounds__(kThreads) void k_init(CdlDev P, const float *__rest
 }
 
 // ---------------------------------------------------------------------------- CSC
-// One iteration of code_update! (train.jl:5-43) for sequence n = blockIdx.x.
-template <int TAPS>
-__global__ __launch_bounds__(kThreads) void k_csc(CdlDev P) {
+// One iteration of code_update! (train.jl:5-43) for sequence n = blockIdx.x.  Each wave owns
+// the columns m = w, w+4, ...; a lane owns the float4 slots s = lane + 64 i (i < NSL) of a
+// column.  Column m+4's loads are issued before column m is processed (two register sets),
+// so a wave always has a column of loads in flight while it computes.
+// LDS (float4 units): twiddles (Lg/2) | h with wrapped tail | 4 x v column with wrapped
+// head | 4 x z partial; the FFT ping-pong buffers reuse the v columns after the loop.
+template <int TAPS, int NSL>
+__global__ __launch_bounds__(kThreads, (NSL <= 4 ? 3 : 2)) void k_csc(CdlDev P) {
     extern __shared__ float4 smem4[];
-    float *sm = reinterpret_cast<float *>(smem4);
     const int n = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int Lg = P.Lg, M = P.M, nslot = Lg >> 2;
-    const int hbLen = Lg + TAPS + 4, vbLen = Lg + TAPS;
-    float *hb = sm;
-    float *vbase = hb + hbLen;
-    float *zbase = vbase + kWaves * vbLen;
-    float2 *fa = reinterpret_cast<float2 *>(zbase + kWaves * Lg);
+    const int hb4Len = (Lg + TAPS + 4) / 4, vb4Len = (Lg + TAPS) / 4;
+    float2 *tw = reinterpret_cast<float2 *>(smem4);
+    float4 *hb4 = smem4 + Lg / 2;
+    float4 *vbase4 = hb4 + hb4Len;
+    float4 *zbase4 = vbase4 + kWaves * vb4Len;
+    float *hb = reinterpret_cast<float *>(hb4);
+    float *zbase = reinterpret_cast<float *>(zbase4);
+    float2 *fa = reinterpret_cast<float2 *>(vbase4);
     float2 *fb = fa + Lg;
     const size_t ro = (size_t)n * Lg;
 
-    for (int l = tid; l < Lg; l += kThreads) hb[l] = P.h[ro + l];
+    for (int l = tid; l < Lg; l += kThreads) { tw[l] = P.tw[l]; hb[l] = P.h[ro + l]; }
     for (int t = tid; t < TAPS + 4; t += kThreads) hb[Lg + t] = P.h[ro + t];
     for (int i = tid; i < kWaves * Lg; i += kThreads) zbase[i] = 0.f;
     __syncthreads();
 
     const float lam = P.lambda;
     const float inv_c0 = 1.0f / P.sh_csc;
-    float *vb = vbase + w * vbLen;
-    float *zacc = zbase + w * Lg;
+    float4 *vb4 = vbase4 + w * vb4Len;
+    float4 *zacc4 = zbase4 + w * (Lg / 4);
+    float4 nx[NSL], n2[NSL], n3[NSL];
+    if (w < M) {
+        const size_t col = ((size_t)n * M + w) * Lg;
+#pragma unroll
+        for (int i = 0; i < NSL; ++i) {
+            const int s = lane + 64 * i;
+            if (s < nslot) { nx[i] = ld4(P.Xt + col, s); n2[i] = ld4(P.y2 + col, s); n3[i] = ld4(P.y3 + col, s); }
+        }
+    }
     for (int m = w; m < M; m += kWaves) {
         const int mm = __builtin_amdgcn_readfirstlane(m);
         const size_t col = ((size_t)n * M + mm) * Lg;
+        float4 xt[NSL], a2[NSL], a3[NSL];
+#pragma unroll
+        for (int i = 0; i < NSL; ++i) { xt[i] = nx[i]; a2[i] = n2[i]; a3[i] = n3[i]; }
+        if (mm + kWaves < M) {
+            const size_t coln = ((size_t)n * M + mm + kWaves) * Lg;
+#pragma unroll
+            for (int i = 0; i < NSL; ++i) {
+                const int s = lane + 64 * i;
+                if (s < nslot) { nx[i] = ld4(P.Xt + coln, s); n2[i] = ld4(P.y2 + coln, s); n3[i] = ld4(P.y3 + coln, s); }
+            }
+        }
         float gr[TAPS], gc[TAPS];
         load_taps<TAPS>(P.G1rec, mm, Lg, gr);
         load_taps<TAPS>(P.G1cur, mm, Lg, gc);
-        for (int s = lane; s < nslot; s += 64) {
-            const float4 xt = ld4(P.Xt + col, s);
-            const float4 a2 = ld4(P.y2 + col, s);
-            const float4 a3 = ld4(P.y3 + col, s);
-            float X[4];
-            recon4<TAPS>(xt, hb, s, gr, X);
-            float y2n[4], y3n[4], v[4], xn[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float x = X[q];
-                const float t = x + f4(a2, q);                 // shrink! (csc_kernels.jl:58-59)
-                const float at = fabsf(t) - lam;
-                const float Y2 = at > 0.f ? copysignf(at, t) : 0.f;
-                const float Y3 = (q == 0) ? x + f4(a3, q) : 0.f;  // projectionX! (:73-77)
-                y2n[q] = f4(a2, q) + x - Y2;                    // train.jl:13
-                y3n[q] = f4(a3, q) + x - Y3;                    // train.jl:14
-                v[q] = (Y2 - y2n[q]) + (Y3 - y3n[q]);
-                xn[q] = v[q] * inv_c0;
+        for (int i = 0; i < NSL; ++i) {
+            const int s = lane + 64 * i;
+            if (s < nslot) {
+                float X[4];
+                recon4<TAPS>(xt[i], hb4, s, gr, X);
+                float y2n[4], y3n[4], v[4], xn[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float x = X[q];
+                    const float t = x + f4(a2[i], q);                   // shrink! (csc_kernels.jl:58-59)
+                    const float at = fabsf(t) - lam;
+                    const float Y2 = at > 0.f ? copysignf(at, t) : 0.f;
+                    const float Y3 = (q == 0) ? x + f4(a3[i], q) : 0.f;  // projectionX! (:73-77)
+                    y2n[q] = f4(a2[i], q) + x - Y2;                       // train.jl:13
+                    y3n[q] = f4(a3[i], q) + x - Y3;                       // train.jl:14
+                    v[q] = (Y2 - y2n[q]) + (Y3 - y3n[q]);
+                    xn[q] = v[q] * inv_c0;
+                }
+                st4(P.y2 + col, s, mk4(y2n));
+                st4(P.y3 + col, s, mk4(y3n));
+                st4(P.Xt + col, s, mk4(xn));
+                vb4[TAPS / 4 + s] = mk4(v);
             }
-            st4(P.y2 + col, s, mk4(y2n));
-            st4(P.y3 + col, s, mk4(y3n));
-            st4(P.Xt + col, s, mk4(xn));
-            reinterpret_cast<float4 *>(vb)[TAPS / 4 + s] = mk4(v);
         }
         wave_lds_sync();
-        if (lane < TAPS / 4)
-            reinterpret_cast<float4 *>(vb)[TAPS / 4 - 1 - lane] = reinterpret_cast<float4 *>(vb)[TAPS / 4 + nslot - 1 - lane];
+        if (lane < TAPS / 4) vb4[TAPS / 4 - 1 - lane] = vb4[TAPS / 4 + nslot - 1 - lane];
         wave_lds_sync();
-        for (int s = lane; s < nslot; s += 64) conv_acc4<TAPS>(vb, s, gc, zacc);
+#pragma unroll
+        for (int i = 0; i < NSL; ++i) {
+            const int s = lane + 64 * i;
+            if (s < nslot) conv_acc4<TAPS>(vb4, s, gc, zacc4);
+        }
         wave_lds_sync();
     }
     __syncthreads();
@@ -263,7 +302,7 @@ __global__ __launch_bounds__(kThreads) void k_csc(CdlDev P) {
         fa[l] = make_float2(z, Y1o - y1n);
     }
     __syncthreads();
-    float2 *C = team_fft<false>(fa, fb, P.plan, P.tw, tid, kThreads, BlockSync{});
+    float2 *C = team_fft<false>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{});
     float2 *O = (C == fa) ? fb : fa;
     const float c0 = P.sh_csc;
     for (int k = tid; k < Lg; k += kThreads) {
@@ -277,7 +316,7 @@ __global__ __launch_bounds__(kThreads) void k_csc(CdlDev P) {
         O[k] = make_float2(hh.x - dx.y, hh.y + dx.x);   // h_hat + i DX_hat
     }
     __syncthreads();
-    float2 *R = team_fft<true>(O, C, P.plan, P.tw, tid, kThreads, BlockSync{});
+    float2 *R = team_fft<true>(O, C, P.plan, tw, tid, kThreads, BlockSync{});
     const float scale = 1.0f / (float)Lg;
     const float rho = P.rho;
     for (int l = tid; l < Lg; l += kThreads) {
@@ -322,7 +361,7 @@ __global__ __launch_bounds__(kThreads) void k_best(CdlDev P, float *xs0, float *
         float abf = 0.f;
         for (int s = lane; s < nslot; s += 64) {
             float X[4];
-            recon4<TAPS>(ld4(P.Xt + col, s), hb, s, gr, X);
+            recon4<TAPS>(ld4(P.Xt + col, s), reinterpret_cast<const float4 *>(hb), s, gr, X);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 abf += fabsf(X[q]);
@@ -400,65 +439,77 @@ __global__ __launch_bounds__(kThreads) void k_decide(CdlScal *sc, float lambda, 
 
 // ---------------------------------------------------------------------------- dict
 // One iteration of dict_update! (train.jl:50-72) for sequence n; the means over n
-// (:76,:83) follow in k_dict_reduce / k_dict_finalize.
+// (:76,:83) follow in k_dict_reduce / k_dict_finalize.  Columns go through the FFTs B at a
+// time (batched team transforms); accumulations over m stay in ascending m.
+// LDS: twiddles | h with wrapped tail | taps of the batch | fa, fb (B x Lg) | ya, yb
+// (B x Lg/4) | accS, accT, accP (Lg/2+1 bins).
 template <int TAPS>
-__global__ __launch_bounds__(kThreads) void k_dict(CdlDev P) {
+__global__ __launch_bounds__(kThreads) void k_dict(CdlDev P, int B) {
     extern __shared__ float4 smem4[];
     float *sm = reinterpret_cast<float *>(smem4);
     const int n = blockIdx.x, tid = threadIdx.x;
     const int Lg = P.Lg, M = P.M, nslot = Lg >> 2, L4q = Lg >> 2;
     const int H = Lg / 2 + 1, Hp = (H + 3) & ~3;
     const int hbLen = Lg + TAPS + 4;
-    float *hb = sm;
-    float2 *fa = reinterpret_cast<float2 *>(hb + hbLen);
-    float2 *fb = fa + Lg;
-    float2 *ya = fb + Lg;
-    float2 *yb = ya + L4q;
-    float *accS = reinterpret_cast<float *>(yb + L4q);
+    float2 *tw = reinterpret_cast<float2 *>(sm);
+    float *hb = sm + 2 * Lg;
+    float *gtap = hb + hbLen;                       // B * TAPS (rounded to 4)
+    float2 *fa = reinterpret_cast<float2 *>(gtap + ((B * TAPS + 3) & ~3));
+    float2 *fb = fa + B * Lg;
+    float2 *ya = fb + B * Lg;
+    float2 *yb = ya + B * L4q;
+    float *accS = reinterpret_cast<float *>(yb + B * L4q);
     float2 *accT = reinterpret_cast<float2 *>(accS + Hp);
     float2 *accP = accT + Hp;
-    float *qb = reinterpret_cast<float *>(accP + Hp);
-    float *g1b = qb + Lg;
-    float *g3b = g1b + Lg;
     const size_t ro = (size_t)n * Lg;
+    const float rslot = 1.0f / (float)nslot, rLg = 1.0f / (float)Lg, rL4q = 1.0f / (float)L4q;
 
-    for (int l = tid; l < Lg; l += kThreads) hb[l] = P.h[ro + l];
+    for (int l = tid; l < Lg; l += kThreads) { tw[l] = P.tw[l]; hb[l] = P.h[ro + l]; }
     for (int t = tid; t < TAPS + 4; t += kThreads) hb[Lg + t] = P.h[ro + t];
     for (int k = tid; k < Hp; k += kThreads) { accS[k] = 0.f; accT[k] = make_float2(0.f, 0.f); accP[k] = make_float2(0.f, 0.f); }
-    __syncthreads();
 
     // ---- pass 1: spectra sums over m
-    for (int m = 0; m < M; ++m) {
-        const size_t col = ((size_t)n * M + m) * Lg;
-        float gr[TAPS];
-        load_taps<TAPS>(P.G1rec, m, Lg, gr);
-        for (int s = tid; s < nslot; s += kThreads) {
-            float X[4];
-            recon4<TAPS>(ld4(P.Xt + col, s), hb, s, gr, X);
+    for (int m0 = 0; m0 < M; m0 += B) {
+        const int nb = (M - m0) < B ? (M - m0) : B;
+        for (int i = tid; i < nb * TAPS; i += kThreads) gtap[i] = P.G1rec[(size_t)(m0 + i / TAPS) * Lg + i % TAPS];
+        __syncthreads();
+        for (int idx = tid; idx < nb * nslot; idx += kThreads) {
+            int s;
+            const int bb = fdivmod(idx, nslot, rslot, s), m = m0 + bb;
+            const size_t col = ((size_t)n * M + m) * Lg;
+            const float4 xt = ld4(P.Xt + col, s);
             const float4 a3 = ld4(P.y3 + col, s);
             const float4 dv = ld4(P.d + col, s);
             const float4 a1 = ld4(P.g1 + col, s);
             const float4 c3 = ld4(P.g3 + col, s);
             const float4 G1v = ld4(P.G1cur + (size_t)m * Lg, s);
             const float4 G3v = ld4(P.G3 + (size_t)m * Lg, s);
-            ya[s] = make_float2(X[0] + a3.x, 0.f);    // Y3 on the stride-4 support
+            const float X0 = recon1<TAPS>(xt.x, hb, s, gtap + bb * TAPS);
+            ya[bb * L4q + s] = make_float2(X0 + a3.x, 0.f);    // Y3 on the stride-4 support
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const float qq = (f4(G1v, q) - f4(a1, q)) + (f4(G3v, q) - f4(c3, q));
-                fa[4 * s + q] = make_float2(f4(dv, q), qq);
+                fa[bb * Lg + 4 * s + q] = make_float2(f4(dv, q), qq);
             }
         }
         __syncthreads();
-        float2 *Y = team_fft<false>(ya, yb, P.plan4, P.tw, tid, kThreads, BlockSync{});
-        float2 *C = team_fft<false>(fa, fb, P.plan, P.tw, tid, kThreads, BlockSync{});
+        float2 *Y = team_fft<false>(ya, yb, P.plan4, tw, tid, kThreads, BlockSync{}, nb);
+        float2 *C = team_fft<false>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{}, nb);
         for (int k = tid; k < H; k += kThreads) {
-            const float2 Ck = C[k], Cm = C[k == 0 ? 0 : Lg - k];
-            const float2 Dh = make_float2(0.5f * (Ck.x + Cm.x), 0.5f * (Ck.y - Cm.y));
-            const float2 Qh = make_float2(0.5f * (Ck.y + Cm.y), -0.5f * (Ck.x - Cm.x));
-            const float2 Yk = Y[k % L4q];
-            accS[k] += Yk.x * Yk.x + Yk.y * Yk.y;
-            accT[k] = cadd(accT[k], cmul(Yk, Dh));
-            accP[k] = cadd(accP[k], cmul(Yk, Qh));
+            int kq;
+            (void)fdivmod(k, L4q, rL4q, kq);
+            float aS = accS[k];
+            float2 aT = accT[k], aP = accP[k];
+            for (int bb = 0; bb < nb; ++bb) {
+                const float2 Ck = C[bb * Lg + k], Cm = C[bb * Lg + (k == 0 ? 0 : Lg - k)];
+                const float2 Dh = make_float2(0.5f * (Ck.x + Cm.x), 0.5f * (Ck.y - Cm.y));
+                const float2 Qh = make_float2(0.5f * (Ck.y + Cm.y), -0.5f * (Ck.x - Cm.x));
+                const float2 Yk = Y[bb * L4q + kq];
+                aS += Yk.x * Yk.x + Yk.y * Yk.y;
+                aT = cadd(aT, cmul(Yk, Dh));
+                aP = cadd(aP, cmul(Yk, Qh));
+            }
+            accS[k] = aS; accT[k] = aT; accP[k] = aP;
         }
         __syncthreads();
     }
@@ -466,16 +517,17 @@ __global__ __launch_bounds__(kThreads) void k_dict(CdlDev P) {
     // ---- per-sequence spectral step: Td, G2/g2 (update_g2!, train.jl:68-71), kappa
     for (int k = tid; k < Lg; k += kThreads) fa[k] = (k < H) ? accT[k] : cconj(accT[Lg - k]);
     __syncthreads();
+    float *tdb = reinterpret_cast<float *>(ya);      // Td (Lg floats fit in ya+yb: B*Lg/2 complex >= Lg floats)
     {
-        float2 *R = team_fft<true>(fa, fb, P.plan, P.tw, tid, kThreads, BlockSync{});
+        float2 *R = team_fft<true>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{}, 1);
         const float scale = 1.0f / (float)Lg;
-        for (int l = tid; l < Lg; l += kThreads) qb[l] = R[l].x * scale;   // Td
+        for (int l = tid; l < Lg; l += kThreads) tdb[l] = R[l].x * scale;
     }
     __syncthreads();
     for (int l = tid; l < Lg; l += kThreads) fa[l] = make_float2(P.G2[ro + l] - P.g2[ro + l], 0.f);
     __syncthreads();
     {
-        float2 *R = team_fft<false>(fa, fb, P.plan, P.tw, tid, kThreads, BlockSync{});
+        float2 *R = team_fft<false>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{}, 1);
         const float c1 = P.sh_cdl;
         for (int k = tid; k < H; k += kThreads) {
             const float2 Gk = R[k];
@@ -484,7 +536,7 @@ __global__ __launch_bounds__(kThreads) void k_dict(CdlDev P) {
         }
     }
     for (int l = tid; l < Lg; l += kThreads) {
-        const float Td = qb[l];
+        const float Td = tdb[l];
         const bool win = (l >= P.gamma) && (l < P.gamma + P.L4);
         const float b = win ? P.s[(size_t)n * P.L4 + (l - P.gamma)] : 0.f;
         const float wl = win ? 1.f : 0.f;
@@ -498,42 +550,41 @@ __global__ __launch_bounds__(kThreads) void k_dict(CdlDev P) {
 
     // ---- pass 2: d_m = q_m + F^-1[conj(Y3h_m) kappa]; g1 += d - G1; g3 += d - G3
     const float scale = 1.0f / (float)Lg;
-    for (int m = 0; m < M; ++m) {
-        const size_t col = ((size_t)n * M + m) * Lg;
-        float gr[TAPS];
-        load_taps<TAPS>(P.G1rec, m, Lg, gr);
-        for (int s = tid; s < nslot; s += kThreads) {
-            float X[4];
-            recon4<TAPS>(ld4(P.Xt + col, s), hb, s, gr, X);
-            const float4 a3 = ld4(P.y3 + col, s);
+    for (int m0 = 0; m0 < M; m0 += B) {
+        const int nb = (M - m0) < B ? (M - m0) : B;
+        for (int i = tid; i < nb * TAPS; i += kThreads) gtap[i] = P.G1rec[(size_t)(m0 + i / TAPS) * Lg + i % TAPS];
+        __syncthreads();
+        for (int idx = tid; idx < nb * nslot; idx += kThreads) {
+            int s;
+            const int bb = fdivmod(idx, nslot, rslot, s), m = m0 + bb;
+            const size_t col = ((size_t)n * M + m) * Lg;
+            const float X0 = recon1<TAPS>(P.Xt[col + 4 * s], hb, s, gtap + bb * TAPS);
+            ya[bb * L4q + s] = make_float2(X0 + P.y3[col + 4 * s], 0.f);
+        }
+        __syncthreads();
+        float2 *Y = team_fft<false>(ya, yb, P.plan4, tw, tid, kThreads, BlockSync{}, nb);
+        for (int idx = tid; idx < nb * Lg; idx += kThreads) {
+            int k, kq;
+            const int bb = fdivmod(idx, Lg, rLg, k);
+            (void)fdivmod(k, L4q, rL4q, kq);
+            const float2 kap = (k < H) ? accP[k] : cconj(accP[Lg - k]);
+            fa[idx] = cmul(cconj(Y[bb * L4q + kq]), kap);
+        }
+        __syncthreads();
+        float2 *R = team_fft<true>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{}, nb);
+        for (int idx = tid; idx < nb * nslot; idx += kThreads) {
+            int s;
+            const int bb = fdivmod(idx, nslot, rslot, s), m = m0 + bb;
+            const size_t col = ((size_t)n * M + m) * Lg;
+            const float4 G1v = ld4(P.G1cur + (size_t)m * Lg, s);
+            const float4 G3v = ld4(P.G3 + (size_t)m * Lg, s);
             const float4 a1 = ld4(P.g1 + col, s);
             const float4 c3 = ld4(P.g3 + col, s);
-            const float4 G1v = ld4(P.G1cur + (size_t)m * Lg, s);
-            const float4 G3v = ld4(P.G3 + (size_t)m * Lg, s);
-            ya[s] = make_float2(X[0] + a3.x, 0.f);
-            float qv[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) qv[q] = (f4(G1v, q) - f4(a1, q)) + (f4(G3v, q) - f4(c3, q));
-            st4(qb, s, mk4(qv));
-            st4(g1b, s, a1);
-            st4(g3b, s, c3);
-        }
-        __syncthreads();
-        float2 *Y = team_fft<false>(ya, yb, P.plan4, P.tw, tid, kThreads, BlockSync{});
-        for (int k = tid; k < Lg; k += kThreads) {
-            const float2 kap = (k < H) ? accP[k] : cconj(accP[Lg - k]);
-            fa[k] = cmul(cconj(Y[k % L4q]), kap);
-        }
-        __syncthreads();
-        float2 *R = team_fft<true>(fa, fb, P.plan, P.tw, tid, kThreads, BlockSync{});
-        for (int s = tid; s < nslot; s += kThreads) {
-            const float4 G1v = ld4(P.G1cur + (size_t)m * Lg, s);
-            const float4 G3v = ld4(P.G3 + (size_t)m * Lg, s);
-            const float4 qv = ld4(qb, s), a1 = ld4(g1b, s), c3 = ld4(g3b, s);
             float dn[4], g1n[4], g3n[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                dn[q] = f4(qv, q) + R[4 * s + q].x * scale;
+                const float qv = (f4(G1v, q) - f4(a1, q)) + (f4(G3v, q) - f4(c3, q));
+                dn[q] = qv + R[bb * Lg + 4 * s + q].x * scale;
                 g1n[q] = f4(a1, q) + dn[q] - f4(G1v, q);       // train.jl:70
                 g3n[q] = f4(c3, q) + dn[q] - f4(G3v, q);       // train.jl:72
             }
@@ -685,6 +736,8 @@ struct cdlm_problem {
     CdlScal *sc = nullptr;
     FftPlan plan, plan4;
     size_t lds_init = 0, lds_csc = 0, lds_best = 0, lds_dict = 0, lds_fin = 0, lds_g1 = 0;
+    int dict_batch = 1;
+    int nsl = 4;
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[4];
 };
@@ -726,13 +779,14 @@ CdlDev dev_view(const cdlm_problem *p) {
 }
 
 size_t lds_bytes_csc(int Lg, int taps) {
-    return sizeof(float) * ((size_t)(Lg + taps + 4) + kWaves * (size_t)(Lg + taps) + kWaves * (size_t)Lg + 4 * (size_t)Lg);
+    return sizeof(float) * (2 * (size_t)Lg + (size_t)(Lg + taps + 4) + kWaves * (size_t)(Lg + taps) + kWaves * (size_t)Lg);
 }
 size_t lds_bytes_init(int Lg, int taps) { return sizeof(float) * (kWaves * (size_t)(Lg + taps) + kWaves * (size_t)Lg); }
 size_t lds_bytes_best(int Lg, int taps) { return sizeof(float) * (size_t)(Lg + taps + 4); }
-size_t lds_bytes_dict(int Lg, int taps) {
+size_t lds_bytes_dict(int Lg, int taps, int B) {
     const int H = Lg / 2 + 1, Hp = (H + 3) & ~3;
-    return sizeof(float) * ((size_t)(Lg + taps + 4) + 4 * (size_t)Lg + (size_t)Lg + 5 * (size_t)Hp + 3 * (size_t)Lg);
+    return sizeof(float) * (2 * (size_t)Lg + (size_t)(Lg + taps + 4) + (size_t)((B * taps + 3) & ~3) + 4 * (size_t)B * Lg +
+                            (size_t)B * Lg + 5 * (size_t)Hp);
 }
 size_t lds_bytes_fin(int Lg) { return sizeof(float) * (((size_t)Lg + 3) / 4 * 4 + 4 * (size_t)Lg); }
 constexpr size_t kLdsLimit = 160 * 1024;
@@ -743,7 +797,8 @@ cdlm_status set_lds_limits(cdlm_problem *p) {
 #define SETA(T)                                                                                            \
     do {                                                                                                   \
         CDLM_HIP(p->ctx, hipFuncSetAttribute((const void *)k_init<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_init)); \
-        CDLM_HIP(p->ctx, hipFuncSetAttribute((const void *)k_csc<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_csc));  \
+        CDLM_HIP(p->ctx, hipFuncSetAttribute((const void *)k_csc<T, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_csc));  \
+        CDLM_HIP(p->ctx, hipFuncSetAttribute((const void *)k_csc<T, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_csc));  \
         CDLM_HIP(p->ctx, hipFuncSetAttribute((const void *)k_best<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_best)); \
         CDLM_HIP(p->ctx, hipFuncSetAttribute((const void *)k_dict<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_dict)); \
     } while (0)
@@ -824,7 +879,9 @@ cdlm_status run_csc(cdlm_problem *p) {
     CdlDev P = dev_view(p);
     {
         TimedLaunch tl(p, 0);
-#define L_CSC(T) hipLaunchKernelGGL(k_csc<T>, dim3(p->N), dim3(kThreads), p->lds_csc, p->ctx->stream, P)
+#define L_CSC(T)                                                                                                   \
+    if (p->nsl == 4) hipLaunchKernelGGL((k_csc<T, 4>), dim3(p->N), dim3(kThreads), p->lds_csc, p->ctx->stream, P); \
+    else hipLaunchKernelGGL((k_csc<T, 8>), dim3(p->N), dim3(kThreads), p->lds_csc, p->ctx->stream, P)
         CDLM_TAPS_SWITCH(p->taps, L_CSC);
 #undef L_CSC
     }
@@ -837,7 +894,7 @@ cdlm_status run_dict(cdlm_problem *p) {
     hipStream_t st = p->ctx->stream;
     {
         TimedLaunch tl(p, 1);
-#define L_DICT(T) hipLaunchKernelGGL(k_dict<T>, dim3(p->N), dim3(kThreads), p->lds_dict, st, P)
+#define L_DICT(T) hipLaunchKernelGGL(k_dict<T>, dim3(p->N), dim3(kThreads), p->lds_dict, st, P, p->dict_batch)
         CDLM_TAPS_SWITCH(p->taps, L_DICT);
 #undef L_DICT
     }
@@ -961,7 +1018,21 @@ extern "C" cdlm_status cdlm_problem_create(cdlm_ctx *ctx, const cdlm_params *prm
     p->lds_csc = lds_bytes_csc(Lg, gamma);
     p->lds_init = lds_bytes_init(Lg, gamma);
     p->lds_best = lds_bytes_best(Lg, gamma);
-    p->lds_dict = lds_bytes_dict(Lg, gamma);
+    if (Lg > 2048) return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "padded length %d > 2048", Lg);
+    p->nsl = (Lg / 4 <= 256) ? 4 : 8;
+    {
+        // columns per batched FFT pass: most columns in flight per CU
+        int best = 1;
+        long bestv = 0;
+        for (int B : {1, 2, 4}) {
+            const size_t b = lds_bytes_dict(Lg, gamma, B);
+            if (b > kLdsLimit) continue;
+            const long v = (long)B * (long)(kLdsLimit / b);
+            if (v > bestv) { bestv = v; best = B; }
+        }
+        p->dict_batch = best;
+    }
+    p->lds_dict = lds_bytes_dict(Lg, gamma, p->dict_batch);
     p->lds_fin = lds_bytes_fin(Lg);
     p->lds_g1 = sizeof(float2) * 2 * (size_t)Lg;
     size_t mx = p->lds_csc;

@@ -78,13 +78,21 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One radix-R butterfly j of a Stockham stage with current sub-length Ns.
+// q = x / d and r = x - q*d for 0 <= x < 2^24, with rcp = 1.0f / d: the float quotient is
+// within one of the true one and is corrected.
+__device__ __forceinline__ int fdivmod(int x, int d, float rcp, int &r) {
+    int q = (int)((float)x * rcp);
+    r = x - q * d;
+    if (r < 0) { --q; r += d; }
+    else if (r >= d) { ++q; r -= d; }
+    return q;
+}
+
+// One radix-R butterfly j (position k = j mod Ns) of a Stockham stage with sub-length Ns.
 template <int R, bool INV>
-__device__ __forceinline__ void fft_bfly(const float2 *__restrict__ a, float2 *__restrict__ b, int j, int T, int Ns,
-                                         int n, const float2 *__restrict__ tw, int tws) {
-    const int k = j % Ns;
+__device__ __forceinline__ void fft_bfly(const float2 *__restrict__ a, float2 *__restrict__ b, int j, int k, int T,
+                                         int Ns, int st, int rs, const float2 *__restrict__ tw) {
     const int ob = (j - k) * R + k;
-    const int st = (n / (Ns * R)) * tws;
     float2 v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -105,7 +113,6 @@ __device__ __forceinline__ void fft_bfly(const float2 *__restrict__ a, float2 *_
         b[ob + 3 * Ns] = csub(d02, rot);
     } else {
         float2 wr[R];
-        const int rs = (n / R) * tws;
 #pragma unroll
         for (int t = 0; t < R; ++t) wr[t] = twid<INV>(tw, t * rs);
 #pragma unroll
@@ -118,44 +125,60 @@ __device__ __forceinline__ void fft_bfly(const float2 *__restrict__ a, float2 *_
     }
 }
 
-// Team-cooperative complex FFT of length P.n from a into b (ping-pong).  The team is
-// threads [tid0, tid0+nthr) that all call this; SYNC is the team barrier.  Returns the
-// buffer holding the (unscaled) result.  Forward: exp(-2 pi i kl/n); INV: exp(+...).
+// Team-cooperative complex FFTs of length P.n on nb consecutive transforms (transform t at
+// offset t*n) from a into b (ping-pong).  The team is threads [0, nthr) that all call this;
+// SYNC is the team barrier.  Returns the buffer holding the (unscaled) results.
+// Forward: exp(-2 pi i kl/n); INV: exp(+2 pi i kl/n).  tw is the twiddle table (LDS).
 template <bool INV, typename Sync>
 __device__ float2 *team_fft(float2 *a, float2 *b, const FftPlan &P, const float2 *__restrict__ tw, int tid, int nthr,
-                            Sync sync) {
+                            Sync sync, int nb = 1) {
     const int n = P.n;
     const int tws = P.twstride;
     int Ns = 1;
     for (int st = 0; st < P.nst; ++st) {
         const int R = P.radix[st];
         const int T = n / R;
+        const int tot = nb * T;
+        const float rT = 1.0f / (float)T, rNs = 1.0f / (float)Ns;
+        const int stw = (n / (Ns * R)) * tws;   // twiddle step of r*k
+        const int rs = (n / R) * tws;           // twiddle step of the radix-R roots
+#define CDLM_BFLY_LOOP(RR)                                                                        \
+    for (int jj = tid; jj < tot; jj += nthr) {                                                    \
+        int j, k;                                                                                 \
+        const int bb = fdivmod(jj, T, rT, j);                                                     \
+        (void)fdivmod(j, Ns, rNs, k);                                                             \
+        fft_bfly<RR, INV>(a + bb * n, b + bb * n, j, k, T, Ns, stw, rs, tw);                      \
+    }
         switch (R) {
-        case 2: for (int j = tid; j < T; j += nthr) fft_bfly<2, INV>(a, b, j, T, Ns, n, tw, tws); break;
-        case 3: for (int j = tid; j < T; j += nthr) fft_bfly<3, INV>(a, b, j, T, Ns, n, tw, tws); break;
-        case 4: for (int j = tid; j < T; j += nthr) fft_bfly<4, INV>(a, b, j, T, Ns, n, tw, tws); break;
-        case 5: for (int j = tid; j < T; j += nthr) fft_bfly<5, INV>(a, b, j, T, Ns, n, tw, tws); break;
-        case 7: for (int j = tid; j < T; j += nthr) fft_bfly<7, INV>(a, b, j, T, Ns, n, tw, tws); break;
+        case 2: CDLM_BFLY_LOOP(2) break;
+        case 3: CDLM_BFLY_LOOP(3) break;
+        case 4: CDLM_BFLY_LOOP(4) break;
+        case 5: CDLM_BFLY_LOOP(5) break;
+        case 7: CDLM_BFLY_LOOP(7) break;
         default: {
             // Large prime radix: one output (j, q) per work item,
             // y = sum_r a[j + rT] * W_{Ns R}^{r (k + q Ns)}.
             const int NsR = Ns * R;
-            const int st2 = (n / NsR) * tws;
-            for (int idx = tid; idx < n; idx += nthr) {
-                const int j = idx % T, q = idx / T;
-                const int k = j % Ns;
+            const float rn = 1.0f / (float)n;
+            for (int idx = tid; idx < nb * n; idx += nthr) {
+                int rem, j, k;
+                const int bb = fdivmod(idx, n, rn, rem);
+                const int q = fdivmod(rem, T, rT, j);
+                (void)fdivmod(j, Ns, rNs, k);
                 const int kk = k + q * Ns;
-                float2 acc = a[j];
+                const float2 *ab = a + bb * n;
+                float2 acc = ab[j];
                 int e = 0;
                 for (int r = 1; r < R; ++r) {
                     e += kk;
                     if (e >= NsR) e -= NsR;
-                    acc = cadd(acc, cmul(a[j + r * T], twid<INV>(tw, e * st2)));
+                    acc = cadd(acc, cmul(ab[j + r * T], twid<INV>(tw, e * stw)));
                 }
-                b[(j - k) * R + kk] = acc;
+                b[bb * n + (j - k) * R + kk] = acc;
             }
         } break;
         }
+#undef CDLM_BFLY_LOOP
         sync();
         float2 *t = a; a = b; b = t;
         Ns *= R;
