@@ -27,6 +27,7 @@
 // one sequence n and streams its contiguous (Lg x M) slab.
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdarg>
 #include <cstring>
 #include <memory>
@@ -61,6 +62,7 @@ struct CdlDev {
     const int *fl;
     const float *cS, *sS;
     const float2 *tw;
+    float2 *y3s;          // per (m, n): Y3 half spectrum, bins 0..Lg/8 (dictionary scratch)
     FftPlan plan, plan4;
 };
 
@@ -117,6 +119,26 @@ __device__ __forceinline__ void conv_acc4(const float4 *vb4, int s, const float 
         o[q] = acc;
     }
     zacc4[s] = mk4(o);
+}
+
+// sum_j g[j] * v[(4s + q - j) mod Lg] for q = 0..3, returned in registers.
+template <int TAPS>
+__device__ __forceinline__ float4 conv4(const float4 *vb4, int s, const float *g) {
+    float vw[TAPS + 4];
+#pragma unroll
+    for (int u = 0; u <= TAPS / 4; ++u) {
+        float4 t = vb4[s + u];
+        vw[4 * u] = t.x; vw[4 * u + 1] = t.y; vw[4 * u + 2] = t.z; vw[4 * u + 3] = t.w;
+    }
+    float o[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float acc = 0.f;
+#pragma unroll
+        for (int j = 0; j < TAPS; ++j) acc = fmaf(g[j], vw[TAPS + q - j], acc);
+        o[q] = acc;
+    }
+    return mk4(o);
 }
 
 template <int TAPS>
@@ -333,6 +355,218 @@ __global__ __launch_bounds__(kThreads, (NSL <= 4 ? 3 : 2)) void k_csc(CdlDev P) 
     }
 }
 
+// ---------------------------------------------------------------------------- resident CSC
+// K consecutive CSC iterations with the per-sequence state held on chip.  Between
+// dictionary updates G1 is fixed and a CSC iteration of sequence n touches only that
+// sequence, so the state (Xt, y2, y3: one column per wave in registers; h, y1, Y1, DX in LDS)
+// stays resident for all K iterations and goes through HBM once per launch.  A sequence is
+// owned by a group of G work-groups (one per CU, co-resident by cooperative launch); member j
+// holds columns [j*Mg, (j+1)*Mg).  Per iteration the members exchange their z partial sums
+// through L2 (agent-scope release/acquire on a per-member counter, bounded spin) and each
+// computes the identical per-sequence spectral step (same summation order in every member).
+// Group g's members are blocks g, g+NG, ... (NG a multiple of 8: one XCD).
+constexpr int kResThreads = 1024;
+constexpr int kResWaves = kResThreads / 64;
+constexpr long long kSpinLimit = 1LL << 22;
+
+struct ResArgs {
+    int K, G, NG, Mg;
+    unsigned long long epoch;
+    unsigned long long *flags;   // [NG * G] progress counters
+    float *xbuf;                 // [NG][G][2][Lg] z partials (double buffered by iteration parity)
+    int *err;                    // set when an exchange wait times out
+};
+
+__device__ __forceinline__ void st_agent(float *p, float v) {
+    __hip_atomic_store(reinterpret_cast<unsigned int *>(p), __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float ld_agent(const float *p) {
+    return __uint_as_float(__hip_atomic_load(reinterpret_cast<const unsigned int *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+template <int TAPS>
+__global__ __launch_bounds__(kResThreads) void k_csc_res(CdlDev P, ResArgs A) {
+    extern __shared__ float4 smem4[];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int Lg = P.Lg, M = P.M, nslot = Lg >> 2;
+    const int group = blockIdx.x % A.NG, member = blockIdx.x / A.NG;
+    const int mw = member * A.Mg + w;
+    const int mm = __builtin_amdgcn_readfirstlane(mw);
+    const bool own = (w < A.Mg) && (mm < M);
+    int nw = M - member * A.Mg;                               // waves holding a column (the lowest ones)
+    nw = nw < A.Mg ? nw : A.Mg;
+    nw = nw < 0 ? 0 : nw;
+    const int hb4Len = (Lg + TAPS + 4) / 4, vb4Len = (Lg + TAPS) / 4;
+    float2 *tw = reinterpret_cast<float2 *>(smem4);
+    float4 *hb4 = smem4 + Lg / 2;
+    float *hb = reinterpret_cast<float *>(hb4);
+    float4 *vbase4 = hb4 + hb4Len;
+    float *y1v = reinterpret_cast<float *>(vbase4 + kResWaves * vb4Len);
+    float *Y1v = y1v + Lg;
+    float *DXv = Y1v + Lg;
+    float *zt = DXv + Lg;
+    float2 *fa = reinterpret_cast<float2 *>(zt + Lg);
+    float2 *fb = fa + Lg;
+    float *taps = reinterpret_cast<float *>(fb + Lg);         // [wave][2][TAPS]: rebuild filter, current filter
+    float4 *vb4 = vbase4 + w * vb4Len;
+    float *gr = taps + w * 2 * TAPS;
+    float *gc = gr + TAPS;
+    const float lam = P.lambda, c0 = P.sh_csc, inv_c0 = 1.0f / P.sh_csc, rho = P.rho;
+    const float scale = 1.0f / (float)Lg;
+
+    for (int l = tid; l < Lg; l += kResThreads) tw[l] = P.tw[l];
+    int round = 0;
+    for (int n = group; n < P.N; n += A.NG, ++round) {
+        const size_t ro = (size_t)n * Lg;
+        const size_t col = ((size_t)n * M + mm) * Lg;
+        if (own && lane < TAPS) {
+            gr[lane] = P.G1rec[(size_t)mm * Lg + lane];     // the filter this sequence's h was made with
+            gc[lane] = P.G1cur[(size_t)mm * Lg + lane];
+        }
+        for (int l = tid; l < Lg; l += kResThreads) {
+            hb[l] = P.h[ro + l];
+            y1v[l] = P.y1[ro + l];
+            Y1v[l] = P.Y1[ro + l];
+            DXv[l] = P.DX[ro + l];
+        }
+        for (int t = tid; t < TAPS + 4; t += kResThreads) hb[Lg + t] = P.h[ro + t];
+        float4 xt[4], a2[4], a3[4];
+        if (own) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int s = lane + 64 * i;
+                if (s < nslot) { xt[i] = ld4(P.Xt + col, s); a2[i] = ld4(P.y2 + col, s); a3[i] = ld4(P.y3 + col, s); }
+            }
+        }
+        __syncthreads();
+        for (int it = 0; it < A.K; ++it) {
+            if (own) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int s = lane + 64 * i;
+                    if (s < nslot) {
+                        float X[4];
+                        recon4<TAPS>(xt[i], hb4, s, gr, X);
+                        float y2n[4], y3n[4], v[4];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const float x = X[q];
+                            const float t = x + f4(a2[i], q);                   // shrink! (csc_kernels.jl:58-59)
+                            const float at = fabsf(t) - lam;
+                            const float Y2 = at > 0.f ? copysignf(at, t) : 0.f;
+                            const float Y3 = (q == 0) ? x + f4(a3[i], q) : 0.f;  // projectionX! (:73-77)
+                            y2n[q] = f4(a2[i], q) + x - Y2;                       // train.jl:13
+                            y3n[q] = f4(a3[i], q) + x - Y3;                       // train.jl:14
+                            v[q] = (Y2 - y2n[q]) + (Y3 - y3n[q]);
+                        }
+                        a2[i] = mk4(y2n);
+                        a3[i] = mk4(y3n);
+                        xt[i] = make_float4(v[0] * inv_c0, v[1] * inv_c0, v[2] * inv_c0, v[3] * inv_c0);
+                        vb4[TAPS / 4 + s] = mk4(v);
+                    }
+                }
+                wave_lds_sync();
+                if (lane < TAPS / 4) vb4[TAPS / 4 - 1 - lane] = vb4[TAPS / 4 + nslot - 1 - lane];
+                if (it == 0 && lane < TAPS) gr[lane] = gc[lane];   // later rebuilds use this filter
+                wave_lds_sync();
+                // z partial of this column, written in place over the v buffer: slot s reads
+                // v slots s..s+TAPS/4 before writing slot s, and slot groups of consecutive
+                // unrolled steps are disjoint
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int s = lane + 64 * i;
+                    if (s < nslot) {
+                        const float4 z = conv4<TAPS>(vb4, s, gc);
+                        vb4[s] = z;
+                    }
+                }
+            }
+            __syncthreads();
+            // z partial of this member: waves in ascending order
+            const int parity = it & 1;
+            float *xme = A.xbuf + (((size_t)group * A.G + member) * 2 + parity) * Lg;
+            for (int l = tid; l < Lg; l += kResThreads) {
+                float acc = 0.f;
+                for (int ww = 0; ww < nw; ++ww) acc += reinterpret_cast<const float *>(vbase4 + ww * vb4Len)[l];
+                if (A.G == 1) zt[l] = acc;
+                else st_agent(xme + l, acc);
+            }
+            if (A.G > 1) {
+                const unsigned long long target = (A.epoch << 32) | (unsigned long long)(round * A.K + it + 1);
+                __syncthreads();
+                if (tid == 0)
+                    __hip_atomic_store(A.flags + (size_t)group * A.G + member, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                if (tid < A.G && tid != member) {
+                    const unsigned long long *f = A.flags + (size_t)group * A.G + tid;
+                    long long spins = 0;
+                    while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                        if (++spins > kSpinLimit) { atomicExch(A.err, 1); break; }
+                        if ((spins & 255) == 0 && __hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                        __builtin_amdgcn_s_sleep(2);
+                    }
+                }
+                __syncthreads();
+                for (int l = tid; l < Lg; l += kResThreads) {
+                    float acc = 0.f;
+                    for (int j = 0; j < A.G; ++j) acc += ld_agent(A.xbuf + (((size_t)group * A.G + j) * 2 + parity) * Lg + l);
+                    zt[l] = acc;
+                }
+            }
+            __syncthreads();
+            // u = Y1 - y1' with y1' = y1 + DX - Y1 (train.jl:12)
+            for (int l = tid; l < Lg; l += kResThreads) {
+                const float Y1o = Y1v[l];
+                const float y1n = y1v[l] + DXv[l] - Y1o;
+                y1v[l] = y1n;
+                fa[l] = make_float2(zt[l], Y1o - y1n);
+            }
+            __syncthreads();
+            float2 *C = team_fft<false>(fa, fb, P.plan, tw, tid, kResThreads, BlockSync{});
+            float2 *O = (C == fa) ? fb : fa;
+            for (int k = tid; k < Lg; k += kResThreads) {
+                const float2 Ck = C[k], Cm = C[k == 0 ? 0 : Lg - k];
+                const float2 Z = make_float2(0.5f * (Ck.x + Cm.x), 0.5f * (Ck.y - Cm.y));
+                const float2 U = make_float2(0.5f * (Ck.y + Cm.y), -0.5f * (Ck.x - Cm.x));
+                const float ss = P.sS[k];
+                const float den = c0 * P.cS[k];
+                const float2 hh = make_float2((c0 * U.x - Z.x) / den, (c0 * U.y - Z.y) / den);
+                const float2 dx = make_float2(Z.x * inv_c0 + ss * hh.x, Z.y * inv_c0 + ss * hh.y);
+                O[k] = make_float2(hh.x - dx.y, hh.y + dx.x);
+            }
+            __syncthreads();
+            float2 *R = team_fft<true>(O, C, P.plan, tw, tid, kResThreads, BlockSync{});
+            for (int l = tid; l < Lg; l += kResThreads) {
+                const float hn = R[l].x * scale;
+                const float DXn = R[l].y * scale;
+                const bool win = (l >= P.gamma) && (l < P.gamma + P.L4);
+                const float b = win ? P.s[(size_t)n * P.L4 + (l - P.gamma)] : 0.f;
+                const float wl = win ? 1.f : 0.f;
+                hb[l] = hn;
+                if (l < TAPS + 4) hb[Lg + l] = hn;
+                DXv[l] = DXn;
+                Y1v[l] = (b + rho * (DXn + y1v[l])) / (wl + rho);   // update_Y1! (csc_kernels.jl:91)
+            }
+            __syncthreads();
+        }
+        // write the state back
+        if (own) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int s = lane + 64 * i;
+                if (s < nslot) { st4(P.Xt + col, s, xt[i]); st4(P.y2 + col, s, a2[i]); st4(P.y3 + col, s, a3[i]); }
+            }
+        }
+        if (member == 0)
+            for (int l = tid; l < Lg; l += kResThreads) {
+                P.h[ro + l] = hb[l];
+                P.y1[ro + l] = y1v[l];
+                P.Y1[ro + l] = Y1v[l];
+                P.DX[ro + l] = DXv[l];
+            }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------- best / objective
 // Per-sequence partials of objective() (CDL_setup.jl:286): sum (DX_window - s)^2, sum |X|,
 // and the positive-X statistics of update_best!/wrap.jl:12; writes the candidate
@@ -439,16 +673,18 @@ __global__ __launch_bounds__(kThreads) void k_decide(CdlScal *sc, float lambda, 
 
 // ---------------------------------------------------------------------------- dict
 // One iteration of dict_update! (train.jl:50-72) for sequence n; the means over n
-// (:76,:83) follow in k_dict_reduce / k_dict_finalize.  Columns go through the FFTs B at a
-// time (batched team transforms); accumulations over m stay in ascending m.
+// (:76,:83) follow in k_dict_reduce / k_dict_finalize.  Columns go through the transforms B
+// at a time (B even): pass 1 packs two columns' Y3 into one complex length-Lg/4 FFT and stores
+// each column's half spectrum in y3s; pass 2 packs two columns' real inverse transforms
+// into one complex FFT.  Accumulations over m stay in ascending m.
 // LDS: twiddles | h with wrapped tail | taps of the batch | fa, fb (B x Lg) | ya, yb
-// (B x Lg/4) | accS, accT, accP (Lg/2+1 bins).
+// (B/2 x P) | ys (B x P) | accS, accT, accP (Lg/2+1 bins), with P = Lg/4.
 template <int TAPS>
 __global__ __launch_bounds__(kThreads) void k_dict(CdlDev P, int B) {
     extern __shared__ float4 smem4[];
     float *sm = reinterpret_cast<float *>(smem4);
     const int n = blockIdx.x, tid = threadIdx.x;
-    const int Lg = P.Lg, M = P.M, nslot = Lg >> 2, L4q = Lg >> 2;
+    const int Lg = P.Lg, M = P.M, nslot = Lg >> 2, Pq = Lg >> 2, Ph = Pq / 2 + 1;
     const int H = Lg / 2 + 1, Hp = (H + 3) & ~3;
     const int hbLen = Lg + TAPS + 4;
     float2 *tw = reinterpret_cast<float2 *>(sm);
@@ -457,12 +693,14 @@ __global__ __launch_bounds__(kThreads) void k_dict(CdlDev P, int B) {
     float2 *fa = reinterpret_cast<float2 *>(gtap + ((B * TAPS + 3) & ~3));
     float2 *fb = fa + B * Lg;
     float2 *ya = fb + B * Lg;
-    float2 *yb = ya + B * L4q;
-    float *accS = reinterpret_cast<float *>(yb + B * L4q);
+    float2 *yb = ya + (B / 2) * Pq;
+    float2 *ys = yb + (B / 2) * Pq;
+    float *accS = reinterpret_cast<float *>(ys + B * Pq);
     float2 *accT = reinterpret_cast<float2 *>(accS + Hp);
     float2 *accP = accT + Hp;
     const size_t ro = (size_t)n * Lg;
-    const float rslot = 1.0f / (float)nslot, rLg = 1.0f / (float)Lg, rL4q = 1.0f / (float)L4q;
+    const float rslot = 1.0f / (float)nslot, rLg = 1.0f / (float)Lg, rPq = 1.0f / (float)Pq;
+    float2 *y3s = P.y3s + (size_t)n * M * Ph;
 
     for (int l = tid; l < Lg; l += kThreads) { tw[l] = P.tw[l]; hb[l] = P.h[ro + l]; }
     for (int t = tid; t < TAPS + 4; t += kThreads) hb[Lg + t] = P.h[ro + t];
@@ -471,7 +709,10 @@ __global__ __launch_bounds__(kThreads) void k_dict(CdlDev P, int B) {
     // ---- pass 1: spectra sums over m
     for (int m0 = 0; m0 < M; m0 += B) {
         const int nb = (M - m0) < B ? (M - m0) : B;
+        const int np = (nb + 1) / 2;
         for (int i = tid; i < nb * TAPS; i += kThreads) gtap[i] = P.G1rec[(size_t)(m0 + i / TAPS) * Lg + i % TAPS];
+        if (nb & 1)
+            for (int s = tid; s < Pq; s += kThreads) ya[(np - 1) * Pq + s].y = 0.f;
         __syncthreads();
         for (int idx = tid; idx < nb * nslot; idx += kThreads) {
             int s;
@@ -485,7 +726,8 @@ __global__ __launch_bounds__(kThreads) void k_dict(CdlDev P, int B) {
             const float4 G1v = ld4(P.G1cur + (size_t)m * Lg, s);
             const float4 G3v = ld4(P.G3 + (size_t)m * Lg, s);
             const float X0 = recon1<TAPS>(xt.x, hb, s, gtap + bb * TAPS);
-            ya[bb * L4q + s] = make_float2(X0 + a3.x, 0.f);    // Y3 on the stride-4 support
+            float *yaf = reinterpret_cast<float *>(ya);
+            yaf[2 * ((bb >> 1) * Pq + s) + (bb & 1)] = X0 + a3.x;    // Y3 on the stride-4 support
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const float qq = (f4(G1v, q) - f4(a1, q)) + (f4(G3v, q) - f4(c3, q));
@@ -493,18 +735,29 @@ __global__ __launch_bounds__(kThreads) void k_dict(CdlDev P, int B) {
             }
         }
         __syncthreads();
-        float2 *Y = team_fft<false>(ya, yb, P.plan4, tw, tid, kThreads, BlockSync{}, nb);
+        float2 *Y = team_fft<false>(ya, yb, P.plan4, tw, tid, kThreads, BlockSync{}, np);
         float2 *C = team_fft<false>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{}, nb);
+        // unpack the column pairs: Y3h of column 2p is the Hermitian part, of 2p+1 the other
+        for (int idx = tid; idx < nb * Pq; idx += kThreads) {
+            int k;
+            const int bb = fdivmod(idx, Pq, rPq, k);
+            const float2 Yk = Y[(bb >> 1) * Pq + k], Ym = Y[(bb >> 1) * Pq + (k == 0 ? 0 : Pq - k)];
+            const float2 v = (bb & 1) ? make_float2(0.5f * (Yk.y + Ym.y), -0.5f * (Yk.x - Ym.x))
+                                      : make_float2(0.5f * (Yk.x + Ym.x), 0.5f * (Yk.y - Ym.y));
+            ys[bb * Pq + k] = v;
+            if (k < Ph) y3s[(size_t)(m0 + bb) * Ph + k] = v;
+        }
+        __syncthreads();
         for (int k = tid; k < H; k += kThreads) {
             int kq;
-            (void)fdivmod(k, L4q, rL4q, kq);
+            (void)fdivmod(k, Pq, rPq, kq);
             float aS = accS[k];
             float2 aT = accT[k], aP = accP[k];
             for (int bb = 0; bb < nb; ++bb) {
                 const float2 Ck = C[bb * Lg + k], Cm = C[bb * Lg + (k == 0 ? 0 : Lg - k)];
                 const float2 Dh = make_float2(0.5f * (Ck.x + Cm.x), 0.5f * (Ck.y - Cm.y));
                 const float2 Qh = make_float2(0.5f * (Ck.y + Cm.y), -0.5f * (Ck.x - Cm.x));
-                const float2 Yk = Y[bb * L4q + kq];
+                const float2 Yk = ys[bb * Pq + kq];
                 aS += Yk.x * Yk.x + Yk.y * Yk.y;
                 aT = cadd(aT, cmul(Yk, Dh));
                 aP = cadd(aP, cmul(Yk, Qh));
@@ -517,16 +770,17 @@ __global__ __launch_bounds__(kThreads) void k_dict(CdlDev P, int B) {
     // ---- per-sequence spectral step: Td, G2/g2 (update_g2!, train.jl:68-71), kappa
     for (int k = tid; k < Lg; k += kThreads) fa[k] = (k < H) ? accT[k] : cconj(accT[Lg - k]);
     __syncthreads();
-    float *tdb = reinterpret_cast<float *>(ya);      // Td (Lg floats fit in ya+yb: B*Lg/2 complex >= Lg floats)
+    float2 *R2 = team_fft<true>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{}, 1);       // Td (fa[0:Lg])
+    float *tdb = reinterpret_cast<float *>(ys);      // Lg floats fit in ys (B * Lg/4 complex, B >= 2)
     {
-        float2 *R = team_fft<true>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{}, 1);
         const float scale = 1.0f / (float)Lg;
-        for (int l = tid; l < Lg; l += kThreads) tdb[l] = R[l].x * scale;
+        for (int l = tid; l < Lg; l += kThreads) tdb[l] = R2[l].x * scale;
     }
     __syncthreads();
-    for (int l = tid; l < Lg; l += kThreads) fa[l] = make_float2(P.G2[ro + l] - P.g2[ro + l], 0.f);
-    __syncthreads();
     {
+        // F(G2 - g2) of the previous dictionary iteration (sherman_CDL_RHS!, cdl_kernels.jl:8)
+        for (int l = tid; l < Lg; l += kThreads) fa[l] = make_float2(P.G2[ro + l] - P.g2[ro + l], 0.f);
+        __syncthreads();
         float2 *R = team_fft<false>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{}, 1);
         const float c1 = P.sh_cdl;
         for (int k = tid; k < H; k += kThreads) {
@@ -552,26 +806,27 @@ __global__ __launch_bounds__(kThreads) void k_dict(CdlDev P, int B) {
     const float scale = 1.0f / (float)Lg;
     for (int m0 = 0; m0 < M; m0 += B) {
         const int nb = (M - m0) < B ? (M - m0) : B;
-        for (int i = tid; i < nb * TAPS; i += kThreads) gtap[i] = P.G1rec[(size_t)(m0 + i / TAPS) * Lg + i % TAPS];
-        __syncthreads();
-        for (int idx = tid; idx < nb * nslot; idx += kThreads) {
-            int s;
-            const int bb = fdivmod(idx, nslot, rslot, s), m = m0 + bb;
-            const size_t col = ((size_t)n * M + m) * Lg;
-            const float X0 = recon1<TAPS>(P.Xt[col + 4 * s], hb, s, gtap + bb * TAPS);
-            ya[bb * L4q + s] = make_float2(X0 + P.y3[col + 4 * s], 0.f);
+        const int np = (nb + 1) / 2;
+        for (int idx = tid; idx < nb * Ph; idx += kThreads) {
+            int k;
+            const int bb = idx / Ph;
+            k = idx - bb * Ph;
+            const float2 v = y3s[(size_t)(m0 + bb) * Ph + k];
+            ys[bb * Pq + k] = v;
+            if (k > 0 && Pq - k >= Ph) ys[bb * Pq + Pq - k] = cconj(v);
         }
         __syncthreads();
-        float2 *Y = team_fft<false>(ya, yb, P.plan4, tw, tid, kThreads, BlockSync{}, nb);
-        for (int idx = tid; idx < nb * Lg; idx += kThreads) {
+        for (int idx = tid; idx < np * Lg; idx += kThreads) {
             int k, kq;
-            const int bb = fdivmod(idx, Lg, rLg, k);
-            (void)fdivmod(k, L4q, rL4q, kq);
+            const int pp = fdivmod(idx, Lg, rLg, k);
+            (void)fdivmod(k, Pq, rPq, kq);
             const float2 kap = (k < H) ? accP[k] : cconj(accP[Lg - k]);
-            fa[idx] = cmul(cconj(Y[bb * L4q + kq]), kap);
+            const float2 CA = cmul(cconj(ys[(2 * pp) * Pq + kq]), kap);
+            const float2 CB = (2 * pp + 1 < nb) ? cmul(cconj(ys[(2 * pp + 1) * Pq + kq]), kap) : make_float2(0.f, 0.f);
+            fa[idx] = make_float2(CA.x - CB.y, CA.y + CB.x);
         }
         __syncthreads();
-        float2 *R = team_fft<true>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{}, nb);
+        float2 *R = team_fft<true>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{}, np);
         for (int idx = tid; idx < nb * nslot; idx += kThreads) {
             int s;
             const int bb = fdivmod(idx, nslot, rslot, s), m = m0 + bb;
@@ -580,11 +835,13 @@ __global__ __launch_bounds__(kThreads) void k_dict(CdlDev P, int B) {
             const float4 G3v = ld4(P.G3 + (size_t)m * Lg, s);
             const float4 a1 = ld4(P.g1 + col, s);
             const float4 c3 = ld4(P.g3 + col, s);
+            const float2 *Rp = R + (bb >> 1) * Lg + 4 * s;
             float dn[4], g1n[4], g3n[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const float qv = (f4(G1v, q) - f4(a1, q)) + (f4(G3v, q) - f4(c3, q));
-                dn[q] = qv + R[bb * Lg + 4 * s + q].x * scale;
+                const float cv = (bb & 1) ? Rp[q].y : Rp[q].x;
+                dn[q] = qv + cv * scale;
                 g1n[q] = f4(a1, q) + dn[q] - f4(G1v, q);       // train.jl:70
                 g3n[q] = f4(c3, q) + dn[q] - f4(G3v, q);       // train.jl:72
             }
@@ -728,6 +985,7 @@ struct cdlm_problem {
     int g1cur = 0, g1rec = 0;
     float *G3 = nullptr, *meand = nullptr, *Dbest = nullptr, *gsq = nullptr, *cS = nullptr, *sS = nullptr;
     float *xs[2] = {nullptr, nullptr};
+    float2 *y3s = nullptr;
     int *fl_dev = nullptr;
     float2 *tw = nullptr;
     double *partials = nullptr, *dpart = nullptr, *trace = nullptr;
@@ -739,7 +997,16 @@ struct cdlm_problem {
     int dict_batch = 1;
     int nsl = 4;
     bool timing = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[4];
+    struct Ev { hipEvent_t a, b; int count; };
+    std::vector<Ev> ev[4];
+    // resident CSC (k_csc_res): groups of G co-resident work-groups per sequence
+    bool res = false;
+    int resG = 1, resNG = 0, resMg = 0;
+    size_t lds_res = 0;
+    unsigned long long epoch = 0;
+    unsigned long long *rflags = nullptr;
+    float *rxbuf = nullptr;
+    int *rerr = nullptr;
 };
 
 namespace {
@@ -773,7 +1040,7 @@ CdlDev dev_view(const cdlm_problem *p) {
     P.Xt = p->Xt; P.y2 = p->y2; P.y3 = p->y3; P.d = p->d; P.g1 = p->g1; P.g3 = p->g3;
     P.h = p->h; P.y1 = p->y1; P.Y1 = p->Y1; P.DX = p->DX; P.G2 = p->G2; P.g2 = p->g2; P.Td = p->Td;
     P.G1cur = p->G1buf[p->g1cur]; P.G1rec = p->G1buf[p->g1rec]; P.G3 = p->G3;
-    P.fl = p->fl_dev; P.cS = p->cS; P.sS = p->sS; P.tw = p->tw;
+    P.fl = p->fl_dev; P.cS = p->cS; P.sS = p->sS; P.tw = p->tw; P.y3s = p->y3s;
     P.plan = p->plan; P.plan4 = p->plan4;
     return P;
 }
@@ -785,8 +1052,9 @@ size_t lds_bytes_init(int Lg, int taps) { return sizeof(float) * (kWaves * (size
 size_t lds_bytes_best(int Lg, int taps) { return sizeof(float) * (size_t)(Lg + taps + 4); }
 size_t lds_bytes_dict(int Lg, int taps, int B) {
     const int H = Lg / 2 + 1, Hp = (H + 3) & ~3;
+    const size_t Pq = (size_t)Lg / 4;
     return sizeof(float) * (2 * (size_t)Lg + (size_t)(Lg + taps + 4) + (size_t)((B * taps + 3) & ~3) + 4 * (size_t)B * Lg +
-                            (size_t)B * Lg + 5 * (size_t)Hp);
+                            2 * 2 * (size_t)(B / 2) * Pq + 2 * (size_t)B * Pq + 5 * (size_t)Hp);
 }
 size_t lds_bytes_fin(int Lg) { return sizeof(float) * (((size_t)Lg + 3) / 4 * 4 + 4 * (size_t)Lg); }
 constexpr size_t kLdsLimit = 160 * 1024;
@@ -801,6 +1069,7 @@ cdlm_status set_lds_limits(cdlm_problem *p) {
         CDLM_HIP(p->ctx, hipFuncSetAttribute((const void *)k_csc<T, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_csc));  \
         CDLM_HIP(p->ctx, hipFuncSetAttribute((const void *)k_best<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_best)); \
         CDLM_HIP(p->ctx, hipFuncSetAttribute((const void *)k_dict<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_dict)); \
+        if (p->res) CDLM_HIP(p->ctx, hipFuncSetAttribute((const void *)k_csc_res<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_res)); \
     } while (0)
     switch (p->taps) {
     case 4: SETA(4); break;
@@ -823,8 +1092,9 @@ cdlm_status set_lds_limits(cdlm_problem *p) {
 struct TimedLaunch {
     cdlm_problem *p;
     int kid;
+    int count;
     hipEvent_t a = nullptr, b = nullptr;
-    TimedLaunch(cdlm_problem *p_, int kid_) : p(p_), kid(kid_) {
+    TimedLaunch(cdlm_problem *p_, int kid_, int count_ = 1) : p(p_), kid(kid_), count(count_) {
         if (p->timing && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
             (void)hipEventRecord(a, p->ctx->stream);
         else if (p->timing) { (void)hipGetLastError(); a = b = nullptr; }
@@ -832,14 +1102,14 @@ struct TimedLaunch {
     ~TimedLaunch() {
         if (a && b) {
             (void)hipEventRecord(b, p->ctx->stream);
-            p->ev[kid].push_back({a, b});
+            p->ev[kid].push_back({a, b, count});
         }
     }
 };
 
 void clear_events(cdlm_problem *p) {
     for (auto &v : p->ev) {
-        for (auto &e : v) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+        for (auto &e : v) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
         v.clear();
     }
 }
@@ -887,6 +1157,46 @@ cdlm_status run_csc(cdlm_problem *p) {
     }
     p->g1rec = p->g1cur;
     return launch_check(p, "k_csc");
+}
+
+// K CSC iterations in one cooperative launch of the resident kernel.
+cdlm_status run_csc_res(cdlm_problem *p, int K) {
+    if (K <= 0) return CDLM_OK;
+    CdlDev P = dev_view(p);
+    ResArgs A;
+    A.K = K; A.G = p->resG; A.NG = p->resNG; A.Mg = p->resMg;
+    A.epoch = ++p->epoch;
+    A.flags = p->rflags; A.xbuf = p->rxbuf; A.err = p->rerr;
+    void *args[] = {(void *)&P, (void *)&A};
+    hipError_t e = hipSuccess;
+    {
+        TimedLaunch tl(p, 0, K);
+#define L_RES(T) e = hipLaunchCooperativeKernel((const void *)k_csc_res<T>, dim3(p->resG * p->resNG), dim3(kResThreads), args, (unsigned)p->lds_res, p->ctx->stream)
+        CDLM_TAPS_SWITCH(p->taps, L_RES);
+#undef L_RES
+    }
+    if (e != hipSuccess) return cdlm_fail(p->ctx, CDLM_ERR_HIP, "cooperative launch of k_csc_res: %s", hipGetErrorString(e));
+    p->g1rec = p->g1cur;
+    return launch_check(p, "k_csc_res");
+}
+
+// CSC iterations through whichever kernel the shape uses.
+cdlm_status run_csc_n(cdlm_problem *p, int K) {
+    if (p->res) return run_csc_res(p, K);
+    for (int i = 0; i < K; ++i) {
+        cdlm_status st = run_csc(p);
+        if (st != CDLM_OK) return st;
+    }
+    return CDLM_OK;
+}
+
+// Reports a timed-out exchange of the resident kernel (set on the device).
+cdlm_status check_res_err(cdlm_problem *p) {
+    if (!p->res) return CDLM_OK;
+    int err = 0;
+    CDLM_HIP(p->ctx, hipMemcpy(&err, p->rerr, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) return cdlm_fail(p->ctx, CDLM_ERR_STATE, "resident CSC exchange timed out (co-residency lost)");
+    return CDLM_OK;
 }
 
 cdlm_status run_dict(cdlm_problem *p) {
@@ -977,6 +1287,7 @@ extern "C" size_t cdlm_footprint_bytes(int32_t L4, int32_t N, int32_t M, int32_t
     if (keep_x0) b += E * 4;
     b += 7 * LN * 4;                          // h, y1, Y1, DX, G2, g2, Td
     b += 2 * L * M * N * 4;                   // X_best slices
+    b += (size_t)M * N * (Lg / 8 + 1) * 8;    // dictionary Y3 half spectra
     b += (3 + 4) * LM * 4 + 2 * Lg * 4;       // G1 x3, G3, meand, Dbest, gsq, cS, sS
     b += (size_t)N * kPartStride * 8;
     const size_t blocks = (LM + kThreads - 1) / kThreads;
@@ -1024,7 +1335,7 @@ extern "C" cdlm_status cdlm_problem_create(cdlm_ctx *ctx, const cdlm_params *prm
         // columns per batched FFT pass: most columns in flight per CU
         int best = 1;
         long bestv = 0;
-        for (int B : {1, 2, 4}) {
+        for (int B : {2, 4}) {
             const size_t b = lds_bytes_dict(Lg, gamma, B);
             if (b > kLdsLimit) continue;
             const long v = (long)B * (long)(kLdsLimit / b);
@@ -1035,11 +1346,45 @@ extern "C" cdlm_status cdlm_problem_create(cdlm_ctx *ctx, const cdlm_params *prm
     p->lds_dict = lds_bytes_dict(Lg, gamma, p->dict_batch);
     p->lds_fin = lds_bytes_fin(Lg);
     p->lds_g1 = sizeof(float2) * 2 * (size_t)Lg;
+    {
+        // resident CSC: one column per wave, 16 waves per work-group, ceil(M/16) work-groups per sequence
+        const char *env = getenv("CDLM_CSC_RESIDENT");
+        int coop = 0;
+        (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, ctx->device);
+        // opt-in: measured slower than the streaming kernel at N=20000, M=64, Lg=840 (7.55 vs
+        // 5.44 ms per iteration) because one sequence per CU exposes the per-iteration FFT and
+        // exchange latency
+        p->res = coop && (Lg / 4 <= 256) && (env && env[0] == '1');
+        if (p->res) {
+            p->resG = (M + kResWaves - 1) / kResWaves;
+            p->resMg = (M + p->resG - 1) / p->resG;
+            p->lds_res = sizeof(float) * (27 * (size_t)Lg + 17 * (size_t)gamma + 4 + kResWaves * 2 * (size_t)gamma);
+            if (p->lds_res > kLdsLimit) p->res = false;
+        }
+    }
     size_t mx = p->lds_csc;
     for (size_t v : {p->lds_init, p->lds_best, p->lds_dict, p->lds_fin, p->lds_g1}) mx = v > mx ? v : mx;
     if (mx > kLdsLimit - kThreads * sizeof(double)) return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "padded length %d needs %zu B of LDS", Lg, mx);
     cdlm_status st = set_lds_limits(p.get());
     if (st != CDLM_OK) return st;
+    if (p->res) {
+        int nb = 0;
+        hipError_t oe = hipErrorUnknown;
+#define OCC(T) oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_csc_res<T>, kResThreads, p->lds_res)
+        CDLM_TAPS_SWITCH(p->taps, OCC);
+#undef OCC
+        if (oe != hipSuccess || nb < 1) {
+            (void)hipGetLastError();
+            p->res = false;
+        } else {
+            int ng = (nb * ctx->num_cu) / p->resG;
+            if (ng >= 8) ng -= ng % 8;              // members of a group on one XCD
+            const int cap = ((N + 7) / 8) * 8;
+            if (ng > cap) ng = cap;
+            p->resNG = ng;
+            if (ng < 1) p->res = false;
+        }
+    }
 
     // host-side masked D0 (P_D .* D, CDL_setup.jl:183)
     const size_t LM = (size_t)Lg * M, E = LM * N, LN = (size_t)Lg * N;
@@ -1065,6 +1410,12 @@ extern "C" cdlm_status cdlm_problem_create(cdlm_ctx *ctx, const cdlm_params *prm
     A(p->G3, LM); A(p->meand, LM); A(p->Dbest, LM); A(p->gsq, LM);
     A(p->cS, (size_t)Lg); A(p->sS, (size_t)Lg);
     A(p->xs[0], (size_t)p->L * M * N); A(p->xs[1], (size_t)p->L * M * N);
+    A(p->y3s, (size_t)M * N * (Lg / 8 + 1));
+    if (p->res) {
+        A(p->rflags, (size_t)p->resNG * p->resG);
+        A(p->rxbuf, (size_t)p->resNG * p->resG * 2 * Lg);
+        A(p->rerr, 1);
+    }
     A(p->fl_dev, (size_t)M);
     A(p->tw, (size_t)Lg);
     A(p->partials, (size_t)N * kPartStride);
@@ -1076,6 +1427,10 @@ extern "C" cdlm_status cdlm_problem_create(cdlm_ctx *ctx, const cdlm_params *prm
     CDLM_HIP(ctx, hipMemcpyAsync(p->s, s, (size_t)L4 * N * sizeof(float), hipMemcpyHostToDevice, hs));
     if (X0) CDLM_HIP(ctx, hipMemcpyAsync(p->X0, X0, E * sizeof(float), hipMemcpyHostToDevice, hs));
     CDLM_HIP(ctx, hipMemcpyAsync(p->fl_dev, p->fl.data(), M * sizeof(int), hipMemcpyHostToDevice, hs));
+    if (p->res) {
+        CDLM_HIP(ctx, hipMemsetAsync(p->rflags, 0, sizeof(unsigned long long) * p->resNG * p->resG, hs));
+        CDLM_HIP(ctx, hipMemsetAsync(p->rerr, 0, sizeof(int), hs));
+    }
     std::vector<float2> tw(Lg);
     for (int t = 0; t < Lg; ++t) {
         const double a = -2.0 * M_PI * (double)t / (double)Lg;
@@ -1122,11 +1477,11 @@ extern "C" size_t cdlm_problem_bytes(const cdlm_problem *p) { return p ? p->tota
 extern "C" cdlm_status cdlm_code_update(cdlm_problem *p, int32_t num_iter) {
     if (!p) return CDLM_ERR_ARG;
     if (num_iter < 0) return cdlm_fail(p->ctx, CDLM_ERR_ARG, "num_iter < 0");
-    for (int i = 0; i < num_iter; ++i) {
-        cdlm_status st = run_csc(p);
-        if (st != CDLM_OK) return st;
-    }
-    return sync_stream(p);
+    cdlm_status st = run_csc_n(p, num_iter);
+    if (st != CDLM_OK) return st;
+    st = sync_stream(p);
+    if (st != CDLM_OK) return st;
+    return check_res_err(p);
 }
 
 extern "C" cdlm_status cdlm_dict_update(cdlm_problem *p, int32_t num_iter) {
@@ -1181,13 +1536,15 @@ extern "C" cdlm_status cdlm_optimize(cdlm_problem *p, int32_t iter_num, int32_t 
     if (iter_num > p->trace_cap) return cdlm_fail(p->ctx, CDLM_ERR_ARG, "iter_num > %d", p->trace_cap);
     cdlm_status st;
     for (int it = 0; it < iter_num; ++it) {
-        for (int i = 0; i < csc_iter; ++i) if ((st = run_csc(p)) != CDLM_OK) return st;
+        if ((st = run_csc_n(p, csc_iter)) != CDLM_OK) return st;
         if ((st = run_best(p, 0, it, nullptr)) != CDLM_OK) return st;
         for (int i = 0; i < cdl_iter; ++i) if ((st = run_dict(p)) != CDLM_OK) return st;
     }
     if (obj_trace && iter_num > 0)
         CDLM_HIP(p->ctx, hipMemcpyAsync(obj_trace, p->trace, iter_num * sizeof(double), hipMemcpyDeviceToHost, p->ctx->stream));
-    return sync_stream(p);
+    st = sync_stream(p);
+    if (st != CDLM_OK) return st;
+    return check_res_err(p);
 }
 
 extern "C" cdlm_status cdlm_code_thresh(cdlm_problem *p, double mult, double *thresh, int64_t *count) {
@@ -1283,12 +1640,14 @@ extern "C" cdlm_status cdlm_timing_read(cdlm_problem *p, int32_t kernel, double 
     if (!p || kernel < 0 || kernel > 3) return CDLM_ERR_ARG;
     CDLM_HIP(p->ctx, hipStreamSynchronize(p->ctx->stream));
     double t = 0.0;
+    int64_t cnt = 0;
     for (auto &e : p->ev[kernel]) {
         float ms = 0.f;
-        CDLM_HIP(p->ctx, hipEventElapsedTime(&ms, e.first, e.second));
+        CDLM_HIP(p->ctx, hipEventElapsedTime(&ms, e.a, e.b));
         t += ms;
+        cnt += e.count;
     }
     if (total_ms) *total_ms = t;
-    if (launches) *launches = (int64_t)p->ev[kernel].size();
+    if (launches) *launches = cnt;
     return CDLM_OK;
 }

@@ -177,3 +177,21 @@ def test_bad_arguments_raise():
         cdlm.CDL(np.zeros((3, 16), np.float32), [18], 1.0, 1.0, 0.1)     # filter_len not a multiple of 4
     with pytest.raises(cdlm.CdlmError):
         cdlm.CDL(np.zeros((3, 16), np.float32), [36], 1.0, 1.0, 0.1)     # beyond 32 taps
+
+
+@pytest.mark.parametrize("M", [8, 40])
+def test_resident_csc_matches_streaming(M, monkeypatch):
+    """The register-resident CSC kernel (opt-in) and the streaming kernel compute the same
+    iteration (different summation order of the z partials): equal to 1e-5 after 2 outer
+    iterations; M=40 uses a 3-member exchange group."""
+    fl = [16] * M
+    s, D0, X0, _ = _synth(31, 20, 40, M, fl)
+    monkeypatch.setenv("CDLM_CSC_RESIDENT", "0")
+    a = cdlm.CDL(s, fl, 1.0, 100.0, 0.05, D0=D0, X0=X0)
+    monkeypatch.setenv("CDLM_CSC_RESIDENT", "1")
+    b = cdlm.CDL(s, fl, 1.0, 100.0, 0.05, D0=D0, X0=X0)
+    ta = a.optimize(2, 7, 1)
+    tb = b.optimize(2, 7, 1)
+    np.testing.assert_allclose(ta, tb, rtol=1e-5)
+    for f in ("X", "DX", "y2", "y3", "d", "G1"):
+        assert rel(a.get(f), b.get(f)) < 1e-5, f
