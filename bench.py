@@ -66,6 +66,75 @@ def cpu_baseline(sample_n: int):
                       % (sample_n, L_BASES, M_FILT, FLEN, CSC_ITER, CDL_ITER, dt)}
 
 
+C3_ITER_NUM, C3_CSC_ITER, C3_CDL_ITER = 10, 15, 1
+
+
+def batched_leg(n_datasets: int, log):
+    """Batched workload sample: n_datasets datasets drawn as in the batched configuration
+    (N in [200, 20000], L in {100..500}, M in {32, 64}, filter_len in {16, 20, 24}), each
+    trained once (10 x (15 CSC + update_best + 1 dict) = 150 ADMM iterations, code
+    threshold).  Plain: one dataset at a time through the single-problem entry points.
+    Batched: datasets packed by device footprint into waves; one batch training (a captured
+    HIP graph) per wave.  Both include problem creation (data upload) and teardown."""
+    import numpy as np
+    import cdlm
+    from cdlm import synth
+    specs = synth.c3_specs(n_datasets)
+    data = [synth.c3_onehot(sp) for sp in specs]
+    ctx = cdlm.default_context()
+
+    def make(i):
+        sp = specs[i]
+        return cdlm.CDL(data[i], [sp["filter_len"]] * sp["M"], RHO, SIGMA, LAM, seed=sp["seed"], ctx=ctx)
+
+    # first use of every kernel variant outside the timed regions
+    warm = [cdlm.CDL(np.zeros((4, 4 * 8), np.float32), [f] * 2, RHO, SIGMA, LAM, seed=1, ctx=ctx) for f in (16, 20, 24)]
+    wb = cdlm.Batch(warm)
+    wb.train(1, 1, 1)
+    wb.close()
+    for w in warm:
+        w.optimize(1, 1, 1)
+        w.close()
+
+    t0 = time.perf_counter()
+    th_plain = []
+    for i in range(len(specs)):
+        c = make(i)
+        c.optimize(C3_ITER_NUM, C3_CSC_ITER, C3_CDL_ITER)
+        th_plain.append(c.code_thresh(1.0)[0])
+        c.close()
+    t_plain = time.perf_counter() - t0
+    log("bench: batched workload, plain loop: %d datasets in %.2f s" % (len(specs), t_plain))
+
+    free, _total = ctx.mem_info()
+    budget = int(free * 0.8)
+    sizes = [cdlm.footprint_bytes(sp["L"], sp["N"], sp["M"], sp["filter_len"]) for sp in specs]
+    waves = cdlm.pack_by_footprint(sizes, budget)
+    th_batch = [None] * len(specs)
+    t0 = time.perf_counter()
+    for w in waves:
+        probs = [make(i) for i in w]
+        b = cdlm.Batch(probs)
+        _tr, th, _cnt = b.train(C3_ITER_NUM, C3_CSC_ITER, C3_CDL_ITER, 1.0)
+        b.close()
+        for p in probs:
+            p.close()
+        for j, i in enumerate(w):
+            th_batch[i] = float(th[j])
+    t_batch = time.perf_counter() - t0
+    log("bench: batched workload, batched: %d waves, %.2f s" % (len(waves), t_batch))
+    seq_iters = sum(sp["N"] for sp in specs) * C3_ITER_NUM * C3_CSC_ITER
+    return {
+        "workload": "%d datasets of the batched configuration (seeds 3..%d), 150 ADMM iterations each"
+                    % (len(specs), 2 + len(specs)),
+        "datasets": len(specs), "waves": len(waves), "wave_budget_bytes": budget,
+        "plain_datasets_per_s": len(specs) / t_plain, "batched_datasets_per_s": len(specs) / t_batch,
+        "speedup": t_plain / t_batch,
+        "plain_seq_iters_per_s": seq_iters / t_plain, "batched_seq_iters_per_s": seq_iters / t_batch,
+        "thresholds_identical": bool(np.array_equal(np.array(th_plain), np.array(th_batch), equal_nan=True)),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +143,7 @@ def main():
     ap.add_argument("--n", type=int, default=N_SEQ, help="sequences (default: the workload's 20000)")
     ap.add_argument("--cpu-sample", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-datasets", type=int, default=24, help="datasets in the batched leg (0: skip it)")
     ap.add_argument("--outer", type=int, default=ITER_NUM, help="outer iterations per step (profiling runs only)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB GiB (HBM counter calibration) and exit")
@@ -168,6 +238,9 @@ def main():
                        "dict_reduce": red_ms / max(red_n, 1)},
     }
     c.close()
+    del c
+    if args.batch_datasets > 0:
+        out["batched"] = batched_leg(args.batch_datasets, lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu:
         print("bench: CPU baseline on a %d-sequence slice" % args.cpu_sample, file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample)

@@ -77,6 +77,14 @@ def load():
         "cdlm_timing_enable": ([vp, i32], C.c_int),
         "cdlm_timing_read": ([vp, i32, P(f64), P(i64)], C.c_int),
         "cdlm_copy_bench": ([vp, C.c_size_t, i32, P(f64)], C.c_int),
+        "cdlm_batch_create": ([vp, P(vp), i32, P(vp)], C.c_int),
+        "cdlm_batch_destroy": ([vp], C.c_int),
+        "cdlm_batch_set_graph": ([vp, i32], C.c_int),
+        "cdlm_batch_reset": ([vp], C.c_int),
+        "cdlm_batch_optimize": ([vp, i32, i32, i32, vp], C.c_int),
+        "cdlm_batch_train": ([vp, i32, i32, i32, f64, vp, vp, vp], C.c_int),
+        "cdlm_batch_timing_enable": ([vp, i32], C.c_int),
+        "cdlm_batch_timing_read": ([vp, i32, P(f64), P(i64)], C.c_int),
         "cdlm_selftest_mfma": ([vp, vp, vp, vp, vp], C.c_int),
         "cdlm_pwm_hits": ([vp, vp, i32, i64, vp, vp, vp, i32, i32, i32, vp, vp, vp, vp, i64, P(i64)], C.c_int),
         "cdlm_pwm_argmax": ([vp, vp, i32, i64, vp, vp, vp, i32, i32, vp], C.c_int),

@@ -89,7 +89,7 @@ class CDL:
     # -- lifecycle
     def close(self):
         if getattr(self, "h", None):
-            load().cdlm_problem_destroy(self.h)
+            check(load().cdlm_problem_destroy(self.h), self.ctx.h)
             self.h = None
 
     def __del__(self):
@@ -162,6 +162,91 @@ class CDL:
         (src/WRAPPER/wrap.jl:12; src/SEARCH/store.jl:15-17)."""
         th, _ = self.code_thresh(mult)
         return self.get("Xbest_slice"), th
+
+
+class Batch:
+    """Several CDL problems trained together: one launch per kernel over all of them (and a
+    captured HIP graph of the whole schedule).  The reference trains one dataset per call of
+    cdl_optimize (src/WRAPPER/wrap.jl:4-12); each problem of a batch gets the results it
+    gets alone.  The batch keeps its problems alive; close() it before closing them."""
+
+    def __init__(self, problems: Sequence[CDL], graph: bool = True):
+        if not problems:
+            raise ValueError("a batch needs at least one problem")
+        self.ctx = problems[0].ctx
+        self.problems = list(problems)
+        arr = (C.c_void_p * len(self.problems))(*[p.h.value for p in self.problems])
+        h = C.c_void_p()
+        check(load().cdlm_batch_create(self.ctx.h, arr, len(self.problems), C.byref(h)), self.ctx.h)
+        self.h = h
+        if not graph:
+            self.set_graph(False)
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(load().cdlm_batch_destroy(self.h), self.ctx.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.problems)
+
+    def set_graph(self, enable: bool):
+        check(load().cdlm_batch_set_graph(self.h, 1 if enable else 0), self.ctx.h)
+
+    def reset(self):
+        check(load().cdlm_batch_reset(self.h), self.ctx.h)
+
+    def optimize(self, iter_num: int, csc_iter: int, cdl_iter: int) -> np.ndarray:
+        """(count, iter_num) objective traces."""
+        tr = np.zeros((len(self), max(iter_num, 1)), np.float64)
+        check(load().cdlm_batch_optimize(self.h, iter_num, csc_iter, cdl_iter, tr.ctypes.data), self.ctx.h)
+        return tr[:, :iter_num]
+
+    def train(self, iter_num: int, csc_iter: int, cdl_iter: int, mult: float = 1.0):
+        """One full training of every problem: (traces (count, iter_num), thresholds, counts)."""
+        n = len(self)
+        tr = np.zeros((n, max(iter_num, 1)), np.float64)
+        th = np.zeros(n, np.float64)
+        cnt = np.zeros(n, np.int64)
+        check(load().cdlm_batch_train(self.h, iter_num, csc_iter, cdl_iter, mult, tr.ctypes.data, th.ctypes.data,
+                                      cnt.ctypes.data), self.ctx.h)
+        return tr[:, :iter_num], th, cnt
+
+    def timing(self, enable: bool):
+        check(load().cdlm_batch_timing_enable(self.h, 1 if enable else 0), self.ctx.h)
+
+    def timing_read(self, kernel: int):
+        t, n = C.c_double(), C.c_int64()
+        check(load().cdlm_batch_timing_read(self.h, kernel, C.byref(t), C.byref(n)), self.ctx.h)
+        return t.value, n.value
+
+
+def footprint_bytes(L: int, N: int, M: int, max_filter_len: int, keep_x0: bool = False) -> int:
+    """Device bytes a problem of this shape holds (cdlm_footprint_bytes)."""
+    return int(load().cdlm_footprint_bytes(4 * L, N, M, max_filter_len, 1 if keep_x0 else 0))
+
+
+def pack_by_footprint(sizes: Sequence[int], budget: int):
+    """Groups of indices whose summed device footprint stays within budget (first fit,
+    largest first); an item larger than the budget gets a group of its own."""
+    order = sorted(range(len(sizes)), key=lambda i: -sizes[i])
+    groups, used = [], []
+    for i in order:
+        for g, u in enumerate(used):
+            if u + sizes[i] <= budget:
+                groups[g].append(i)
+                used[g] += sizes[i]
+                break
+        else:
+            groups.append([i])
+            used.append(sizes[i])
+    return [sorted(g) for g in groups]
 
 
 # ---- reference-named free functions ---------------------------------------------------

@@ -97,3 +97,29 @@ def pwm_workload(seed: int, N: int, L: int, K: int, len_range=(8, 24), gc: float
             acc = np.float32(acc + w[:, j].max())
         thresh[k] = np.float32(np.float32(0.8) * acc)
     return bases, pwms, lens, thresh
+
+
+def c3_specs(count: int, seed0: int = 3, N_range=(200, 20000)):
+    """Dataset descriptions of the batched workload: dataset i uses seed seed0+i; N uniform
+    in N_range, L from {100,150,200,300,500}, 1-6 planted motifs of length 8-15 at
+    occupancy U[0.1,0.4], M from {32,64}, flattened filter length from {16,20,24}."""
+    out = []
+    for i in range(count):
+        rng = np.random.default_rng(seed0 + i)
+        nm = int(rng.integers(1, 7))
+        out.append({
+            "seed": seed0 + i,
+            "N": int(rng.integers(N_range[0], N_range[1] + 1)),
+            "L": int(rng.choice([100, 150, 200, 300, 500])),
+            "M": int(rng.choice([32, 64])),
+            "filter_len": int(rng.choice([16, 20, 24])),
+            "motif_lens": [int(x) for x in rng.integers(8, 16, nm)],
+            "occupancy": [float(x) for x in rng.uniform(0.1, 0.4, nm)],
+        })
+    return out
+
+
+def c3_onehot(spec) -> np.ndarray:
+    """(N, 4L) one-hot data of one batched-workload dataset."""
+    bases, _ = planted_bases(spec["seed"], spec["N"], spec["L"], spec["motif_lens"], spec["occupancy"], alpha=0.5)
+    return onehot(bases)

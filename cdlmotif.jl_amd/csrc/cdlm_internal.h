@@ -130,7 +130,7 @@ __device__ __forceinline__ void fft_bfly(const float2 *__restrict__ a, float2 *_
 // SYNC is the team barrier.  Returns the buffer holding the (unscaled) results.
 // Forward: exp(-2 pi i kl/n); INV: exp(+2 pi i kl/n).  tw is the twiddle table (LDS).
 template <bool INV, typename Sync>
-__device__ float2 *team_fft(float2 *a, float2 *b, const FftPlan &P, const float2 *__restrict__ tw, int tid, int nthr,
+__device__ __forceinline__ float2 *team_fft(float2 *a, float2 *b, const FftPlan &P, const float2 *__restrict__ tw, int tid, int nthr,
                             Sync sync, int nb = 1) {
     const int n = P.n;
     const int tws = P.twstride;

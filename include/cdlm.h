@@ -131,6 +131,32 @@ cdlm_status cdlm_get(cdlm_problem *p, int32_t field, void *dst, size_t bytes);
  * 2 = best/objective pass, 3 = dictionary mean reduction. */
 cdlm_status cdlm_timing_enable(cdlm_problem *p, int32_t enable);
 cdlm_status cdlm_timing_read(cdlm_problem *p, int32_t kernel, double *total_ms, int64_t *launches);
+/* ---- batches of problems ----------------------------------------------------------------- */
+/* The reference trains one dataset per cdl struct and loops over datasets on the host
+ * (src/WRAPPER/wrap.jl:140-149 is called once per dataset).  A batch runs several problems
+ * (of any N, L, M and filter lengths, all on one context) with one launch per kernel over
+ * all of them; each problem's results are those it gets alone.  The problems stay owned by
+ * the caller and must outlive the batch (cdlm_problem_destroy refuses a problem that is
+ * still in a batch).  A problem must not appear twice in one batch. */
+typedef struct cdlm_batch cdlm_batch;
+cdlm_status cdlm_batch_create(cdlm_ctx *ctx, cdlm_problem *const *problems, int32_t count, cdlm_batch **out);
+cdlm_status cdlm_batch_destroy(cdlm_batch *b);
+/* enable != 0 (the default): a batch call replays a captured HIP graph of its launches
+ * (captured on first use of a schedule, reused while the schedule is unchanged). */
+cdlm_status cdlm_batch_set_graph(cdlm_batch *b, int32_t enable);
+/* cdlm_problem_reset of every problem. */
+cdlm_status cdlm_batch_reset(cdlm_batch *b);
+/* cdlm_optimize of every problem; obj_traces (may be NULL): count x iter_num, problem-major. */
+cdlm_status cdlm_batch_optimize(cdlm_batch *b, int32_t iter_num, int32_t csc_iter, int32_t cdl_iter, double *obj_traces);
+/* One full training of every problem (cdl_optimize, src/WRAPPER/wrap.jl:4-12): reset,
+ * the schedule, and the code thresholds mult * std(X_best[X_best .> 0]).  Outputs (each may
+ * be NULL): obj_traces count x iter_num, thresh[count], counts[count]. */
+cdlm_status cdlm_batch_train(cdlm_batch *b, int32_t iter_num, int32_t csc_iter, int32_t cdl_iter, double mult,
+                             double *obj_traces, double *thresh, int64_t *counts);
+/* Kernel timing of batch calls (as cdlm_timing_*); graphs are not used while it is on. */
+cdlm_status cdlm_batch_timing_enable(cdlm_batch *b, int32_t enable);
+cdlm_status cdlm_batch_timing_read(cdlm_batch *b, int32_t kernel, double *total_ms, int64_t *launches);
+
 /* Device copy of `bytes` bytes, `reps` times, on the context stream; average ms per copy.
  * A plain streaming kernel of known traffic, used to calibrate HBM byte counters. */
 cdlm_status cdlm_copy_bench(cdlm_ctx *ctx, size_t bytes, int32_t reps, double *ms_per_copy);
