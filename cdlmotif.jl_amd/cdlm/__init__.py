@@ -9,6 +9,7 @@ All compute runs in libcdlm.so on a gfx950 device.
 """
 from ._lib import CdlmError, CdlmOOM, load  # noqa: F401
 from .device import Context, default_context  # noqa: F401
-from .cdl import CDL, Batch, footprint_bytes, pack_by_footprint, CDL_setup, init_cdl, code_update, dict_update, update_best, objective, cdl_optimize  # noqa: F401
+from .cdl import CDL, Batch, footprint_bytes, pack_by_footprint, autotune, tuning_set, tuning_get, CDL_setup, init_cdl, code_update, dict_update, update_best, objective, cdl_optimize  # noqa: F401
+from .serve import Server  # noqa: F401
 from .scan import (pfm2pwm, pwm_complement, best_score, pack_pwms, scan_hits, scan_w_gpu,  # noqa: F401
                    overlapping_scan, onehot_to_bases)

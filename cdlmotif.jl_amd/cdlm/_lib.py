@@ -38,6 +38,31 @@ class cdlm_params(C.Structure):
                 ("lam", C.c_float), ("sherman_csc", C.c_float), ("sherman_cdl", C.c_float)]
 
 
+class cdlm_tuning(C.Structure):
+    _fields_ = [("fft_order", C.c_int32), ("fft4_order", C.c_int32), ("dict_batch", C.c_int32),
+                ("csc_slots", C.c_int32)]
+
+
+class cdlm_server_opts(C.Structure):
+    _fields_ = [("mem_budget", C.c_uint64), ("max_active", C.c_int32), ("csc_iter", C.c_int32), ("cdl_iter", C.c_int32),
+                ("use_graph", C.c_int32), ("fill_seqs", C.c_int32)]
+
+
+class cdlm_job_spec(C.Structure):
+    _fields_ = [("params", cdlm_params), ("s", C.c_void_p), ("L4", C.c_int32), ("N", C.c_int32), ("D0", C.c_void_p),
+                ("seed", C.c_uint64), ("iter_num", C.c_int32), ("thresh_mult", C.c_double), ("keep_codes", C.c_int32)]
+
+
+class cdlm_job_result(C.Structure):
+    _fields_ = [("latency_s", C.c_double), ("queue_s", C.c_double), ("thresh", C.c_double), ("count", C.c_int64),
+                ("obj_best", C.c_float), ("iter_num", C.c_int32), ("L", C.c_int32), ("M", C.c_int32), ("N", C.c_int32)]
+
+
+class cdlm_server_stats_t(C.Structure):
+    _fields_ = [("completed", C.c_int64), ("active", C.c_int64), ("queued", C.c_int64), ("ticks", C.c_int64),
+                ("p50_s", C.c_double), ("p99_s", C.c_double), ("bytes_in_use", C.c_uint64), ("budget_bytes", C.c_uint64)]
+
+
 def header_functions():
     """Names of the functions declared in include/cdlm.h."""
     txt = open(HEADER).read()
@@ -77,10 +102,24 @@ def load():
         "cdlm_timing_enable": ([vp, i32], C.c_int),
         "cdlm_timing_read": ([vp, i32, P(f64), P(i64)], C.c_int),
         "cdlm_copy_bench": ([vp, C.c_size_t, i32, P(f64)], C.c_int),
+        "cdlm_autotune": ([vp, i32, i32, i32, i32, P(cdlm_tuning)], C.c_int),
+        "cdlm_tuning_set": ([vp, i32, i32, i32, P(cdlm_tuning)], C.c_int),
+        "cdlm_tuning_get": ([vp, i32, i32, i32, P(cdlm_tuning)], C.c_int),
+        "cdlm_server_create": ([i32, P(cdlm_server_opts), P(vp)], C.c_int),
+        "cdlm_server_destroy": ([vp], C.c_int),
+        "cdlm_server_last_error": ([vp], C.c_char_p),
+        "cdlm_server_submit": ([vp, P(cdlm_job_spec), P(C.c_uint64)], C.c_int),
+        "cdlm_server_poll": ([vp, C.c_uint64, P(i32)], C.c_int),
+        "cdlm_server_wait": ([vp, C.c_uint64, i32, P(i32)], C.c_int),
+        "cdlm_server_result": ([vp, C.c_uint64, P(cdlm_job_result), vp, vp, C.c_size_t], C.c_int),
+        "cdlm_server_release": ([vp, C.c_uint64], C.c_int),
+        "cdlm_server_stats": ([vp, P(cdlm_server_stats_t)], C.c_int),
+        "cdlm_selftest_fft": ([vp, i32, i32, i32, i32, i32, vp, vp], C.c_int),
         "cdlm_batch_create": ([vp, P(vp), i32, P(vp)], C.c_int),
         "cdlm_batch_destroy": ([vp], C.c_int),
         "cdlm_batch_set_graph": ([vp, i32], C.c_int),
         "cdlm_batch_reset": ([vp], C.c_int),
+        "cdlm_batch_classes": ([vp, P(i32)], C.c_int),
         "cdlm_batch_optimize": ([vp, i32, i32, i32, vp], C.c_int),
         "cdlm_batch_train": ([vp, i32, i32, i32, f64, vp, vp, vp], C.c_int),
         "cdlm_batch_timing_enable": ([vp, i32], C.c_int),

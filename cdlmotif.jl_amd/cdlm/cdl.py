@@ -196,6 +196,13 @@ class Batch:
     def __len__(self):
         return len(self.problems)
 
+    @property
+    def classes(self) -> int:
+        """Shape classes (launch sets) the batch runs as."""
+        n = C.c_int32()
+        check(load().cdlm_batch_classes(self.h, C.byref(n)), self.ctx.h)
+        return n.value
+
     def set_graph(self, enable: bool):
         check(load().cdlm_batch_set_graph(self.h, 1 if enable else 0), self.ctx.h)
 
@@ -225,6 +232,36 @@ class Batch:
         t, n = C.c_double(), C.c_int64()
         check(load().cdlm_batch_timing_read(self.h, kernel, C.byref(t), C.byref(n)), self.ctx.h)
         return t.value, n.value
+
+
+def autotune(L: int, M: int, max_filter_len: int, n_seq: int = 0, ctx: Optional[Context] = None) -> dict:
+    """Measures the launch choices for problems of this shape and keeps the fastest for
+    problems created afterwards (cdlm_autotune); returns the choice."""
+    from ._lib import cdlm_tuning
+    ctx = ctx or default_context()
+    t = cdlm_tuning()
+    check(load().cdlm_autotune(ctx.h, 4 * L, M, max_filter_len, n_seq, C.byref(t)), ctx.h)
+    return {f: getattr(t, f) for f, _ in cdlm_tuning._fields_}
+
+
+def tuning_set(L: int, M: int, max_filter_len: int, choice: Optional[dict], ctx: Optional[Context] = None):
+    """Sets (dict with fft_order, fft4_order, dict_batch, csc_slots) or clears (None) the
+    launch choices for a shape."""
+    from ._lib import cdlm_tuning
+    ctx = ctx or default_context()
+    if choice is None:
+        check(load().cdlm_tuning_set(ctx.h, 4 * L, M, max_filter_len, None), ctx.h)
+        return
+    t = cdlm_tuning(**{f: int(choice.get(f, 0)) for f, _ in cdlm_tuning._fields_})
+    check(load().cdlm_tuning_set(ctx.h, 4 * L, M, max_filter_len, C.byref(t)), ctx.h)
+
+
+def tuning_get(L: int, M: int, max_filter_len: int, ctx: Optional[Context] = None) -> dict:
+    from ._lib import cdlm_tuning
+    ctx = ctx or default_context()
+    t = cdlm_tuning()
+    check(load().cdlm_tuning_get(ctx.h, 4 * L, M, max_filter_len, C.byref(t)), ctx.h)
+    return {f: getattr(t, f) for f, _ in cdlm_tuning._fields_}
 
 
 def footprint_bytes(L: int, N: int, M: int, max_filter_len: int, keep_x0: bool = False) -> int:

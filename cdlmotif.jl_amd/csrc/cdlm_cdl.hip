@@ -19,6 +19,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <tuple>
 
 #include "cdlm_cdl_dev.h"
 
@@ -72,6 +73,8 @@ struct LaunchSet {
     void *mem = nullptr;  // descriptors followed by the prefix arrays
     CdlDev *tab = nullptr;
     int *st_seq = nullptr, *st_filt = nullptr, *st_cs = nullptr, *st_red = nullptr;
+    int *slot = nullptr;  // position of each member in the caller's list (gathered outputs)
+    hipStream_t stream = nullptr;  // null: the context stream
     size_t lds_init = 0, lds_csc = 0, lds_best = 0, lds_dict = 0, lds_fin = 0, lds_g1 = 0;
     void release() {
         if (mem) (void)hipFree(mem);
@@ -134,6 +137,24 @@ cdlm_status ensure_lds(cdlm_ctx *ctx, const void *fn, size_t bytes) {
     return CDLM_OK;
 }
 
+// Launch choices of one problem shape (Lg, M, max filter length), set by the autotuner or
+// by the caller; kept per device for the life of the process.  Only the FFT stage orders
+// change rounding; the others change scheduling only.
+struct Tuning {
+    int order = kFftSmallFirst;   // stage order of the length-Lg transforms
+    int order4 = kFftSmallFirst;  // ... of the length-Lg/4 transforms (dictionary step)
+    int B = 0;                    // dictionary columns per pass (0: largest-occupancy choice)
+    int nsl = 0;                  // CSC float4 slots per lane (0: the fewest that cover Lg)
+};
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int>, Tuning> g_tune;
+
+Tuning tuning_of(int device, int Lg, int M, int gamma) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(std::make_tuple(device, Lg, M, gamma));
+    return it == g_tune.end() ? Tuning() : it->second;
+}
+
 cdlm_status lcheck(cdlm_ctx *ctx, const char *what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return cdlm_fail(ctx, CDLM_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e));
@@ -152,13 +173,19 @@ struct cdlm_problem {
     CdlDev dv;           // descriptor with the device pointers
     LaunchSet set;       // this problem alone
     Timer timer;
+    Tuning tune;         // launch choices in force for this problem
     int batches = 0;     // batches that reference this problem
 };
 
 struct cdlm_batch {
     cdlm_ctx *ctx = nullptr;
     std::vector<cdlm_problem *> probs;
-    LaunchSet set;
+    // one launch set per shape class (members sharing slots per lane, dictionary batch,
+    // filter taps and LDS occupancy); in a graph each class is a concurrent branch
+    std::vector<LaunchSet> sets;
+    std::vector<hipStream_t> streams;  // capture streams of classes 1.. (class 0: the context stream)
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> joins;
     Timer timer;
     bool use_graph = true;
     // one cached graph: (reset, iter_num, csc_iter, cdl_iter, gather count)
@@ -230,7 +257,8 @@ void free_problem(cdlm_problem *p) {
 }
 
 // Builds the launch set of np problems (all on ctx): maxima, LDS sizes, device table.
-cdlm_status set_build(cdlm_ctx *ctx, cdlm_problem *const *ps, int np, LaunchSet *S) {
+// slots[i] (identity when null) is member i's position in the caller's list.
+cdlm_status set_build(cdlm_ctx *ctx, cdlm_problem *const *ps, int np, LaunchSet *S, const int *slots = nullptr) {
     LaunchSet T;
     T.np = np;
     for (int i = 0; i < np; ++i) {
@@ -239,8 +267,17 @@ cdlm_status set_build(cdlm_ctx *ctx, cdlm_problem *const *ps, int np, LaunchSet 
     }
     const int Lg = T.LgMax, taps = T.taps;
     if (Lg > 2048) return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "padded length %d > 2048", Lg);
+    // slots per lane: enough for the longest member, more if a member's tuning asks;
+    // dictionary batch: the tuning of the longest member when it fits, else by occupancy
     T.nsl = (Lg / 4 <= 256) ? 4 : 8;
+    int lead = 0;
+    for (int i = 0; i < np; ++i) {
+        T.nsl = std::max(T.nsl, ps[i]->tune.nsl);
+        if (ps[i]->Lg > ps[lead]->Lg) lead = i;
+    }
     T.B = pick_dict_batch(Lg, taps);
+    const int tuned_B = ps[lead]->tune.B;
+    if ((tuned_B == 2 || tuned_B == 4) && lds_bytes_dict(Lg, taps, tuned_B) <= kLdsLimit - kThreads * sizeof(double)) T.B = tuned_B;
     if (T.B == 0) return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "padded length %d: dictionary step exceeds LDS", Lg);
     T.lds_csc = lds_bytes_csc(Lg, taps);
     T.lds_init = lds_bytes_init(Lg, taps);
@@ -278,7 +315,7 @@ cdlm_status set_build(cdlm_ctx *ctx, cdlm_problem *const *ps, int np, LaunchSet 
     if ((st = ensure_lds(ctx, (const void *)k_g1_spectrum, T.lds_g1)) != CDLM_OK) return st;
 
     std::vector<CdlDev> tab(np);
-    std::vector<int> pre(4 * (size_t)np);
+    std::vector<int> pre(5 * (size_t)np);
     long a_seq = 0, a_filt = 0, a_cs = 0, a_red = 0;
     for (int i = 0; i < np; ++i) {
         const cdlm_problem *p = ps[i];
@@ -287,6 +324,7 @@ cdlm_status set_build(cdlm_ctx *ctx, cdlm_problem *const *ps, int np, LaunchSet 
         pre[np + i] = (int)a_filt;
         pre[2 * np + i] = (int)a_cs;
         pre[3 * np + i] = (int)a_red;
+        pre[4 * np + i] = slots ? slots[i] : i;
         const long LM = (long)p->Lg * p->M;
         a_seq += p->N;
         a_filt += p->M;
@@ -311,6 +349,7 @@ cdlm_status set_build(cdlm_ctx *ctx, cdlm_problem *const *ps, int np, LaunchSet 
     T.st_filt = pr + np;
     T.st_cs = pr + 2 * np;
     T.st_red = pr + 3 * np;
+    T.slot = pr + 4 * np;
     e = hipMemcpyAsync(T.tab, tab.data(), sizeof(CdlDev) * np, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(pr, pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -323,8 +362,10 @@ cdlm_status set_build(cdlm_ctx *ctx, cdlm_problem *const *ps, int np, LaunchSet 
 }
 
 // ---------------------------------------------------------------------------- launches
+hipStream_t set_stream(cdlm_ctx *ctx, const LaunchSet &S) { return S.stream ? S.stream : ctx->stream; }
+
 cdlm_status run_g1_cs(cdlm_ctx *ctx, const LaunchSet &S, int flip) {
-    hipStream_t st = ctx->stream;
+    hipStream_t st = set_stream(ctx, S);
     if (!flip)
         hipLaunchKernelGGL(k_g1_spectrum, dim3(S.blk_filt), dim3(kThreads), S.lds_g1, st, S.tab, S.st_filt, S.np, S.LgMax);
     hipLaunchKernelGGL(k_cs, dim3(S.blk_cs), dim3(kThreads), 0, st, S.tab, S.st_cs, S.np, flip);
@@ -334,7 +375,7 @@ cdlm_status run_g1_cs(cdlm_ctx *ctx, const LaunchSet &S, int flip) {
 // Objective partials, their reduction and the update_best decision.  decide = 0 only
 // records the objective; slot < 0 writes no trace entry.
 cdlm_status run_best(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm, int force, int decide, int slot, float *Xfull) {
-    hipStream_t st = ctx->stream;
+    hipStream_t st = set_stream(ctx, S);
     {
         TimedLaunch tl(tm, st, 2);
 #define L_BEST(T) hipLaunchKernelGGL(k_best<T>, dim3(S.blk_seq), dim3(kThreads), S.lds_best, st, S.tab, S.st_seq, S.np, Xfull)
@@ -347,13 +388,14 @@ cdlm_status run_best(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm, int force, in
 }
 
 cdlm_status run_csc(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm) {
+    hipStream_t st = set_stream(ctx, S);
     {
-        TimedLaunch tl(tm, ctx->stream, 0);
+        TimedLaunch tl(tm, st, 0);
 #define L_CSC(T)                                                                                                     \
     if (S.nsl == 4)                                                                                                  \
-        hipLaunchKernelGGL((k_csc<T, 4>), dim3(S.blk_seq), dim3(kThreads), S.lds_csc, ctx->stream, S.tab, S.st_seq, S.np); \
+        hipLaunchKernelGGL((k_csc<T, 4>), dim3(S.blk_seq), dim3(kThreads), S.lds_csc, st, S.tab, S.st_seq, S.np);    \
     else                                                                                                             \
-        hipLaunchKernelGGL((k_csc<T, 8>), dim3(S.blk_seq), dim3(kThreads), S.lds_csc, ctx->stream, S.tab, S.st_seq, S.np)
+        hipLaunchKernelGGL((k_csc<T, 8>), dim3(S.blk_seq), dim3(kThreads), S.lds_csc, st, S.tab, S.st_seq, S.np)
         CDLM_TAPS_SWITCH(S.taps, L_CSC);
 #undef L_CSC
     }
@@ -361,7 +403,7 @@ cdlm_status run_csc(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm) {
 }
 
 cdlm_status run_dict(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm) {
-    hipStream_t st = ctx->stream;
+    hipStream_t st = set_stream(ctx, S);
     {
         TimedLaunch tl(tm, st, 1);
 #define L_DICT(T) hipLaunchKernelGGL(k_dict<T>, dim3(S.blk_seq), dim3(kThreads), S.lds_dict, st, S.tab, S.st_seq, S.np, S.B, S.LgMax)
@@ -381,7 +423,7 @@ cdlm_status run_dict(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm) {
 // The constructor's device work (CDL_setup.jl:183-236): filters, |G1h|^2, codes, DX, and
 // the forced first update_best.
 cdlm_status run_reset(cdlm_ctx *ctx, const LaunchSet &S) {
-    hipStream_t st = ctx->stream;
+    hipStream_t st = set_stream(ctx, S);
     hipLaunchKernelGGL(k_reset, dim3(S.blk_filt), dim3(kThreads), 0, st, S.tab, S.st_filt, S.np);
     cdlm_status s = run_g1_cs(ctx, S, 0);
     if (s != CDLM_OK) return s;
@@ -487,7 +529,8 @@ extern "C" cdlm_status cdlm_problem_create(cdlm_ctx *ctx, const cdlm_params *prm
     d.rho = prm->rho; d.sigma = prm->sigma; d.lambda = prm->lambda;
     d.sh_csc = prm->sherman_csc; d.sh_cdl = prm->sherman_cdl;
     d.seed = seed;
-    if (!make_fft_plan(Lg, 1, &d.plan) || !make_fft_plan(Lg / 4, 4, &d.plan4))
+    p->tune = tuning_of(ctx->device, Lg, M, gamma);
+    if (!make_fft_plan(Lg, 1, &d.plan, p->tune.order) || !make_fft_plan(Lg / 4, 4, &d.plan4, p->tune.order4))
         return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "no FFT plan for length %d", Lg);
 
     // host-side masked D0 (P_D .* D, CDL_setup.jl:183)
@@ -734,14 +777,50 @@ extern "C" cdlm_status cdlm_timing_read(cdlm_problem *p, int32_t kernel, double 
 // ============================================================================ C ABI: batch
 namespace {
 
+// Shape class of a problem run alone: members of one class share a launch set without
+// lowering each other's occupancy.
+struct ShapeClass {
+    int nsl, B, taps, occ_csc, occ_dict;
+    bool operator<(const ShapeClass &o) const {
+        return std::tie(nsl, B, taps, occ_csc, occ_dict) < std::tie(o.nsl, o.B, o.taps, o.occ_csc, o.occ_dict);
+    }
+};
+
+ShapeClass shape_class(const cdlm_problem *p) {
+    ShapeClass c;
+    c.nsl = std::max((p->Lg / 4 <= 256) ? 4 : 8, p->tune.nsl);
+    c.B = pick_dict_batch(p->Lg, p->gamma);
+    if ((p->tune.B == 2 || p->tune.B == 4) && lds_bytes_dict(p->Lg, p->gamma, p->tune.B) <= kLdsLimit - kThreads * sizeof(double))
+        c.B = p->tune.B;
+    c.taps = p->gamma;
+    c.occ_csc = std::min<int>(c.nsl == 4 ? 3 : 2, (int)(kLdsLimit / lds_bytes_csc(p->Lg, p->gamma)));
+    c.occ_dict = std::min<int>(8, (int)(kLdsLimit / lds_bytes_dict(p->Lg, p->gamma, c.B)));
+    return c;
+}
+
 void drop_graph(cdlm_batch *b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     b->exec = nullptr;
     for (int &k : b->key) k = -1;
 }
 
+void release_batch(cdlm_batch *b) {
+    drop_graph(b);
+    b->timer.clear();
+    if (b->stage) (void)hipFree(b->stage);
+    b->stage = nullptr;
+    for (auto &S : b->sets) S.release();
+    b->sets.clear();
+    for (auto s : b->streams) (void)hipStreamDestroy(s);
+    b->streams.clear();
+    for (auto e : b->joins) (void)hipEventDestroy(e);
+    b->joins.clear();
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    b->fork = nullptr;
+}
+
 cdlm_status ensure_stage(cdlm_batch *b, int cnt) {
-    const size_t need = (size_t)b->set.np * (cnt + kGatherExtra);
+    const size_t need = b->probs.size() * (size_t)(cnt + kGatherExtra);
     if (need > b->stage_cap) {
         drop_graph(b);  // a cached graph holds the old pointer
         if (b->stage) (void)hipFree(b->stage);
@@ -759,13 +838,36 @@ cdlm_status ensure_stage(cdlm_batch *b, int cnt) {
     return CDLM_OK;
 }
 
-// The device work of one batch call: [reset], schedule, gather of cnt trace entries.
-cdlm_status batch_enqueue(cdlm_batch *b, int reset, int iter_num, int csc_iter, int cdl_iter, int cnt, Timer *tm) {
+// The device work of one class: [reset], schedule, gather of cnt trace entries.
+cdlm_status class_enqueue(cdlm_batch *b, const LaunchSet &S, int reset, int iter_num, int csc_iter, int cdl_iter, int cnt,
+                          Timer *tm) {
     cdlm_status st;
-    if (reset && (st = run_reset(b->ctx, b->set)) != CDLM_OK) return st;
-    if ((st = run_schedule(b->ctx, b->set, tm, iter_num, csc_iter, cdl_iter)) != CDLM_OK) return st;
-    hipLaunchKernelGGL(k_gather, dim3(b->set.np), dim3(64), 0, b->ctx->stream, b->set.tab, b->set.np, cnt, b->stage);
+    if (reset && (st = run_reset(b->ctx, S)) != CDLM_OK) return st;
+    if ((st = run_schedule(b->ctx, S, tm, iter_num, csc_iter, cdl_iter)) != CDLM_OK) return st;
+    hipLaunchKernelGGL(k_gather, dim3(S.np), dim3(64), 0, set_stream(b->ctx, S), S.tab, S.slot, S.np, cnt, b->stage);
     return lcheck(b->ctx, "k_gather");
+}
+
+// Eager: classes one after another on the context stream.  Captured: class 0 on the
+// context stream, the others on their own streams forked from and joined back into it.
+cdlm_status batch_enqueue(cdlm_batch *b, bool concurrent, int reset, int iter_num, int csc_iter, int cdl_iter, int cnt,
+                          Timer *tm) {
+    cdlm_ctx *ctx = b->ctx;
+    const int nc = (int)b->sets.size();
+    if (concurrent && nc > 1) CDLM_HIP(ctx, hipEventRecord(b->fork, ctx->stream));
+    for (int c = 0; c < nc; ++c) {
+        LaunchSet &S = b->sets[c];
+        S.stream = (concurrent && c > 0) ? b->streams[c - 1] : nullptr;
+        if (S.stream) CDLM_HIP(ctx, hipStreamWaitEvent(S.stream, b->fork, 0));
+        cdlm_status st = class_enqueue(b, S, reset, iter_num, csc_iter, cdl_iter, cnt, tm);
+        if (S.stream) {
+            CDLM_HIP(ctx, hipEventRecord(b->joins[c - 1], S.stream));
+            CDLM_HIP(ctx, hipStreamWaitEvent(ctx->stream, b->joins[c - 1], 0));
+        }
+        S.stream = nullptr;
+        if (st != CDLM_OK) return st;
+    }
+    return CDLM_OK;
 }
 
 // Runs the batch work, from the cached graph when graphs are on and timing is off.
@@ -775,14 +877,14 @@ cdlm_status batch_run(cdlm_batch *b, int reset, int iter_num, int csc_iter, int 
     if (st != CDLM_OK) return st;
     hipStream_t s = b->ctx->stream;
     if (!b->use_graph || b->timer.on) {
-        if ((st = batch_enqueue(b, reset, iter_num, csc_iter, cdl_iter, cnt, &b->timer)) != CDLM_OK) return st;
+        if ((st = batch_enqueue(b, false, reset, iter_num, csc_iter, cdl_iter, cnt, &b->timer)) != CDLM_OK) return st;
     } else {
         const int key[5] = {reset, iter_num, csc_iter, cdl_iter, cnt};
         if (!b->exec || memcmp(key, b->key, sizeof(key)) != 0) {
             drop_graph(b);
             hipGraph_t g = nullptr;
             CDLM_HIP(b->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            cdlm_status cst = batch_enqueue(b, reset, iter_num, csc_iter, cdl_iter, cnt, nullptr);
+            cdlm_status cst = batch_enqueue(b, true, reset, iter_num, csc_iter, cdl_iter, cnt, nullptr);
             hipError_t e = hipStreamEndCapture(s, &g);
             if (cst != CDLM_OK) {
                 if (g) (void)hipGraphDestroy(g);
@@ -799,15 +901,15 @@ cdlm_status batch_run(cdlm_batch *b, int reset, int iter_num, int csc_iter, int 
         }
         CDLM_HIP(b->ctx, hipGraphLaunch(b->exec, s));
     }
-    CDLM_HIP(b->ctx, hipMemcpyAsync(b->host.data(), b->stage, (size_t)b->set.np * (cnt + kGatherExtra) * sizeof(double),
+    CDLM_HIP(b->ctx, hipMemcpyAsync(b->host.data(), b->stage, b->probs.size() * (cnt + kGatherExtra) * sizeof(double),
                                     hipMemcpyDeviceToHost, s));
     return sync_ctx(b->ctx);
 }
 
 void batch_outputs(cdlm_batch *b, int cnt, double mult, double *obj_traces, double *thresh, int64_t *counts) {
-    for (int i = 0; i < b->set.np; ++i) {
-        const double *o = b->host.data() + (size_t)i * (cnt + kGatherExtra);
-        if (obj_traces && cnt > 0) memcpy(obj_traces + (size_t)i * cnt, o, cnt * sizeof(double));
+    for (size_t i = 0; i < b->probs.size(); ++i) {
+        const double *o = b->host.data() + i * (cnt + kGatherExtra);
+        if (obj_traces && cnt > 0) memcpy(obj_traces + i * cnt, o, cnt * sizeof(double));
         if (thresh) thresh[i] = thresh_of(o[cnt + 1], o[cnt + 2], o[cnt + 3], mult);
         if (counts) counts[i] = (int64_t)o[cnt + 1];
     }
@@ -828,8 +930,32 @@ extern "C" cdlm_status cdlm_batch_create(cdlm_ctx *ctx, cdlm_problem *const *pro
     std::unique_ptr<cdlm_batch> b(new cdlm_batch());
     b->ctx = ctx;
     b->probs.assign(problems, problems + count);
-    cdlm_status st = set_build(ctx, problems, count, &b->set);
-    if (st != CDLM_OK) return st;
+    std::map<ShapeClass, std::vector<int>> classes;
+    for (int i = 0; i < count; ++i) classes[shape_class(problems[i])].push_back(i);
+    cdlm_status st = CDLM_OK;
+    for (auto &kv : classes) {
+        std::vector<cdlm_problem *> ps;
+        for (int i : kv.second) ps.push_back(problems[i]);
+        LaunchSet S;
+        if ((st = set_build(ctx, ps.data(), (int)ps.size(), &S, kv.second.data())) != CDLM_OK) {
+            release_batch(b.get());
+            return st;
+        }
+        b->sets.push_back(S);
+    }
+    hipError_t e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    for (size_t c = 1; c < b->sets.size() && e == hipSuccess; ++c) {
+        hipStream_t s = nullptr;
+        hipEvent_t j = nullptr;
+        e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        if (e == hipSuccess) b->streams.push_back(s);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&j, hipEventDisableTiming);
+        if (e == hipSuccess) b->joins.push_back(j);
+    }
+    if (e != hipSuccess) {
+        release_batch(b.get());
+        return cdlm_fail(ctx, CDLM_ERR_HIP, "batch streams: %s", hipGetErrorString(e));
+    }
     for (auto *p : b->probs) ++p->batches;
     *out = b.release();
     return CDLM_OK;
@@ -838,10 +964,7 @@ extern "C" cdlm_status cdlm_batch_create(cdlm_ctx *ctx, cdlm_problem *const *pro
 extern "C" cdlm_status cdlm_batch_destroy(cdlm_batch *b) {
     if (!b) return CDLM_OK;
     (void)hipStreamSynchronize(b->ctx->stream);
-    drop_graph(b);
-    b->timer.clear();
-    if (b->stage) (void)hipFree(b->stage);
-    b->set.release();
+    release_batch(b);
     for (auto *p : b->probs) --p->batches;
     delete b;
     return CDLM_OK;
@@ -854,10 +977,18 @@ extern "C" cdlm_status cdlm_batch_set_graph(cdlm_batch *b, int32_t enable) {
     return CDLM_OK;
 }
 
+extern "C" cdlm_status cdlm_batch_classes(const cdlm_batch *b, int32_t *count) {
+    if (!b || !count) return CDLM_ERR_ARG;
+    *count = (int32_t)b->sets.size();
+    return CDLM_OK;
+}
+
 extern "C" cdlm_status cdlm_batch_reset(cdlm_batch *b) {
     if (!b) return CDLM_ERR_ARG;
-    cdlm_status st = run_reset(b->ctx, b->set);
-    if (st != CDLM_OK) return st;
+    for (auto &S : b->sets) {
+        cdlm_status st = run_reset(b->ctx, S);
+        if (st != CDLM_OK) return st;
+    }
     return sync_ctx(b->ctx);
 }
 
@@ -894,4 +1025,188 @@ extern "C" cdlm_status cdlm_batch_timing_enable(cdlm_batch *b, int32_t enable) {
 extern "C" cdlm_status cdlm_batch_timing_read(cdlm_batch *b, int32_t kernel, double *total_ms, int64_t *launches) {
     if (!b || kernel < 0 || kernel > 3) return CDLM_ERR_ARG;
     return timing_read(b->ctx, b->timer, kernel, total_ms, launches);
+}
+
+// ============================================================================ C ABI: FFT self-test
+extern "C" cdlm_status cdlm_selftest_fft(cdlm_ctx *ctx, int32_t n, int32_t twstride, int32_t nb, int32_t inverse,
+                                         int32_t order, const float *in, float *out) {
+    if (!ctx) return CDLM_ERR_ARG;
+    if (!in || !out || n < 1 || twstride < 1 || nb < 1 || order < 0 || order >= kFftOrders)
+        return cdlm_fail(ctx, CDLM_ERR_ARG, "bad FFT self-test arguments");
+    FftPlan plan;
+    if (!make_fft_plan(n, twstride, &plan, order)) return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "no plan for %d", n);
+    const int N = n * twstride;
+    const size_t lds = sizeof(float2) * ((size_t)N + 2 * (size_t)nb * n);
+    if (lds > kLdsLimit) return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "FFT self-test needs %zu B of LDS", lds);
+    cdlm_status st = ensure_lds(ctx, (const void *)k_fft_selftest, lds);
+    if (st != CDLM_OK) return st;
+    std::vector<float2> tw(N);
+    for (int t = 0; t < N; ++t) {
+        const double ang = -2.0 * M_PI * (double)t / (double)N;
+        tw[t] = make_float2((float)cos(ang), (float)sin(ang));
+    }
+    float2 *d = nullptr;
+    const size_t cnt = (size_t)nb * n;
+    CDLM_HIP(ctx, hipMalloc(&d, sizeof(float2) * (2 * cnt + N)));
+    hipError_t e = hipMemcpyAsync(d, in, sizeof(float2) * cnt, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d + 2 * cnt, tw.data(), sizeof(float2) * N, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_fft_selftest, dim3(1), dim3(kThreads), lds, ctx->stream, d, d + cnt, d + 2 * cnt, plan, nb,
+                           inverse ? 1 : 0);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d + cnt, sizeof(float2) * cnt, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return cdlm_fail(ctx, CDLM_ERR_HIP, "FFT self-test: %s", hipGetErrorString(e));
+    return CDLM_OK;
+}
+
+// ============================================================================ C ABI: tuning
+namespace {
+cdlm_status tune_args(cdlm_ctx *ctx, int32_t L4, int32_t M, int32_t gamma) {
+    if (L4 <= 0 || L4 % 4 || M <= 0 || gamma <= 0 || gamma % 4 || gamma > kMaxTaps || L4 + 2 * gamma > 2048)
+        return cdlm_fail(ctx, CDLM_ERR_ARG, "bad tuning shape (L4=%d, M=%d, gamma=%d)", L4, M, gamma);
+    return CDLM_OK;
+}
+}  // namespace
+
+extern "C" cdlm_status cdlm_tuning_set(cdlm_ctx *ctx, int32_t L4, int32_t M, int32_t gamma, const cdlm_tuning *t) {
+    if (!ctx) return CDLM_ERR_ARG;
+    cdlm_status st = tune_args(ctx, L4, M, gamma);
+    if (st != CDLM_OK) return st;
+    const auto key = std::make_tuple(ctx->device, L4 + 2 * gamma, (int)M, (int)gamma);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (!t) {
+        g_tune.erase(key);
+        return CDLM_OK;
+    }
+    if (t->fft_order < 0 || t->fft_order >= kFftOrders || t->fft4_order < 0 || t->fft4_order >= kFftOrders ||
+        (t->dict_batch != 0 && t->dict_batch != 2 && t->dict_batch != 4) ||
+        (t->csc_slots != 0 && t->csc_slots != 4 && t->csc_slots != 8))
+        return cdlm_fail(ctx, CDLM_ERR_ARG, "bad tuning values");
+    Tuning tu;
+    tu.order = t->fft_order;
+    tu.order4 = t->fft4_order;
+    tu.B = t->dict_batch;
+    tu.nsl = t->csc_slots;
+    g_tune[key] = tu;
+    return CDLM_OK;
+}
+
+extern "C" cdlm_status cdlm_tuning_get(cdlm_ctx *ctx, int32_t L4, int32_t M, int32_t gamma, cdlm_tuning *out) {
+    if (!ctx || !out) return CDLM_ERR_ARG;
+    cdlm_status st = tune_args(ctx, L4, M, gamma);
+    if (st != CDLM_OK) return st;
+    const Tuning tu = tuning_of(ctx->device, L4 + 2 * gamma, M, gamma);
+    out->fft_order = tu.order;
+    out->fft4_order = tu.order4;
+    out->dict_batch = tu.B;
+    out->csc_slots = tu.nsl;
+    return CDLM_OK;
+}
+
+// Times the candidate launch choices on a scratch problem of n_seq random sequences and
+// keeps the fastest for one outer iteration of 15 CSC iterations and 1 dictionary
+// iteration (the reference's default schedule, WRAPPER_setup.jl:8-10).
+extern "C" cdlm_status cdlm_autotune(cdlm_ctx *ctx, int32_t L4, int32_t M, int32_t gamma, int32_t n_seq,
+                                     cdlm_tuning *out) {
+    if (!ctx) return CDLM_ERR_ARG;
+    cdlm_status st = tune_args(ctx, L4, M, gamma);
+    if (st != CDLM_OK) return st;
+    if (n_seq <= 0) n_seq = 3 * (ctx->num_cu > 0 ? ctx->num_cu : 256);
+    const int Lg = L4 + 2 * gamma;
+    // scratch data: one-hot columns from a fixed LCG
+    std::vector<float> s((size_t)L4 * n_seq, 0.f);
+    uint64_t x = 0x2545F4914F6CDD1DULL;
+    for (size_t i = 0; i < s.size(); i += 4) {
+        x = x * 6364136223846793005ULL + 1442695040888963407ULL;
+        s[i + ((x >> 33) & 3)] = 1.f;
+    }
+    std::vector<int32_t> fl(M, gamma);
+    cdlm_params prm;
+    prm.M = M; prm.filter_len = fl.data(); prm.rho = 1.f; prm.sigma = 100.f; prm.lambda = 0.05f;
+    prm.sherman_csc = 2.f; prm.sherman_cdl = 2.f;
+    cdlm_problem *p = nullptr;
+    if ((st = cdlm_problem_create(ctx, &prm, s.data(), L4, n_seq, nullptr, nullptr, 7, &p)) != CDLM_OK) return st;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+        (void)hipGetLastError();
+        cdlm_problem_destroy(p);
+        return cdlm_fail(ctx, CDLM_ERR_HIP, "event creation failed");
+    }
+    auto timed = [&](bool dict, int reps, float *ms) -> cdlm_status {
+        cdlm_status r = dict ? run_dict(ctx, p->set, nullptr) : run_csc(ctx, p->set, nullptr);   // warm-up
+        if (r != CDLM_OK) return r;
+        CDLM_HIP(ctx, hipEventRecord(a, ctx->stream));
+        for (int i = 0; i < reps && r == CDLM_OK; ++i) r = dict ? run_dict(ctx, p->set, nullptr) : run_csc(ctx, p->set, nullptr);
+        if (r != CDLM_OK) return r;
+        CDLM_HIP(ctx, hipEventRecord(b, ctx->stream));
+        CDLM_HIP(ctx, hipEventSynchronize(b));
+        CDLM_HIP(ctx, hipEventElapsedTime(ms, a, b));
+        *ms /= reps;
+        return CDLM_OK;
+    };
+    auto apply = [&](const Tuning &tu) -> cdlm_status {
+        p->tune = tu;
+        if (!make_fft_plan(Lg, 1, &p->dv.plan, tu.order) || !make_fft_plan(Lg / 4, 4, &p->dv.plan4, tu.order4))
+            return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "no FFT plan for length %d", Lg);
+        (void)hipStreamSynchronize(ctx->stream);
+        p->set.release();
+        return set_build(ctx, &p, 1, &p->set);
+    };
+    const int need = (Lg / 4 <= 256) ? 4 : 8;
+    float best_csc[kFftOrders][2], best_dict[kFftOrders][kFftOrders][2];
+    Tuning best;
+    float best_t = 0.f;
+    bool have = false;
+    for (int o = 0; o < kFftOrders && st == CDLM_OK; ++o) {
+        for (int si = 0; si < 2 && st == CDLM_OK; ++si) {
+            const int nsl = si ? 8 : 4;
+            best_csc[o][si] = -1.f;
+            if (nsl < need) continue;
+            Tuning tu;
+            tu.order = o; tu.nsl = nsl;
+            if ((st = apply(tu)) == CDLM_OK) st = timed(false, 3, &best_csc[o][si]);
+        }
+        for (int o4 = 0; o4 < kFftOrders && st == CDLM_OK; ++o4)
+            for (int bi = 0; bi < 2 && st == CDLM_OK; ++bi) {
+                const int B = bi ? 4 : 2;
+                best_dict[o][o4][bi] = -1.f;
+                if (lds_bytes_dict(Lg, gamma, B) > kLdsLimit - kThreads * sizeof(double)) continue;
+                Tuning tu;
+                tu.order = o; tu.order4 = o4; tu.B = B;
+                if ((st = apply(tu)) == CDLM_OK) st = timed(true, 2, &best_dict[o][o4][bi]);
+            }
+    }
+    for (int o = 0; o < kFftOrders && st == CDLM_OK; ++o)
+        for (int si = 0; si < 2; ++si) {
+            if (best_csc[o][si] < 0) continue;
+            for (int o4 = 0; o4 < kFftOrders; ++o4)
+                for (int bi = 0; bi < 2; ++bi) {
+                    if (best_dict[o][o4][bi] < 0) continue;
+                    const float t = 15.f * best_csc[o][si] + best_dict[o][o4][bi];
+                    if (!have || t < best_t) {
+                        have = true;
+                        best_t = t;
+                        best.order = o; best.order4 = o4; best.nsl = si ? 8 : 4; best.B = bi ? 4 : 2;
+                    }
+                }
+        }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    cdlm_problem_destroy(p);
+    if (st != CDLM_OK) return st;
+    if (!have) return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "no launch choice fits shape Lg=%d", Lg);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[std::make_tuple(ctx->device, Lg, (int)M, (int)gamma)] = best;
+    }
+    if (out) {
+        out->fft_order = best.order;
+        out->fft4_order = best.order4;
+        out->dict_batch = best.B;
+        out->csc_slots = best.nsl;
+    }
+    return CDLM_OK;
 }

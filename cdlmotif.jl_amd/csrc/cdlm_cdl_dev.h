@@ -201,14 +201,15 @@ __global__ __launch_bounds__(kThreads) void k_reset(const CdlDev *__restrict__ t
     }
 }
 
-// Per problem p: the first cnt trace entries and the scalar state, into
-// out[p * (cnt + kGatherExtra) ...] (one host copy returns the whole batch).
+// Per member p: the first cnt trace entries and the scalar state, into
+// out[slot[p] * (cnt + kGatherExtra) ...] (one host copy returns the whole batch).
 constexpr int kGatherExtra = 8;
-__global__ void k_gather(const CdlDev *__restrict__ tab, int np, int cnt, double *__restrict__ out) {
+__global__ void k_gather(const CdlDev *__restrict__ tab, const int *__restrict__ slot, int np, int cnt,
+                         double *__restrict__ out) {
     const int p = blockIdx.x;
     if (p >= np) return;
     const CdlDev &P = tab[p];
-    double *o = out + (size_t)p * (cnt + kGatherExtra);
+    double *o = out + (size_t)slot[p] * (cnt + kGatherExtra);
     for (int i = threadIdx.x; i < cnt; i += blockDim.x) o[i] = (i < P.trace_cap) ? P.trace[i] : 0.0;
     if (threadIdx.x == 0) {
         const CdlScal *sc = P.sc;
@@ -860,6 +861,23 @@ __global__ __launch_bounds__(kThreads) void k_cs(const CdlDev *__restrict__ tab,
     for (int m = 0; m < P.M; ++m) a += P.gsq[(size_t)m * P.Lg + k];
     P.sS[k] = a;
     P.cS[k] = a + P.sh_csc;
+}
+
+// nb transforms of length plan.n (table length plan.n * plan.twstride) by the team FFT, for
+// the FFT self-test.  LDS: twiddles | a | b.
+__global__ __launch_bounds__(kThreads) void k_fft_selftest(const float2 *__restrict__ in, float2 *__restrict__ out,
+                                                           const float2 *__restrict__ twg, FftPlan plan, int nb, int inv) {
+    extern __shared__ float4 smem4[];
+    const int n = plan.n, N = n * plan.twstride, tid = threadIdx.x;
+    float2 *tw = reinterpret_cast<float2 *>(smem4);
+    float2 *a = tw + N;
+    float2 *b = a + (size_t)nb * n;
+    for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
+    for (int i = tid; i < nb * n; i += kThreads) a[i] = in[i];
+    __syncthreads();
+    float2 *R = inv ? team_fft<true>(a, b, plan, tw, tid, kThreads, BlockSync{}, nb)
+                    : team_fft<false>(a, b, plan, tw, tid, kThreads, BlockSync{}, nb);
+    for (int i = tid; i < nb * n; i += kThreads) out[i] = R[i];
 }
 
 #define CDLM_TAPS_SWITCH(taps, CALL)                                                       \

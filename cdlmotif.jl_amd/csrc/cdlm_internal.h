@@ -42,15 +42,28 @@ struct FftPlan {
     int radix[kMaxStages] = {0};
 };
 
-inline bool make_fft_plan(int n, int twstride, FftPlan *p) {
+// Stage orders: radix 4 and 2 stages first, then odd primes ascending; or odd primes
+// first, descending (the largest prime then runs without input twiddles).
+enum FftOrder { kFftSmallFirst = 0, kFftPrimesFirst = 1, kFftOrders = 2 };
+
+inline bool make_fft_plan(int n, int twstride, FftPlan *p, int order = kFftSmallFirst) {
     p->n = n;
     p->twstride = twstride;
     p->nst = 0;
     int m = n;
-    while (m % 4 == 0) { if (p->nst >= kMaxStages) return false; p->radix[p->nst++] = 4; m /= 4; }
-    while (m % 2 == 0) { if (p->nst >= kMaxStages) return false; p->radix[p->nst++] = 2; m /= 2; }
+    int pw[kMaxStages], npw = 0, od[kMaxStages], nod = 0;
+    while (m % 4 == 0) { if (npw >= kMaxStages) return false; pw[npw++] = 4; m /= 4; }
+    while (m % 2 == 0) { if (npw >= kMaxStages) return false; pw[npw++] = 2; m /= 2; }
     for (int q = 3; m > 1; q += 2)
-        while (m % q == 0) { if (p->nst >= kMaxStages) return false; p->radix[p->nst++] = q; m /= q; }
+        while (m % q == 0) { if (nod >= kMaxStages) return false; od[nod++] = q; m /= q; }
+    if (npw + nod > kMaxStages) return false;
+    if (order == kFftPrimesFirst) {
+        for (int i = nod - 1; i >= 0; --i) p->radix[p->nst++] = od[i];
+        for (int i = 0; i < npw; ++i) p->radix[p->nst++] = pw[i];
+    } else {
+        for (int i = 0; i < npw; ++i) p->radix[p->nst++] = pw[i];
+        for (int i = 0; i < nod; ++i) p->radix[p->nst++] = od[i];
+    }
     return true;
 }
 
@@ -88,7 +101,9 @@ __device__ __forceinline__ int fdivmod(int x, int d, float rcp, int &r) {
     return q;
 }
 
-// One radix-R butterfly j (position k = j mod Ns) of a Stockham stage with sub-length Ns.
+// One radix-R butterfly j (position k = j mod Ns) of a Stockham stage with sub-length Ns:
+// inputs a[j + r T] times W_{Ns R}^{r k} (table step st), outputs b[(j - k) R + k + q Ns].
+// rs is the table step of the radix-R roots W_R.
 template <int R, bool INV>
 __device__ __forceinline__ void fft_bfly(const float2 *__restrict__ a, float2 *__restrict__ b, int j, int k, int T,
                                          int Ns, int st, int rs, const float2 *__restrict__ tw) {
@@ -112,17 +127,87 @@ __device__ __forceinline__ void fft_bfly(const float2 *__restrict__ a, float2 *_
         b[ob + 2 * Ns] = csub(s02, s13);
         b[ob + 3 * Ns] = csub(d02, rot);
     } else {
-        float2 wr[R];
+        // odd R: outputs q and R - q from a_r = v_r + v_{R-r} and b_r = v_r - v_{R-r}:
+        //   y_q = v_0 + sum_r a_r cos(2 pi r q / R) -/+ i sum_r b_r sin(2 pi r q / R)
+        constexpr int H = (R - 1) / 2;
+        float cs[H], sn[H];
 #pragma unroll
-        for (int t = 0; t < R; ++t) wr[t] = twid<INV>(tw, t * rs);
+        for (int m = 1; m <= H; ++m) {
+            const float2 w = tw[m * rs];   // (cos, -sin)(2 pi m / R)
+            cs[m - 1] = w.x;
+            sn[m - 1] = -w.y;
+        }
+        float2 ap[H], bm[H];
+        float2 y0 = v[0];
 #pragma unroll
-        for (int q = 0; q < R; ++q) {
-            float2 acc = v[0];
+        for (int r = 1; r <= H; ++r) {
+            ap[r - 1] = cadd(v[r], v[R - r]);
+            bm[r - 1] = csub(v[r], v[R - r]);
+            y0 = cadd(y0, ap[r - 1]);
+        }
+        b[ob] = y0;
 #pragma unroll
-            for (int r = 1; r < R; ++r) acc = cadd(acc, cmul(v[r], wr[(r * q) % R]));
-            b[ob + q * Ns] = acc;
+        for (int q = 1; q <= H; ++q) {
+            float2 re = v[0], im = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int r = 1; r <= H; ++r) {
+                const int m = (r * q) % R;
+                const int mm = m <= H ? m : R - m;
+                const float sg = m <= H ? 1.f : -1.f;
+                re.x = fmaf(cs[mm - 1], ap[r - 1].x, re.x);
+                re.y = fmaf(cs[mm - 1], ap[r - 1].y, re.y);
+                im.x = fmaf(sg * sn[mm - 1], bm[r - 1].x, im.x);
+                im.y = fmaf(sg * sn[mm - 1], bm[r - 1].y, im.y);
+            }
+            // forward: y_q = re - i im, y_{R-q} = re + i im (inverse: the reverse)
+            const float2 mi = make_float2(re.x + im.y, re.y - im.x);
+            const float2 pi = make_float2(re.x - im.y, re.y + im.x);
+            b[ob + q * Ns] = INV ? pi : mi;
+            b[ob + (R - q) * Ns] = INV ? mi : pi;
         }
     }
+}
+
+// Large odd radix R: one work item per (butterfly j, output pair u); u = 0 writes y_0,
+// u > 0 writes y_u and y_{R-u} with the same pairing as the small odd butterflies.
+template <bool INV>
+__device__ __forceinline__ void fft_pair_item(const float2 *__restrict__ a, float2 *__restrict__ b, int j, int k, int u,
+                                              int R, int T, int Ns, int st, int rs,
+                                              const float2 *__restrict__ tw) {
+    const int ob = (j - k) * R + k;
+    const float2 x0 = a[j];
+    const bool twd = Ns > 1;
+    if (u == 0) {
+        float2 y = x0;
+        for (int r = 1; r < R; ++r) {
+            float2 x = a[j + r * T];
+            if (twd) x = cmul(x, twid<INV>(tw, r * k * st));
+            y = cadd(y, x);
+        }
+        b[ob] = y;
+        return;
+    }
+    float2 re = x0, im = make_float2(0.f, 0.f);
+    int e = 0;
+    for (int r = 1; 2 * r < R; ++r) {
+        float2 xr = a[j + r * T], xm = a[j + (R - r) * T];
+        if (twd) {
+            xr = cmul(xr, twid<INV>(tw, r * k * st));
+            xm = cmul(xm, twid<INV>(tw, (R - r) * k * st));
+        }
+        e += u;
+        if (e >= R) e -= R;
+        const float2 w = tw[e * rs];   // (cos, -sin)(2 pi r u / R)
+        const float2 ap = cadd(xr, xm), bm = csub(xr, xm);
+        re.x = fmaf(w.x, ap.x, re.x);
+        re.y = fmaf(w.x, ap.y, re.y);
+        im.x = fmaf(-w.y, bm.x, im.x);
+        im.y = fmaf(-w.y, bm.y, im.y);
+    }
+    const float2 mi = make_float2(re.x + im.y, re.y - im.x);
+    const float2 pi = make_float2(re.x - im.y, re.y + im.x);
+    b[ob + u * Ns] = INV ? pi : mi;
+    b[ob + (R - u) * Ns] = INV ? mi : pi;
 }
 
 // Team-cooperative complex FFTs of length P.n on nb consecutive transforms (transform t at
@@ -156,25 +241,16 @@ __device__ __forceinline__ float2 *team_fft(float2 *a, float2 *b, const FftPlan 
         case 5: CDLM_BFLY_LOOP(5) break;
         case 7: CDLM_BFLY_LOOP(7) break;
         default: {
-            // Large prime radix: one output (j, q) per work item,
-            // y = sum_r a[j + rT] * W_{Ns R}^{r (k + q Ns)}.
-            const int NsR = Ns * R;
-            const float rn = 1.0f / (float)n;
-            for (int idx = tid; idx < nb * n; idx += nthr) {
+            // large odd radix: (butterfly, output pair) work items
+            const int H = (R + 1) / 2;
+            const int per = H * T;
+            const float rper = 1.0f / (float)per;
+            for (int idx = tid; idx < nb * per; idx += nthr) {
                 int rem, j, k;
-                const int bb = fdivmod(idx, n, rn, rem);
-                const int q = fdivmod(rem, T, rT, j);
+                const int bb = fdivmod(idx, per, rper, rem);
+                const int u = fdivmod(rem, T, rT, j);
                 (void)fdivmod(j, Ns, rNs, k);
-                const int kk = k + q * Ns;
-                const float2 *ab = a + bb * n;
-                float2 acc = ab[j];
-                int e = 0;
-                for (int r = 1; r < R; ++r) {
-                    e += kk;
-                    if (e >= NsR) e -= NsR;
-                    acc = cadd(acc, cmul(ab[j + r * T], twid<INV>(tw, e * stw)));
-                }
-                b[bb * n + (j - k) * R + kk] = acc;
+                fft_pair_item<INV>(a + bb * n, b + bb * n, j, k, u, R, T, Ns, stw, rs, tw);
             }
         } break;
         }

@@ -144,6 +144,9 @@ cdlm_status cdlm_batch_destroy(cdlm_batch *b);
 /* enable != 0 (the default): a batch call replays a captured HIP graph of its launches
  * (captured on first use of a schedule, reused while the schedule is unchanged). */
 cdlm_status cdlm_batch_set_graph(cdlm_batch *b, int32_t enable);
+/* Number of shape classes the batch was split into: problems that share a launch
+ * configuration run in one launch per kernel; classes run as concurrent graph branches. */
+cdlm_status cdlm_batch_classes(const cdlm_batch *b, int32_t *count);
 /* cdlm_problem_reset of every problem. */
 cdlm_status cdlm_batch_reset(cdlm_batch *b);
 /* cdlm_optimize of every problem; obj_traces (may be NULL): count x iter_num, problem-major. */
@@ -157,6 +160,97 @@ cdlm_status cdlm_batch_train(cdlm_batch *b, int32_t iter_num, int32_t csc_iter, 
 cdlm_status cdlm_batch_timing_enable(cdlm_batch *b, int32_t enable);
 cdlm_status cdlm_batch_timing_read(cdlm_batch *b, int32_t kernel, double *total_ms, int64_t *launches);
 
+/* ---- serving --------------------------------------------------------------------------- */
+/* A continuous-batching server of CDL training jobs on one device (the reference runs one
+ * dataset at a time from find_motifs_fasta -> cdl_optimize, src/WRAPPER/wrap.jl:4-12,140-149).
+ * The server owns its own context and a host thread.  Jobs are admitted in arrival order
+ * (smaller later jobs may pass a waiting one) while their device footprint
+ * (cdlm_footprint_bytes) fits the memory budget and the resident jobs do not yet fill the
+ * device (fill_seqs sequences); every tick advances all resident jobs by
+ * one outer iteration (csc_iter CSC iterations, update_best, cdl_iter dictionary
+ * iterations) in one batch; a finished job's results move to host memory and its device
+ * memory is freed for the next admission.  A job's results do not depend on the jobs it
+ * shares the device with.  The calls below are thread-safe. */
+typedef struct cdlm_server cdlm_server;
+typedef struct {
+    uint64_t mem_budget;   /* device bytes for resident jobs (0: 80% of the free memory at start) */
+    int32_t max_active;    /* resident jobs at most (0: no limit besides memory) */
+    int32_t csc_iter;      /* CSC iterations per outer iteration, all jobs (0: 15, WRAPPER_setup.jl:8) */
+    int32_t cdl_iter;      /* dictionary iterations per outer iteration (0: 1, WRAPPER_setup.jl:9) */
+    int32_t use_graph;     /* replay ticks from captured HIP graphs */
+    int32_t fill_seqs;     /* admit no job while resident jobs hold this many sequences
+                            * (0: twelve per CU; < 0: no limit) */
+} cdlm_server_opts;
+typedef struct {
+    cdlm_params params;
+    const float *s;        /* (L4, N) data, copied at submission */
+    int32_t L4;
+    int32_t N;
+    const float *D0;       /* (Lg, M) initial filters or NULL (drawn from seed) */
+    uint64_t seed;
+    int32_t iter_num;      /* outer iterations (wrap.jl:6) */
+    double thresh_mult;    /* code threshold multiplier (wrap.jl:12) */
+    int32_t keep_codes;    /* keep the X_best slice (L, M, N) for cdlm_server_result */
+} cdlm_job_spec;
+typedef enum { CDLM_JOB_QUEUED = 0, CDLM_JOB_RUNNING = 1, CDLM_JOB_DONE = 2, CDLM_JOB_FAILED = 3 } cdlm_job_state;
+typedef struct {
+    double latency_s;      /* submission -> results in host memory */
+    double queue_s;        /* submission -> admission */
+    double thresh;         /* thresh_mult * std(X_best[X_best .> 0]) */
+    int64_t count;         /* positive X_best entries */
+    float obj_best;
+    int32_t iter_num, L, M, N;
+} cdlm_job_result;
+typedef struct {
+    int64_t completed, active, queued, ticks;
+    double p50_s, p99_s;   /* latency quantiles (nearest rank) of the completed jobs */
+    uint64_t bytes_in_use, budget_bytes;
+} cdlm_server_stats_t;
+cdlm_status cdlm_server_create(int32_t device, const cdlm_server_opts *opts, cdlm_server **out);
+/* Stops the thread; queued and resident jobs end as CDLM_JOB_FAILED. */
+cdlm_status cdlm_server_destroy(cdlm_server *srv);
+const char *cdlm_server_last_error(cdlm_server *srv);
+/* Queues a job; CDLM_ERR_OOM when its footprint exceeds the whole budget (it could never
+ * be admitted), CDLM_ERR_UNSUPPORTED outside the kernels' shape range. */
+cdlm_status cdlm_server_submit(cdlm_server *srv, const cdlm_job_spec *job, uint64_t *id);
+cdlm_status cdlm_server_poll(cdlm_server *srv, uint64_t id, int32_t *state);
+/* Waits until the job is done or failed, at most timeout_ms (< 0: no limit). */
+cdlm_status cdlm_server_wait(cdlm_server *srv, uint64_t id, int32_t timeout_ms, int32_t *state);
+/* Results of a done job; obj_trace (iter_num entries) and codes may be NULL. */
+cdlm_status cdlm_server_result(cdlm_server *srv, uint64_t id, cdlm_job_result *res, double *obj_trace, float *codes,
+                               size_t codes_bytes);
+/* Forgets a finished job. */
+cdlm_status cdlm_server_release(cdlm_server *srv, uint64_t id);
+cdlm_status cdlm_server_stats(cdlm_server *srv, cdlm_server_stats_t *out);
+
+/* ---- launch tuning ------------------------------------------------------------------ */
+/* Launch choices for problems of one shape (L4, M, max filter length gamma); problems
+ * created afterwards with that shape use them.  The reference's cuFFT plans choose their
+ * own algorithms (CDL_setup.jl:158-163); here the choice is explicit and measured.
+ * fft_order / fft4_order: stage order of the length-Lg and length-Lg/4 transforms
+ * (0 = radix-4/2 stages first, 1 = odd primes first; the only choices that change
+ * rounding); dict_batch: dictionary columns per pass (2 or 4; 0 = by occupancy);
+ * csc_slots: CSC float4 slots per lane (4 or 8; 0 = the fewest that cover Lg). */
+typedef struct {
+    int32_t fft_order;
+    int32_t fft4_order;
+    int32_t dict_batch;
+    int32_t csc_slots;
+} cdlm_tuning;
+/* Times every candidate on a scratch problem of n_seq sequences (<= 0: three per CU),
+ * keeps the fastest for 15 CSC + 1 dictionary iterations, and returns it in *out (may be NULL). */
+cdlm_status cdlm_autotune(cdlm_ctx *ctx, int32_t L4, int32_t M, int32_t gamma, int32_t n_seq, cdlm_tuning *out);
+/* Sets (t != NULL) or clears (t == NULL) the choices for a shape; get returns those in force. */
+cdlm_status cdlm_tuning_set(cdlm_ctx *ctx, int32_t L4, int32_t M, int32_t gamma, const cdlm_tuning *t);
+cdlm_status cdlm_tuning_get(cdlm_ctx *ctx, int32_t L4, int32_t M, int32_t gamma, cdlm_tuning *out);
+
+/* Self-test of the device FFT (no reference counterpart; it pins the transforms that replace
+ * the reference's cuFFT plans, CDL_setup.jl:158-163,195-204): nb complex transforms of
+ * length n, interleaved (re, im) FP32, unscaled, forward exp(-2 pi i kl/n) or inverse
+ * exp(+2 pi i kl/n); twstride = table length / n as the kernels use it (1, or 4 for the
+ * quarter-length transforms); order: 0 = radix-4/2 stages first, 1 = odd primes first. */
+cdlm_status cdlm_selftest_fft(cdlm_ctx *ctx, int32_t n, int32_t twstride, int32_t nb, int32_t inverse, int32_t order,
+                              const float *in, float *out);
 /* Device copy of `bytes` bytes, `reps` times, on the context stream; average ms per copy.
  * A plain streaming kernel of known traffic, used to calibrate HBM byte counters. */
 cdlm_status cdlm_copy_bench(cdlm_ctx *ctx, size_t bytes, int32_t reps, double *ms_per_copy);

@@ -124,3 +124,54 @@ def test_footprint_is_what_a_problem_holds():
     X0 = np.zeros((6, 2, 12 + 8), np.float32)
     c = cdlm.CDL(_data(1, 6, 3), [4, 4], RHO, SIGMA, LAM, X0=X0)
     assert c.device_bytes == L.cdlm_footprint_bytes(12, 6, 2, 4, 1)
+
+
+def test_tuned_launch_choices_keep_parity():
+    """Every stage order and dictionary batch meets the oracle tolerance; the autotuner's
+    choice is recorded and used by problems created afterwards; a batch of problems with
+    different tunings still equals each problem alone."""
+    import oracle as O
+    from gpu_util import rel
+    L, M, fl = 31, 4, [20, 12, 20, 8]            # Lg = 124 + 40 = 164 = 4 * 41
+    s = _data(77, 12, L)
+    D0 = cdlm.synth.rand_probvec_filters(9, M, 4 * L + 40, fl)
+    X0 = np.random.default_rng(10).standard_normal((12, M, 4 * L + 40)).astype(np.float32)
+    o = O.OracleCDL(s, D0, X0, fl, RHO, SIGMA, LAM, "f32")
+    o.code_update(6)
+    o.update_best()
+    o.dict_update(1)
+    o.code_update(3)
+    try:
+        for order in (0, 1):
+            for order4 in (0, 1):
+                for B in (2, 4):
+                    cdlm.tuning_set(L, M, 20, dict(fft_order=order, fft4_order=order4, dict_batch=B, csc_slots=8))
+                    assert cdlm.tuning_get(L, M, 20)["fft_order"] == order
+                    c = cdlm.CDL(s, fl, RHO, SIGMA, LAM, D0=D0, X0=X0)
+                    c.code_update(6)
+                    c.update_best()
+                    c.dict_update(1)
+                    c.code_update(3)
+                    for f in ("X", "DX", "d", "G1"):
+                        e = rel(c.get(f), o.get(f))
+                        flush("orders %d/%d B=%d %s: %.2e" % (order, order4, B, f, e))
+                        assert e <= 1e-4, (order, order4, B, f, e)
+                    c.close()
+        got = cdlm.autotune(L, M, 20, n_seq=64)
+        flush("autotune: %s" % got)
+        assert got == cdlm.tuning_get(L, M, 20)
+        a = cdlm.CDL(s, fl, RHO, SIGMA, LAM, D0=D0, X0=X0)
+        cdlm.tuning_set(L, M, 20, None)
+        b_alone = cdlm.CDL(s, fl, RHO, SIGMA, LAM, D0=D0, X0=X0)
+        ref = [a.optimize(2, 3, 1), b_alone.optimize(2, 3, 1)]
+        refx = [a.get("X"), b_alone.get("X")]
+        a2 = cdlm.CDL(s, fl, RHO, SIGMA, LAM, D0=D0, X0=X0)         # untuned (cleared)
+        cdlm.tuning_set(L, M, 20, got)
+        b2 = cdlm.CDL(s, fl, RHO, SIGMA, LAM, D0=D0, X0=X0)         # tuned
+        bt = cdlm.Batch([b2, a2])
+        tr = bt.optimize(2, 3, 1)
+        assert np.array_equal(tr[0], ref[0]) and np.array_equal(tr[1], ref[1])
+        assert np.array_equal(b2.get("X"), refx[0]) and np.array_equal(a2.get("X"), refx[1])
+        bt.close()
+    finally:
+        cdlm.tuning_set(L, M, 20, None)
