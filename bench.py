@@ -69,7 +69,7 @@ def cpu_baseline(sample_n: int):
 C3_ITER_NUM, C3_CSC_ITER, C3_CDL_ITER = 10, 15, 1
 
 
-def batched_leg(n_datasets: int, log):
+def batched_leg(n_datasets: int, log, N_range=(200, 20000), cache=True):
     """Batched workload sample: n_datasets datasets drawn as in the batched configuration
     (N in [200, 20000], L in {100..500}, M in {32, 64}, filter_len in {16, 20, 24}), each
     trained once (10 x (15 CSC + update_best + 1 dict) = 150 ADMM iterations, code
@@ -79,7 +79,7 @@ def batched_leg(n_datasets: int, log):
     import numpy as np
     import cdlm
     from cdlm import synth
-    specs = synth.c3_specs(n_datasets)
+    specs = synth.c3_specs(n_datasets, N_range=N_range)
     data = [synth.c3_onehot(sp) for sp in specs]
     ctx = cdlm.default_context()
 
@@ -132,10 +132,11 @@ def batched_leg(n_datasets: int, log):
     t_batch = time.perf_counter() - t0
     log("bench: batched workload, batched: %d waves, %.2f s" % (len(waves), t_batch))
     seq_iters = sum(sp["N"] for sp in specs) * C3_ITER_NUM * C3_CSC_ITER
-    batched_leg.cache = (specs, data, t_each, th_plain)
+    if cache:
+        batched_leg.cache = (specs, data, t_each, th_plain)
     return {
-        "workload": "%d datasets of the batched configuration (seeds 3..%d), 150 ADMM iterations each"
-                    % (len(specs), 2 + len(specs)),
+        "workload": "%d datasets of the batched configuration (seeds 3..%d, N in [%d, %d]), 150 ADMM iterations each"
+                    % (len(specs), 2 + len(specs), N_range[0], N_range[1]),
         "datasets": len(specs), "waves": len(waves), "wave_budget_bytes": budget,
         "plain_datasets_per_s": len(specs) / t_plain, "batched_datasets_per_s": len(specs) / t_batch,
         "speedup": t_plain / t_batch,
@@ -144,7 +145,7 @@ def batched_leg(n_datasets: int, log):
     }
 
 
-def served_leg(rate: float, n_jobs: int, log):
+def served_leg(rate: float, n_jobs: int, log, policy: str = "fifo"):
     """Served workload sample: the first n_jobs datasets of the batched leg arrive as a
     Poisson process of the given rate (seeded) and are submitted to the library's server
     (continuous batching, admission by device footprint, one outer iteration per tick).
@@ -157,7 +158,7 @@ def served_leg(rate: float, n_jobs: int, log):
     n = min(n_jobs, len(specs))
     gaps = np.random.default_rng(4).exponential(1.0 / rate, n)
     arrive = np.cumsum(gaps) - gaps[0]
-    srv = cdlm.Server(csc_iter=C3_CSC_ITER, cdl_iter=C3_CDL_ITER)
+    srv = cdlm.Server(csc_iter=C3_CSC_ITER, cdl_iter=C3_CDL_ITER, policy=policy)
     ids = [None] * n
     t0 = time.perf_counter()
 
@@ -172,11 +173,12 @@ def served_leg(rate: float, n_jobs: int, log):
     th = threading.Thread(target=feeder)
     th.start()
     th.join()
-    lat, same = [], True
+    lat, svc, same = [], [], True
     for i in range(n):
         srv.wait(ids[i])
         r = srv.result(ids[i])
         lat.append(r["latency_s"])
+        svc.append(r["service_s"])
         same = same and (r["thresh"] == th_plain[i] or (np.isnan(r["thresh"]) and np.isnan(th_plain[i])))
     t_end = time.perf_counter()
     st = srv.stats()
@@ -188,17 +190,55 @@ def served_leg(rate: float, n_jobs: int, log):
         plat.append(done - arrive[i])
     lat, plat = np.array(lat), np.array(plat)
     q = lambda v, p: float(np.sort(v)[max(0, int(np.ceil(p * len(v))) - 1)])
-    log("bench: served %d jobs at %.1f jobs/s: p50 %.2f s p99 %.2f s (plain loop p50 %.2f s p99 %.2f s)"
-        % (n, rate, q(lat, 0.5), q(lat, 0.99), q(plat, 0.5), q(plat, 0.99)))
+    log("bench: served (%s) %d jobs at %.1f jobs/s: p50 %.2f s p99 %.2f s, %.2f datasets/s (plain loop p50 %.2f s "
+        "p99 %.2f s, %.2f datasets/s); service time sum %.2f s vs alone %.2f s"
+        % (policy, n, rate, q(lat, 0.5), q(lat, 0.99), n / (t_end - t0), q(plat, 0.5), q(plat, 0.99), n / float(done),
+           sum(svc), sum(t_each[:n])))
     return {
         "workload": "%d batched-configuration datasets arriving as Poisson(%.1f jobs/s), 150 ADMM iterations each"
-                    % (n, rate),
+                    % (n, rate), "policy": policy,
         "jobs": n, "arrival_rate": rate, "p50_latency_s": q(lat, 0.5), "p99_latency_s": q(lat, 0.99),
         "datasets_per_s": n / (t_end - t0), "ticks": st["ticks"],
         "plain_loop_p50_latency_s": q(plat, 0.5), "plain_loop_p99_latency_s": q(plat, 0.99),
         "plain_loop_datasets_per_s": n / float(done),
         "results_identical_to_alone": bool(same),
     }
+
+
+PWM_L, PWM_K = 500, 512
+MFMA_FP32_PEAK_TFLOPS = 157.3
+
+
+def pwm_leg(n_seq: int, log):
+    """PWM scan sample: n_seq sequences of L=500 iid bases (GC 0.42) against 512 log-odds
+    PWMs of length 8-24 (the PWM-scan configuration at a bounded N), both strands,
+    threshold 0.8 x best score, full hit list.  Kernel time from HIP events around the
+    MFMA hit kernel; algorithmic flops = sum_k N (L - m_k + 1) x 2 strands x 2 x 4 m_k."""
+    import ctypes as Cc
+    import numpy as np
+    import cdlm
+    from cdlm import _lib, synth
+    bases, pwms, lens, thresh = synth.pwm_workload(5, n_seq, PWM_L, PWM_K)
+    ctx = cdlm.default_context()
+    hits = cdlm.scan_hits(bases, pwms, lens, thresh, both_strands=True, sort=False)   # warm-up, sizes the buffer
+    ms = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        h = cdlm.scan_hits(bases, pwms, lens, thresh, both_strands=True, cap=max(len(hits), 1024), sort=False)
+        wall = time.perf_counter() - t0
+        v = Cc.c_double()
+        _lib.check(_lib.load().cdlm_pwm_last_ms(ctx.h, Cc.byref(v)), ctx.h)
+        ms.append(v.value)
+    k_ms = float(np.median(ms))
+    flops = float(sum(n_seq * (PWM_L - int(m) + 1) * 2 * 2 * 4 * int(m) for m in lens))
+    windows = float(sum(n_seq * (PWM_L - int(m) + 1) * 2 for m in lens))
+    tf = flops / (k_ms * 1e-3) / 1e12
+    log("bench: PWM scan %d x %d, %d PWMs: kernel %.2f ms (%.1f TFLOP/s), %d hits, call %.2f s"
+        % (n_seq, PWM_L, PWM_K, k_ms, tf, len(h), wall))
+    return {"workload": "PWM scan: %d sequences x L=%d, %d PWMs of length 8-24, both strands, hit list" % (n_seq, PWM_L, PWM_K),
+            "kernel_ms": k_ms, "windows_per_s": windows / (k_ms * 1e-3), "hits": int(len(h)),
+            "roofline": {"bound": "mfma", "achieved": tf, "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": tf / MFMA_FP32_PEAK_TFLOPS, "algorithmic_flops": flops}}
 
 
 def main():
@@ -210,7 +250,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-datasets", type=int, default=16, help="datasets in the batched leg (0: skip it)")
+    ap.add_argument("--pwm-seqs", type=int, default=100000, help="sequences in the PWM-scan leg (0: skip it)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="job arrival rate of the served leg (jobs/s)")
+    ap.add_argument("--small-datasets", type=int, default=64,
+                    help="datasets with N in [200, 2000] in the small-dataset batched leg (0: skip it)")
+    ap.add_argument("--serve-policy", default="fifo", choices=["fifo", "shortest"], help="server admission order")
     ap.add_argument("--serve-jobs", type=int, default=16, help="jobs in the served leg (0: skip it)")
     ap.add_argument("--outer", type=int, default=ITER_NUM, help="outer iterations per step (profiling runs only)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
@@ -309,9 +353,14 @@ def main():
     del c
     if args.batch_datasets > 0:
         out["batched"] = batched_leg(args.batch_datasets, lambda m: print(m, file=sys.stderr, flush=True))
+        if args.small_datasets > 0:
+            out["batched_small"] = batched_leg(args.small_datasets, lambda m: print(m, file=sys.stderr, flush=True),
+                                               N_range=(200, 2000), cache=False)
         if args.serve_jobs > 0:
             out["served"] = served_leg(args.serve_rate, args.serve_jobs,
-                                       lambda m: print(m, file=sys.stderr, flush=True))
+                                       lambda m: print(m, file=sys.stderr, flush=True), args.serve_policy)
+    if args.pwm_seqs > 0:
+        out["pwm"] = pwm_leg(args.pwm_seqs, lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu:
         print("bench: CPU baseline on a %d-sequence slice" % args.cpu_sample, file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample)

@@ -45,7 +45,7 @@ class cdlm_tuning(C.Structure):
 
 class cdlm_server_opts(C.Structure):
     _fields_ = [("mem_budget", C.c_uint64), ("max_active", C.c_int32), ("csc_iter", C.c_int32), ("cdl_iter", C.c_int32),
-                ("use_graph", C.c_int32), ("fill_seqs", C.c_int32)]
+                ("use_graph", C.c_int32), ("fill_seqs", C.c_int32), ("policy", C.c_int32)]
 
 
 class cdlm_job_spec(C.Structure):
@@ -54,7 +54,8 @@ class cdlm_job_spec(C.Structure):
 
 
 class cdlm_job_result(C.Structure):
-    _fields_ = [("latency_s", C.c_double), ("queue_s", C.c_double), ("thresh", C.c_double), ("count", C.c_int64),
+    _fields_ = [("latency_s", C.c_double), ("queue_s", C.c_double), ("service_s", C.c_double), ("thresh", C.c_double),
+                ("count", C.c_int64),
                 ("obj_best", C.c_float), ("iter_num", C.c_int32), ("L", C.c_int32), ("M", C.c_int32), ("N", C.c_int32)]
 
 
@@ -128,6 +129,7 @@ def load():
         "cdlm_pwm_hits": ([vp, vp, i32, i64, vp, vp, vp, i32, i32, i32, vp, vp, vp, vp, i64, P(i64)], C.c_int),
         "cdlm_pwm_argmax": ([vp, vp, i32, i64, vp, vp, vp, i32, i32, vp], C.c_int),
         "cdlm_pwm_best": ([vp, vp, i32, i64, vp, vp, vp, i32, i32, vp, vp, vp], C.c_int),
+        "cdlm_pwm_last_ms": ([vp, P(f64)], C.c_int),
         "cdlm_onehot_to_bases": ([vp, vp, i32, i64, vp], C.c_int),
         "cdlm_pwm_hits_device": ([vp, vp, i32, i64, vp, vp, vp, i32, i32, i32, vp, vp, vp, vp, i64, P(i64)], C.c_int),
     }

@@ -21,10 +21,12 @@ QUEUED, RUNNING, DONE, FAILED = range(4)
 
 class Server:
     def __init__(self, device: int = 0, mem_budget: int = 0, max_active: int = 0, csc_iter: int = 15,
-                 cdl_iter: int = 1, graph: bool = True, fill_seqs: int = 0):
+                 cdl_iter: int = 1, graph: bool = True, fill_seqs: int = 0, policy: str = "fifo"):
         """fill_seqs: admit no job while the resident jobs hold this many sequences (0: twelve
-        per CU, < 0: no limit besides memory)."""
-        o = cdlm_server_opts(mem_budget, max_active, csc_iter, cdl_iter, 1 if graph else 0, fill_seqs)
+        per CU, < 0: no limit besides memory).  policy: "fifo" (arrival order) or "shortest"
+        (least work first); a job passed by 4 later arrivals is admitted first under either."""
+        pol = {"fifo": 0, "shortest": 1}[policy]
+        o = cdlm_server_opts(mem_budget, max_active, csc_iter, cdl_iter, 1 if graph else 0, fill_seqs, pol)
         h = C.c_void_p()
         st = load().cdlm_server_create(device, C.byref(o), C.byref(h))
         if st != 0:

@@ -15,6 +15,7 @@ struct cdlm_ctx {
     hipStream_t stream = nullptr;
     std::string last_error;
     int num_cu = 0;
+    double last_scan_ms = 0.0;   // device time of the last PWM hit-scan kernel
 };
 
 // Records an error message on the context and returns the status.

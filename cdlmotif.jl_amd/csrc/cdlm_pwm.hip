@@ -380,6 +380,12 @@ cdlm_status launch_hits(cdlm_ctx *ctx, const uint8_t *d_bases, int L, long N, co
         if (yb > target) yb = target;
         if (yb > 65535) yb = 65535;
         if (yb < 1) yb = 1;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+            (void)hipGetLastError();
+            e0 = e1 = nullptr;
+        }
+        if (e0) (void)hipEventRecord(e0, st);
         if (lds > 64 * 1024) s = cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "sequence length %d too long for the scan", L);
         else if (G == 4)
             hipLaunchKernelGGL((k_pwm_hits_mfma<4, 32>), dim3(P.groups, (unsigned)yb), dim3(256), lds, st, d_bases, L, N,
@@ -389,6 +395,12 @@ cdlm_status launch_hits(cdlm_ctx *ctx, const uint8_t *d_bases, int L, long N, co
                                P, hk, hn, hp, hs, cap, d_count);
         hipError_t e = hipGetLastError();
         if (s == CDLM_OK && e != hipSuccess) s = cdlm_fail(ctx, CDLM_ERR_HIP, "k_pwm_hits_mfma: %s", hipGetErrorString(e));
+        if (e1) (void)hipEventRecord(e1, st);
+        (void)hipStreamSynchronize(st);
+        float ms = 0.f;
+        if (e0 && e1 && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) ctx->last_scan_ms = ms;
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
     }
     (void)hipStreamSynchronize(st);
     dt.release();
@@ -594,4 +606,10 @@ extern "C" cdlm_status cdlm_selftest_mfma(cdlm_ctx *ctx, const float *a64, const
     }
     (void)hipFree(buf);
     return s;
+}
+
+extern "C" cdlm_status cdlm_pwm_last_ms(cdlm_ctx *ctx, double *ms) {
+    if (!ctx || !ms) return CDLM_ERR_ARG;
+    *ms = ctx->last_scan_ms;
+    return CDLM_OK;
 }

@@ -9,9 +9,11 @@
 //     (cdlm_footprint_bytes, the exact bytes a problem holds) fits the free budget and the
 //     resident jobs hold fewer sequences than fill_seqs (the sequences that occupy every
 //     CU: each sequence is one work-group of every per-sequence kernel, so more resident
-//     work only shares the device between jobs and delays each of them); a smaller later
-//     job may pass a waiting one (backfill) unless the waiting one has waited kStarveTicks
-//     ticks;
+//     work only shares the device between jobs and delays each of them).  Candidates are
+//     taken in arrival order (policy FIFO) or least work first, arrival order breaking ties
+//     (policy SHORTEST; work = iter_num x N x Lg x M, proportional to the job's HBM bytes);
+//     a candidate that does not fit may be passed by a later one (backfill), but a job
+//     passed kMaxBypass times goes first and nothing passes it any more;
 //   * a tick runs one outer iteration of the reference schedule (csc_iter CSC iterations,
 //     update_best, cdl_iter dictionary iterations; wrap.jl:6-11) for every resident job
 //     as one batch (one launch per kernel per shape class, replayed from a HIP graph while
@@ -42,7 +44,7 @@
 
 namespace {
 
-constexpr int kStarveTicks = 8;
+constexpr int kMaxBypass = 4;
 using Clock = std::chrono::steady_clock;
 
 struct Job {
@@ -55,9 +57,10 @@ struct Job {
     uint64_t seed = 0;
     double mult = 1.0;
     size_t footprint = 0;
+    double work = 0.0;     // iter_num * N * Lg * M
     // state
     int32_t state = CDLM_JOB_QUEUED;
-    int waited_ticks = 0;
+    int bypassed = 0;      // later arrivals admitted before this job
     cdlm_problem *p = nullptr;
     int outer_done = 0;
     Clock::time_point t_submit, t_admit, t_done;
@@ -122,21 +125,35 @@ std::vector<Job *> pick_admissions(cdlm_server *srv) {
     int slots = srv->opts.max_active > 0 ? srv->opts.max_active - (int)srv->active.size() : 1 << 30;
     long seqs = 0;
     for (Job *j : srv->active) seqs += j->N;
-    bool blocked = false;   // an earlier job is waiting too long: no backfill past it
-    for (auto it = srv->queue.begin(); it != srv->queue.end() && slots > 0;) {
-        if (srv->fill > 0 && seqs >= srv->fill) break;
-        Job *j = srv->jobs[*it].get();
-        if (j->footprint <= free_b && !blocked) {
+    // candidate order: jobs passed kMaxBypass times first (arrival order), then by policy
+    std::vector<Job *> cand;
+    for (uint64_t id : srv->queue) cand.push_back(srv->jobs[id].get());
+    auto starving = [](const Job *j) { return j->bypassed >= kMaxBypass; };
+    std::stable_sort(cand.begin(), cand.end(), [&](const Job *a, const Job *b) {
+        const bool sa = starving(a), sb = starving(b);
+        if (sa != sb) return sa;
+        if (sa || srv->opts.policy != CDLM_POLICY_SHORTEST) return false;   // keep arrival order
+        return a->work < b->work;
+    });
+    bool blocked = false;   // a starving job does not fit: nothing passes it
+    for (Job *j : cand) {
+        if (slots <= 0 || (srv->fill > 0 && seqs >= srv->fill) || blocked) break;
+        if (j->footprint <= free_b) {
             free_b -= j->footprint;
             srv->in_use += j->footprint;
             seqs += j->N;
             --slots;
             j->state = CDLM_JOB_RUNNING;
             out.push_back(j);
-            it = srv->queue.erase(it);
-        } else {
-            if (j->waited_ticks >= kStarveTicks) blocked = true;
-            ++it;
+        } else if (starving(j)) {
+            blocked = true;
+        }
+    }
+    for (Job *j : out) {
+        srv->queue.erase(std::find(srv->queue.begin(), srv->queue.end(), j->id));
+        for (uint64_t id : srv->queue) {
+            Job *w = srv->jobs[id].get();
+            if (w->id < j->id) ++w->bypassed;   // an earlier arrival was passed
         }
     }
     return out;
@@ -208,7 +225,6 @@ bool tick(cdlm_server *srv) {
     {
         std::lock_guard<std::mutex> lk(srv->mu);
         ++srv->ticks;
-        for (uint64_t id : srv->queue) ++srv->jobs[id]->waited_ticks;
     }
     if (!done.empty()) {
         drop_batch(srv);   // the batch references the finished problems
@@ -392,6 +408,7 @@ extern "C" cdlm_status cdlm_server_submit(cdlm_server *srv, const cdlm_job_spec 
     if (gamma > 32 || spec->L4 + 2 * gamma > 2048)
         return reject(CDLM_ERR_UNSUPPORTED, "job shape outside the kernels' range (filter length <= 32, Lg <= 2048)");
     j->footprint = cdlm_footprint_bytes(spec->L4, spec->N, prm.M, gamma, 0);
+    j->work = (double)spec->iter_num * spec->N * (double)(spec->L4 + 2 * gamma) * prm.M;
     if (j->footprint > srv->budget) {
         char buf[256];
         snprintf(buf, sizeof(buf), "job needs %zu device bytes, more than the server budget of %zu", j->footprint,
@@ -456,6 +473,7 @@ extern "C" cdlm_status cdlm_server_result(cdlm_server *srv, uint64_t id, cdlm_jo
     if (j->state != CDLM_JOB_DONE) return CDLM_ERR_STATE;
     res->latency_s = std::chrono::duration<double>(j->t_done - j->t_submit).count();
     res->queue_s = std::chrono::duration<double>(j->t_admit - j->t_submit).count();
+    res->service_s = std::chrono::duration<double>(j->t_done - j->t_admit).count();
     res->thresh = j->thresh;
     res->count = j->count;
     res->obj_best = j->obj_best;

@@ -180,7 +180,11 @@ typedef struct {
     int32_t use_graph;     /* replay ticks from captured HIP graphs */
     int32_t fill_seqs;     /* admit no job while resident jobs hold this many sequences
                             * (0: twelve per CU; < 0: no limit) */
+    int32_t policy;        /* admission order: CDLM_POLICY_FIFO or CDLM_POLICY_SHORTEST (least work
+                            * iter_num*N*Lg*M first); a job passed by 4 later arrivals goes first
+                            * either way */
 } cdlm_server_opts;
+enum { CDLM_POLICY_FIFO = 0, CDLM_POLICY_SHORTEST = 1 };
 typedef struct {
     cdlm_params params;
     const float *s;        /* (L4, N) data, copied at submission */
@@ -196,6 +200,7 @@ typedef enum { CDLM_JOB_QUEUED = 0, CDLM_JOB_RUNNING = 1, CDLM_JOB_DONE = 2, CDL
 typedef struct {
     double latency_s;      /* submission -> results in host memory */
     double queue_s;        /* submission -> admission */
+    double service_s;      /* admission -> results in host memory */
     double thresh;         /* thresh_mult * std(X_best[X_best .> 0]) */
     int64_t count;         /* positive X_best entries */
     float obj_best;
@@ -282,6 +287,8 @@ cdlm_status cdlm_pwm_argmax(cdlm_ctx *ctx, const uint8_t *bases, int32_t L, int6
 cdlm_status cdlm_pwm_best(cdlm_ctx *ctx, const uint8_t *bases, int32_t L, int64_t N, const float *pwms,
                           const int32_t *lens, const float *thresh, int32_t K, int32_t maxlen, int32_t *out_pos,
                           int8_t *out_strand, float *out_score);
+/* Device time (HIP events on the context stream) of the last hit-list scan kernel. */
+cdlm_status cdlm_pwm_last_ms(cdlm_ctx *ctx, double *ms);
 /* One-hot (4L, N) FP32 columns to base codes (N, L); a column with no 1 in a 4-group
  * gets code 4 (never matches a PWM row).  Replaces the device side of data_2_dummy. */
 cdlm_status cdlm_onehot_to_bases(cdlm_ctx *ctx, const float *onehot, int32_t L4, int64_t N, uint8_t *bases);

@@ -115,3 +115,27 @@ def test_destroy_with_pending_jobs_fails_them_cleanly():
     time.sleep(0.05)
     srv.close()   # must not hang; queued and resident jobs end as failed
     assert ids
+
+
+@pytest.mark.parametrize("policy", ["fifo", "shortest"])
+def test_admission_order_policy(policy):
+    """One resident job at a time: a long job runs while a big and then a small job queue.
+    FIFO admits the big one first; shortest-work-first admits the small one first."""
+    srv = cdlm.Server(csc_iter=CSC, cdl_iter=CDL_IT, max_active=1, policy=policy)
+    a = srv.submit(_data(301, 60, 200), [16] * 8, RHO, SIGMA, LAM, 400, seed=1)
+    time.sleep(0.01)
+    big = srv.submit(_data(302, 60, 200), [16] * 8, RHO, SIGMA, LAM, 20, seed=2)
+    small = srv.submit(_data(303, 10, 20), [8] * 2, RHO, SIGMA, LAM, 2, seed=3)
+    assert srv.poll(a) == cdlm.serve.RUNNING
+    first = None
+    while first is None:
+        sb, ss = srv.poll(big), srv.poll(small)
+        if sb != cdlm.serve.QUEUED:
+            first = "big"
+        elif ss != cdlm.serve.QUEUED:
+            first = "small"
+        time.sleep(0.0005)
+    assert first == ("big" if policy == "fifo" else "small"), (policy, first)
+    for j in (a, big, small):
+        assert srv.wait(j, 120) == cdlm.serve.DONE
+    srv.close()
