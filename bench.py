@@ -35,9 +35,10 @@ HBM_PEAK_GBS = 8000.0
 
 
 def csc_bytes(N, Lg, M):
-    """Algorithmic HBM bytes of one CSC iteration: X, y2, y3 read and written (6M) plus
-    y1, Y1, DX read and written and the data read (7), per (l, n), 4 bytes each."""
-    return 4.0 * N * Lg * (6 * M + 7)
+    """Algorithmic HBM bytes of one CSC iteration: X and y2 read and written (4M), y3 read and
+    written off the stride-4 support (1.5M; on the support y3 is identically zero), y1, Y1,
+    DX read and written and the data read (7), per (l, n), 4 bytes each."""
+    return 4.0 * N * Lg * (5.5 * M + 7)
 
 
 def cpu_baseline(sample_n: int):

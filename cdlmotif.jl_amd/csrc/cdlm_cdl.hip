@@ -201,7 +201,7 @@ namespace {
 // Buffer plan of one problem: sizes in order of allocation.  The footprint query and the
 // allocator both use it, so cdlm_problem_bytes == cdlm_footprint_bytes.
 struct BufPlan {
-    size_t s, E, Ex0, LN, LM, Lg, xs, y3s, M, partials, dpart, trace, sc;
+    size_t s, E, E3, Ex0, LN, LM, Lg, xs, y3s, M, partials, dpart, trace, sc;
 };
 
 BufPlan buf_plan(int L4, int N, int M, int gamma, bool keep_x0) {
@@ -209,6 +209,7 @@ BufPlan buf_plan(int L4, int N, int M, int gamma, bool keep_x0) {
     const size_t Lg = (size_t)L4 + 2 * (size_t)gamma;
     b.s = (size_t)L4 * N * sizeof(float);
     b.E = Lg * M * (size_t)N * sizeof(float);
+    b.E3 = 3 * (Lg / 4) * M * (size_t)N * sizeof(float);   // y3 off the stride-4 support
     b.Ex0 = keep_x0 ? b.E : 0;
     b.LN = Lg * (size_t)N * sizeof(float);
     b.LM = Lg * (size_t)M * sizeof(float);
@@ -224,7 +225,7 @@ BufPlan buf_plan(int L4, int N, int M, int gamma, bool keep_x0) {
 }
 
 size_t plan_total(const BufPlan &b) {
-    return b.s + 6 * b.E + b.Ex0 + 7 * b.LN + 7 * b.LM + 2 * b.Lg /* cS, sS */ + 2 * b.Lg /* tw */ + 2 * b.xs + b.y3s +
+    return b.s + 5 * b.E + b.E3 + b.Ex0 + 7 * b.LN + 7 * b.LM + 2 * b.Lg /* cS, sS */ + 2 * b.Lg /* tw */ + 2 * b.xs + b.y3s +
            b.M + b.partials + b.dpart + b.trace + b.sc;
 }
 
@@ -548,7 +549,7 @@ extern "C" cdlm_status cdlm_problem_create(cdlm_ctx *ctx, const cdlm_params *prm
     float2 *tw_dev = nullptr;
 #define A(ptr, bytes) do { st = palloc_t(p.get(), &(ptr), (bytes)); if (st != CDLM_OK) { free_problem(p.get()); return st; } } while (0)
     A(s_dev, b.s);
-    A(d.Xt, b.E); A(d.y2, b.E); A(d.y3, b.E); A(d.d, b.E); A(d.g1, b.E); A(d.g3, b.E);
+    A(d.Xt, b.E); A(d.y2, b.E); A(d.y3, b.E3); A(d.d, b.E); A(d.g1, b.E); A(d.g3, b.E);
     if (X0) A(X0_dev, b.Ex0);
     A(d.h, b.LN); A(d.y1, b.LN); A(d.Y1, b.LN); A(d.DX, b.LN); A(d.G2, b.LN); A(d.g2, b.LN); A(d.Td, b.LN);
     A(d.G1b[0], b.LM); A(d.G1b[1], b.LM); A(d.G3, b.LM); A(d.meand, b.LM); A(d.Dbest, b.LM); A(d.gsq, b.LM);
@@ -731,7 +732,23 @@ extern "C" cdlm_status cdlm_get(cdlm_problem *p, int32_t field, void *dst, size_
     case CDLM_F_Y1: src = d.Y1; cnt = LN; break;
     case CDLM_F_y1: src = d.y1; cnt = LN; break;
     case CDLM_F_y2: src = d.y2; cnt = E; break;
-    case CDLM_F_y3: src = d.y3; cnt = E; break;
+    case CDLM_F_y3: {
+        // stored without the stride-4 support samples (identically zero); expand
+        if (bytes != E * sizeof(float)) return cdlm_fail(p->ctx, CDLM_ERR_ARG, "field y3 needs %zu bytes", E * sizeof(float));
+        const size_t nslot = (size_t)p->Lg / 4;
+        std::vector<float> packed(3 * nslot * p->M * (size_t)p->N);
+        CDLM_HIP(p->ctx, hipMemcpyAsync(packed.data(), d.y3, packed.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+        cdlm_status sy = sync_ctx(p->ctx);
+        if (sy != CDLM_OK) return sy;
+        float *o = static_cast<float *>(dst);
+        for (size_t c = 0; c < (size_t)p->M * p->N; ++c)
+            for (size_t sl = 0; sl < nslot; ++sl) {
+                const float *q = &packed[(c * nslot + sl) * 3];
+                float *w = o + c * p->Lg + 4 * sl;
+                w[0] = 0.f; w[1] = q[0]; w[2] = q[1]; w[3] = q[2];
+            }
+        return CDLM_OK;
+    }
     case CDLM_F_g1: src = d.g1; cnt = E; break;
     case CDLM_F_g2: src = d.g2; cnt = LN; break;
     case CDLM_F_g3: src = d.g3; cnt = E; break;

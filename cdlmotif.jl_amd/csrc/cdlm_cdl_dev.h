@@ -11,7 +11,7 @@
 //     with S_v = sum_m G1_m V_m, the rank-1 solve gives
 //       X_m = v_m / c0 + g_m (*) h,      h  = F^-1[(c0 U - S_v) / (c0 (|G1|^2 + c0))]
 //       DX  = F^-1[S_v / c0 + |G1|^2 h]                      (c0 = sherman_const_csc)
-//     where (*) is circular cross-correlation with the filter g_m = G1[:, m], which is zero
+//     where (*) is circular cross-correlation with the filter g_m = gp(G1)[:, m], which is zero
 //     past filter_len (prox_simplex_zeros!), so it costs filter_len taps per sample;
 //     S_v = F(sum_m g_m (x) v_m) is a filter_len-tap circular convolution summed over m.
 //   * X is held as the pair (Xt = v/c0, h): X = Xt + g (*) h is rebuilt on the fly by the
@@ -33,7 +33,10 @@
 // wrap modulo the problem's own Lg.
 //
 // HBM layout: the reference's column-major (Lg, M, N) with l fastest; one work-group owns
-// one sequence n and streams its contiguous (Lg x M) slab.
+// one sequence n and streams its contiguous (Lg x M) slab.  y3 is stored without its
+// stride-4 support samples: there y3 + X - Pi4(X + y3) = (y3 + X) - (X + y3) = 0 exactly
+// (IEEE addition commutes) from the first iteration on, and y3 starts at 0, so the
+// support samples are identically zero and only l % 4 != 0 is kept, 3 floats per slot.
 #pragma once
 
 #include "cdlm_internal.h"
@@ -63,7 +66,8 @@ struct CdlDev {
     float rho, sigma, lambda, sh_csc, sh_cdl;
     const float *s;
     const int *fl;
-    float *Xt, *y2, *y3, *d, *g1, *g3;            // (Lg, M, N)
+    float *Xt, *y2, *d, *g1, *g3;                 // (Lg, M, N)
+    float *y3;                                     // (3 Lg/4, M, N): y3 off the stride-4 support
     float *h, *y1, *Y1, *DX, *G2, *g2, *Td;       // (Lg, N)
     float *G1b[2], *G3, *meand, *Dbest, *gsq;     // (Lg, M)
     float *cS, *sS;                                // (Lg)
@@ -89,12 +93,35 @@ __device__ __forceinline__ int find_problem(const int *__restrict__ starts, int 
     return lo;
 }
 
+// Device pointers read from the descriptor table carry no address space the compiler can
+// prove, so every access goes through gp(), which makes it a global-memory access
+// (global_load/global_store instead of flat instructions that also count against the
+// LDS wait counter).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define CDLM_GLOBAL __attribute__((address_space(1)))
+#else
+#define CDLM_GLOBAL   // the host pass only parses device code
+#endif
+template <typename T>
+__device__ __forceinline__ CDLM_GLOBAL T *gp(T *p) { return (CDLM_GLOBAL T *)p; }
+
 __device__ __forceinline__ const float *g1cur_of(const CdlDev &P) { return P.G1b[P.sc->g1cur & 1]; }
 
-__device__ __forceinline__ float4 ld4(const float *p, int s) { return reinterpret_cast<const float4 *>(p)[s]; }
-__device__ __forceinline__ void st4(float *p, int s, float4 v) { reinterpret_cast<float4 *>(p)[s] = v; }
+__device__ __forceinline__ float4 ld4(const float *p, int s) { return ((const CDLM_GLOBAL float4 *)p)[s]; }
+__device__ __forceinline__ void st4(float *p, int s, float4 v) { ((CDLM_GLOBAL float4 *)p)[s] = v; }
 __device__ __forceinline__ float f4(const float4 &v, int q) { return q == 0 ? v.x : q == 1 ? v.y : q == 2 ? v.z : v.w; }
 __device__ __forceinline__ float4 mk4(const float *a) { return make_float4(a[0], a[1], a[2], a[3]); }
+// Samples 1..3 of slot s of a y3 column (3 floats per slot).
+__device__ __forceinline__ float4 ld3(const float *p, int s) {
+    const CDLM_GLOBAL float *q = gp(p) + 3 * s;
+    return make_float4(0.f, q[0], q[1], q[2]);
+}
+__device__ __forceinline__ void st3(float *p, int s, const float *v) {
+    CDLM_GLOBAL float *q = gp(p) + 3 * s;
+    q[0] = v[1];
+    q[1] = v[2];
+    q[2] = v[3];
+}
 
 // X on the 4 samples of slot s: Xt + sum_j g[j] * h[l + j] (circular; hb4 holds h plus a
 // wrapped tail of TAPS+4 samples, as float4).
@@ -140,14 +167,14 @@ __device__ __forceinline__ void conv_acc4(const float4 *vb4, int s, const float 
 template <int TAPS>
 __device__ __forceinline__ void load_taps(const float *G, int m, int Lg, float *g) {
 #pragma unroll
-    for (int j = 0; j < TAPS; ++j) g[j] = (j < Lg) ? G[(size_t)m * Lg + j] : 0.f;
+    for (int j = 0; j < TAPS; ++j) g[j] = (j < Lg) ? gp(G)[(size_t)m * Lg + j] : 0.f;
 }
 
 // h of sequence n into LDS with its circular tail: hb[Lg + t] = h[t mod Lg], t < TAPS + 4.
 template <int TAPS>
 __device__ __forceinline__ void load_h(const CdlDev &P, size_t ro, float *hb, int tid, int nthr) {
     const int Lg = P.Lg;
-    for (int l = tid; l < Lg + TAPS + 4; l += nthr) hb[l] = P.h[ro + (l < Lg ? l : (l - Lg) % Lg)];
+    for (int l = tid; l < Lg + TAPS + 4; l += nthr) hb[l] = gp(P.h)[ro + (l < Lg ? l : (l - Lg) % Lg)];
 }
 
 // Wrapped head of a v column (float4 units): vb4[TAPS/4 - u] = v4[(nslot - u) mod nslot].
@@ -158,6 +185,9 @@ __device__ __forceinline__ void v_head(float4 *vb4, int nslot, int lane) {
         vb4[TAPS / 4 - u] = vb4[TAPS / 4 + ((nslot - u % nslot) % nslot)];
     }
 }
+
+// Offset of column (n, m) in the y3 array (3 floats per slot).
+__device__ __forceinline__ size_t col3_of(int n, int m, int M, int nslot) { return ((size_t)n * M + m) * 3 * nslot; }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t x) {
     x ^= x >> 30; x *= 0xbf58476d1ce4e5b9ULL;
@@ -183,11 +213,11 @@ __global__ __launch_bounds__(kThreads) void k_reset(const CdlDev *__restrict__ t
     const int Lg = P.Lg;
     for (int l = threadIdx.x; l < Lg; l += kThreads) {
         const size_t i = (size_t)m * Lg + l;
-        const float v = P.D0[i];
-        P.G1b[0][i] = v;
-        P.G3[i] = v;
-        P.meand[i] = v;
-        P.Dbest[i] = v;
+        const float v = gp(P.D0)[i];
+        gp(P.G1b[0])[i] = v;
+        gp(P.G3)[i] = v;
+        gp(P.meand)[i] = v;
+        gp(P.Dbest)[i] = v;
     }
     if (m == 0 && threadIdx.x == 0) {
         CdlScal *sc = P.sc;
@@ -210,7 +240,7 @@ __global__ void k_gather(const CdlDev *__restrict__ tab, const int *__restrict__
     if (p >= np) return;
     const CdlDev &P = tab[p];
     double *o = out + (size_t)slot[p] * (cnt + kGatherExtra);
-    for (int i = threadIdx.x; i < cnt; i += blockDim.x) o[i] = (i < P.trace_cap) ? P.trace[i] : 0.0;
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) o[i] = (i < P.trace_cap) ? gp(P.trace)[i] : 0.0;
     if (threadIdx.x == 0) {
         const CdlScal *sc = P.sc;
         o[cnt + 0] = (double)sc->obj_best;
@@ -249,11 +279,11 @@ __global__ __launch_bounds__(kThreads) void k_init(const CdlDev *__restrict__ ta
         float g[TAPS];
         load_taps<TAPS>(G1, mm, Lg, g);
         for (int s = lane; s < nslot; s += 64) {
-            const float x = P.X0 ? P.X0[col + 4 * s] : gauss_of(P.seed, col + 4 * (size_t)s);
+            const float x = P.X0 ? gp(P.X0)[col + 4 * s] : gauss_of(P.seed, col + 4 * (size_t)s);
             const float4 xp = make_float4(x, 0.f, 0.f, 0.f);
             st4(P.Xt + col, s, xp);
             st4(P.y2 + col, s, zero4);
-            st4(P.y3 + col, s, zero4);
+            st3(P.y3 + col3_of(n, mm, M, nslot), s, reinterpret_cast<const float *>(&zero4));
             st4(P.g1 + col, s, zero4);
             st4(P.g3 + col, s, zero4);
             st4(P.d + col, s, ld4(G1 + (size_t)mm * Lg, s));
@@ -271,13 +301,13 @@ __global__ __launch_bounds__(kThreads) void k_init(const CdlDev *__restrict__ ta
         float z = zbase[l];
 #pragma unroll
         for (int ww = 1; ww < kWaves; ++ww) z += zbase[ww * Lg + l];
-        P.DX[ro + l] = z;
-        P.G2[ro + l] = z;
-        P.h[ro + l] = 0.f;
-        P.y1[ro + l] = 0.f;
-        P.Y1[ro + l] = 0.f;
-        P.g2[ro + l] = 0.f;
-        P.Td[ro + l] = 0.f;
+        gp(P.DX)[ro + l] = z;
+        gp(P.G2)[ro + l] = z;
+        gp(P.h)[ro + l] = 0.f;
+        gp(P.y1)[ro + l] = 0.f;
+        gp(P.Y1)[ro + l] = 0.f;
+        gp(P.g2)[ro + l] = 0.f;
+        gp(P.Td)[ro + l] = 0.f;
     }
 }
 
@@ -307,7 +337,7 @@ __global__ __launch_bounds__(kThreads, (NSL <= 4 ? 3 : 2)) void k_csc(const CdlD
     const size_t ro = (size_t)n * Lg;
     const float *G1 = g1cur_of(P);
 
-    for (int l = tid; l < Lg; l += kThreads) tw[l] = P.tw[l];
+    for (int l = tid; l < Lg; l += kThreads) tw[l] = gp(P.tw)[l];
     load_h<TAPS>(P, ro, hb, tid, kThreads);
     for (int i = tid; i < kWaves * Lg; i += kThreads) zbase[i] = 0.f;
     if (n == 0 && tid == 0) P.sc->xmat = 0;   // this iteration leaves h != 0
@@ -319,25 +349,25 @@ __global__ __launch_bounds__(kThreads, (NSL <= 4 ? 3 : 2)) void k_csc(const CdlD
     float4 *zacc4 = zbase4 + w * (Lg / 4);
     float4 nx[NSL], n2[NSL], n3[NSL];
     if (w < M) {
-        const size_t col = ((size_t)n * M + w) * Lg;
+        const size_t col = ((size_t)n * M + w) * Lg, col3 = col3_of(n, w, M, nslot);
 #pragma unroll
         for (int i = 0; i < NSL; ++i) {
             const int s = lane + 64 * i;
-            if (s < nslot) { nx[i] = ld4(P.Xt + col, s); n2[i] = ld4(P.y2 + col, s); n3[i] = ld4(P.y3 + col, s); }
+            if (s < nslot) { nx[i] = ld4(P.Xt + col, s); n2[i] = ld4(P.y2 + col, s); n3[i] = ld3(P.y3 + col3, s); }
         }
     }
     for (int m = w; m < M; m += kWaves) {
         const int mm = __builtin_amdgcn_readfirstlane(m);
-        const size_t col = ((size_t)n * M + mm) * Lg;
+        const size_t col = ((size_t)n * M + mm) * Lg, col3 = col3_of(n, mm, M, nslot);
         float4 xt[NSL], a2[NSL], a3[NSL];
 #pragma unroll
         for (int i = 0; i < NSL; ++i) { xt[i] = nx[i]; a2[i] = n2[i]; a3[i] = n3[i]; }
         if (mm + kWaves < M) {
-            const size_t coln = ((size_t)n * M + mm + kWaves) * Lg;
+            const size_t coln = ((size_t)n * M + mm + kWaves) * Lg, coln3 = col3_of(n, mm + kWaves, M, nslot);
 #pragma unroll
             for (int i = 0; i < NSL; ++i) {
                 const int s = lane + 64 * i;
-                if (s < nslot) { nx[i] = ld4(P.Xt + coln, s); n2[i] = ld4(P.y2 + coln, s); n3[i] = ld4(P.y3 + coln, s); }
+                if (s < nslot) { nx[i] = ld4(P.Xt + coln, s); n2[i] = ld4(P.y2 + coln, s); n3[i] = ld3(P.y3 + coln3, s); }
             }
         }
         float g[TAPS];
@@ -355,14 +385,14 @@ __global__ __launch_bounds__(kThreads, (NSL <= 4 ? 3 : 2)) void k_csc(const CdlD
                     const float t = x + f4(a2[i], q);                   // shrink! (csc_kernels.jl:58-59)
                     const float at = fabsf(t) - lam;
                     const float Y2 = at > 0.f ? copysignf(at, t) : 0.f;
-                    const float Y3 = (q == 0) ? x + f4(a3[i], q) : 0.f;  // projectionX! (:73-77)
+                    const float Y3 = (q == 0) ? x : 0.f;                  // projectionX! (:73-77), y3 = 0 there
                     y2n[q] = f4(a2[i], q) + x - Y2;                       // train.jl:13
-                    y3n[q] = f4(a3[i], q) + x - Y3;                       // train.jl:14
+                    y3n[q] = (q == 0) ? 0.f : f4(a3[i], q) + x;           // train.jl:14
                     v[q] = (Y2 - y2n[q]) + (Y3 - y3n[q]);
                     xn[q] = v[q] * inv_c0;
                 }
                 st4(P.y2 + col, s, mk4(y2n));
-                st4(P.y3 + col, s, mk4(y3n));
+                st3(P.y3 + col3, s, y3n);
                 st4(P.Xt + col, s, mk4(xn));
                 vb4[TAPS / 4 + s] = mk4(v);
             }
@@ -384,8 +414,8 @@ __global__ __launch_bounds__(kThreads, (NSL <= 4 ? 3 : 2)) void k_csc(const CdlD
         float z = zbase[l];
 #pragma unroll
         for (int ww = 1; ww < kWaves; ++ww) z += zbase[ww * Lg + l];
-        const float Y1o = P.Y1[ro + l];
-        const float y1n = P.y1[ro + l] + P.DX[ro + l] - Y1o;
+        const float Y1o = gp(P.Y1)[ro + l];
+        const float y1n = gp(P.y1)[ro + l] + gp(P.DX)[ro + l] - Y1o;
         zbase[l] = y1n;
         fa[l] = make_float2(z, Y1o - y1n);
     }
@@ -397,8 +427,8 @@ __global__ __launch_bounds__(kThreads, (NSL <= 4 ? 3 : 2)) void k_csc(const CdlD
         const float2 Ck = C[k], Cm = C[k == 0 ? 0 : Lg - k];
         const float2 Z = make_float2(0.5f * (Ck.x + Cm.x), 0.5f * (Ck.y - Cm.y));   // F(z)
         const float2 U = make_float2(0.5f * (Ck.y + Cm.y), -0.5f * (Ck.x - Cm.x));  // F(u)
-        const float ss = P.sS[k];
-        const float den = c0 * P.cS[k];
+        const float ss = gp(P.sS)[k];
+        const float den = c0 * gp(P.cS)[k];
         const float2 hh = make_float2((c0 * U.x - Z.x) / den, (c0 * U.y - Z.y) / den);
         const float2 dx = make_float2(Z.x * inv_c0 + ss * hh.x, Z.y * inv_c0 + ss * hh.y);
         O[k] = make_float2(hh.x - dx.y, hh.y + dx.x);   // h_hat + i DX_hat
@@ -411,13 +441,13 @@ __global__ __launch_bounds__(kThreads, (NSL <= 4 ? 3 : 2)) void k_csc(const CdlD
         const float hn = R[l].x * scale;
         const float DXn = R[l].y * scale;
         const bool win = (l >= P.gamma) && (l < P.gamma + P.L4);
-        const float b = win ? P.s[(size_t)n * P.L4 + (l - P.gamma)] : 0.f;
+        const float b = win ? gp(P.s)[(size_t)n * P.L4 + (l - P.gamma)] : 0.f;
         const float wl = win ? 1.f : 0.f;
         const float y1n = zbase[l];
-        P.h[ro + l] = hn;
-        P.DX[ro + l] = DXn;
-        P.y1[ro + l] = y1n;
-        P.Y1[ro + l] = (b + rho * (DXn + y1n)) / (wl + rho);   // update_Y1! (csc_kernels.jl:91)
+        gp(P.h)[ro + l] = hn;
+        gp(P.DX)[ro + l] = DXn;
+        gp(P.y1)[ro + l] = y1n;
+        gp(P.Y1)[ro + l] = (b + rho * (DXn + y1n)) / (wl + rho);   // update_Y1! (csc_kernels.jl:91)
     }
 }
 
@@ -461,15 +491,15 @@ __global__ __launch_bounds__(kThreads) void k_best(const CdlDev *__restrict__ ta
             }
             if (Xfull) st4(Xfull + col, s, mk4(X));
             if (mat) st4(P.Xt + col, s, mk4(X));
-            if (s >= s0 && s < s0 + L) xc[((size_t)n * M + mm) * L + (s - s0)] = X[0];
+            if (s >= s0 && s < s0 + L) gp(xc)[((size_t)n * M + mm) * L + (s - s0)] = X[0];
         }
         ab += abf;
     }
     if (mat)
-        for (int l = tid; l < Lg; l += kThreads) P.h[ro + l] = 0.f;
+        for (int l = tid; l < Lg; l += kThreads) gp(P.h)[ro + l] = 0.f;
     double fid = 0.0;
     for (int i = tid; i < P.L4; i += kThreads) {
-        const float r = P.DX[ro + P.gamma + i] - P.s[(size_t)n * P.L4 + i];
+        const float r = gp(P.DX)[ro + P.gamma + i] - gp(P.s)[(size_t)n * P.L4 + i];
         fid += (double)r * r;
     }
     double vals[5] = {fid, ab, cnt, sx, sxx};
@@ -480,7 +510,7 @@ __global__ __launch_bounds__(kThreads) void k_best(const CdlDev *__restrict__ ta
             if (tid < st) red[tid] += red[tid + st];
             __syncthreads();
         }
-        if (tid == 0) P.partials[(size_t)n * kPartStride + f] = red[0];
+        if (tid == 0) gp(P.partials)[(size_t)n * kPartStride + f] = red[0];
         __syncthreads();
     }
 }
@@ -492,7 +522,7 @@ __global__ __launch_bounds__(kThreads) void k_reduce_partials(const CdlDev *__re
     const int tid = threadIdx.x;
     for (int f = 0; f < 5; ++f) {
         double a = 0.0;
-        for (int n = tid; n < P.N; n += kThreads) a += P.partials[(size_t)n * kPartStride + f];
+        for (int n = tid; n < P.N; n += kThreads) a += gp(P.partials)[(size_t)n * kPartStride + f];
         red[tid] = a;
         __syncthreads();
         for (int st = kThreads / 2; st > 0; st >>= 1) {
@@ -519,7 +549,7 @@ __global__ __launch_bounds__(kThreads) void k_decide(const CdlDev *__restrict__ 
         const float reg = P.lambda * (float)sc->red[1];
         const double obj = 0.5 * (double)nrm2 + (double)reg;
         int better = decide && (force || (obj < (double)sc->obj_best));
-        if (slot >= 0 && slot < P.trace_cap) P.trace[slot] = obj;
+        if (slot >= 0 && slot < P.trace_cap) gp(P.trace)[slot] = obj;
         sc->red[5] = obj;
         if (decide) sc->improved = better;
         if (better) {
@@ -533,7 +563,7 @@ __global__ __launch_bounds__(kThreads) void k_decide(const CdlDev *__restrict__ 
     }
     __syncthreads();
     if (imp)
-        for (int i = threadIdx.x; i < P.Lg * P.M; i += kThreads) P.Dbest[i] = P.meand[i];
+        for (int i = threadIdx.x; i < P.Lg * P.M; i += kThreads) gp(P.Dbest)[i] = gp(P.meand)[i];
 }
 
 // X on the first sample of slot s only (the stride-4 support of Y3).
@@ -584,7 +614,7 @@ __global__ __launch_bounds__(kThreads) void k_dict(const CdlDev *__restrict__ ta
     const float *G1 = g1cur_of(P);
     const bool mat = !P.sc->xmat;   // set again only by k_cs after this step
 
-    for (int l = tid; l < Lg; l += kThreads) tw[l] = P.tw[l];
+    for (int l = tid; l < Lg; l += kThreads) tw[l] = gp(P.tw)[l];
     if (mat) load_h<TAPS>(P, ro, hb, tid, kThreads);
     for (int k = tid; k < HpX; k += kThreads) { accS[k] = 0.f; accT[k] = make_float2(0.f, 0.f); accP[k] = make_float2(0.f, 0.f); }
 
@@ -595,7 +625,7 @@ __global__ __launch_bounds__(kThreads) void k_dict(const CdlDev *__restrict__ ta
         if (mat)
             for (int i = tid; i < nb * TAPS; i += kThreads) {
                 const int j = i % TAPS;
-                gtap[i] = (j < Lg) ? G1[(size_t)(m0 + i / TAPS) * Lg + j] : 0.f;
+                gtap[i] = (j < Lg) ? gp(G1)[(size_t)(m0 + i / TAPS) * Lg + j] : 0.f;
             }
         if (nb & 1)
             for (int s = tid; s < Pq; s += kThreads) ya[(npair - 1) * Pq + s].y = 0.f;
@@ -605,7 +635,6 @@ __global__ __launch_bounds__(kThreads) void k_dict(const CdlDev *__restrict__ ta
             const int bb = fdivmod(idx, nslot, rslot, s), m = m0 + bb;
             const size_t col = ((size_t)n * M + m) * Lg;
             const float4 xt = ld4(P.Xt + col, s);
-            const float4 a3 = ld4(P.y3 + col, s);
             const float4 dv = ld4(P.d + col, s);
             const float4 a1 = ld4(P.g1 + col, s);
             const float4 c3 = ld4(P.g3 + col, s);
@@ -619,7 +648,7 @@ __global__ __launch_bounds__(kThreads) void k_dict(const CdlDev *__restrict__ ta
                 x0 = X[0];
             }
             float *yaf = reinterpret_cast<float *>(ya);
-            yaf[2 * ((bb >> 1) * Pq + s) + (bb & 1)] = x0 + a3.x;    // Y3 on the stride-4 support
+            yaf[2 * ((bb >> 1) * Pq + s) + (bb & 1)] = x0;    // Y3 = Pi4(X + y3) = Pi4(X): y3 = 0 on the support
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const float qq = (f4(G1v, q) - f4(a1, q)) + (f4(G3v, q) - f4(c3, q));
@@ -637,7 +666,7 @@ __global__ __launch_bounds__(kThreads) void k_dict(const CdlDev *__restrict__ ta
             const float2 v = (bb & 1) ? make_float2(0.5f * (Yk.y + Ym.y), -0.5f * (Yk.x - Ym.x))
                                       : make_float2(0.5f * (Yk.x + Ym.x), 0.5f * (Yk.y - Ym.y));
             ys[bb * Pq + k] = v;
-            if (k < Ph) y3s[(size_t)(m0 + bb) * Ph + k] = v;
+            if (k < Ph) gp(y3s)[(size_t)(m0 + bb) * Ph + k] = v;
         }
         __syncthreads();
         for (int k = tid; k < H; k += kThreads) {
@@ -659,7 +688,7 @@ __global__ __launch_bounds__(kThreads) void k_dict(const CdlDev *__restrict__ ta
         __syncthreads();
     }
     if (mat)
-        for (int l = tid; l < Lg; l += kThreads) P.h[ro + l] = 0.f;
+        for (int l = tid; l < Lg; l += kThreads) gp(P.h)[ro + l] = 0.f;
 
     // ---- per-sequence spectral step: Td, G2/g2 (update_g2!, train.jl:68-71), kappa
     for (int k = tid; k < Lg; k += kThreads) fa[k] = (k < H) ? accT[k] : cconj(accT[Lg - k]);
@@ -673,7 +702,7 @@ __global__ __launch_bounds__(kThreads) void k_dict(const CdlDev *__restrict__ ta
     __syncthreads();
     {
         // F(G2 - g2) of the previous dictionary iteration (sherman_CDL_RHS!, cdl_kernels.jl:8)
-        for (int l = tid; l < Lg; l += kThreads) fa[l] = make_float2(P.G2[ro + l] - P.g2[ro + l], 0.f);
+        for (int l = tid; l < Lg; l += kThreads) fa[l] = make_float2(gp(P.G2)[ro + l] - gp(P.g2)[ro + l], 0.f);
         __syncthreads();
         float2 *R = team_fft<false>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{}, 1);
         const float c1 = P.sh_cdl;
@@ -686,13 +715,13 @@ __global__ __launch_bounds__(kThreads) void k_dict(const CdlDev *__restrict__ ta
     for (int l = tid; l < Lg; l += kThreads) {
         const float Td = tdb[l];
         const bool win = (l >= P.gamma) && (l < P.gamma + P.L4);
-        const float b = win ? P.s[(size_t)n * P.L4 + (l - P.gamma)] : 0.f;
+        const float b = win ? gp(P.s)[(size_t)n * P.L4 + (l - P.gamma)] : 0.f;
         const float wl = win ? 1.f : 0.f;
-        const float g2o = P.g2[ro + l];
+        const float g2o = gp(P.g2)[ro + l];
         const float G2n = (b + P.sigma * (Td + g2o)) / (wl + P.sigma);
-        P.G2[ro + l] = G2n;
-        P.g2[ro + l] = g2o + Td - G2n;
-        P.Td[ro + l] = Td;
+        gp(P.G2)[ro + l] = G2n;
+        gp(P.g2)[ro + l] = g2o + Td - G2n;
+        gp(P.Td)[ro + l] = Td;
     }
     __syncthreads();
 
@@ -704,7 +733,7 @@ __global__ __launch_bounds__(kThreads) void k_dict(const CdlDev *__restrict__ ta
         for (int idx = tid; idx < nb * Ph; idx += kThreads) {
             const int bb = idx / Ph;
             const int k = idx - bb * Ph;
-            const float2 v = y3s[(size_t)(m0 + bb) * Ph + k];
+            const float2 v = gp(y3s)[(size_t)(m0 + bb) * Ph + k];
             ys[bb * Pq + k] = v;
             if (k > 0 && Pq - k >= Ph) ys[bb * Pq + Pq - k] = cconj(v);
         }
@@ -762,14 +791,14 @@ __global__ __launch_bounds__(kThreads) void k_dict_reduce(const CdlDev *__restri
     double a1 = 0.0, a3 = 0.0, ad = 0.0;
     for (int n = n0; n < n1; ++n) {
         const size_t e = (size_t)n * LM + i;
-        const float dv = P.d[e];
-        a1 += (double)(dv + P.g1[e]);
-        a3 += (double)(dv + P.g3[e]);
+        const float dv = gp(P.d)[e];
+        a1 += (double)(dv + gp(P.g1)[e]);
+        a3 += (double)(dv + gp(P.g3)[e]);
         ad += (double)dv;
     }
-    P.dpart[((size_t)c * 3 + 0) * LM + i] = a1;
-    P.dpart[((size_t)c * 3 + 1) * LM + i] = a3;
-    P.dpart[((size_t)c * 3 + 2) * LM + i] = ad;
+    gp(P.dpart)[((size_t)c * 3 + 0) * LM + i] = a1;
+    gp(P.dpart)[((size_t)c * 3 + 1) * LM + i] = a3;
+    gp(P.dpart)[((size_t)c * 3 + 2) * LM + i] = ad;
 }
 
 // Column m: v1 = mean(d+g1), v2 = mean(d+g3); G3 = projection2box!(v2) (cdl_kernels.jl:37-51);
@@ -787,17 +816,17 @@ __global__ __launch_bounds__(kThreads) void k_dict_finalize(const CdlDev *__rest
     float2 *fb = fa + LgMax;
     const int tid = threadIdx.x, Lg = P.Lg;
     const int LM = Lg * P.M;
-    const int flm = P.fl[m];
+    const int flm = gp(P.fl)[m];
     const double invN = 1.0 / (double)P.N;
     float *G1new = P.G1b[(P.sc->g1cur & 1) ^ 1];
-    for (int l = tid; l < Lg; l += kThreads) tw[l] = P.tw[l];
+    for (int l = tid; l < Lg; l += kThreads) tw[l] = gp(P.tw)[l];
     for (int l = tid; l < Lg; l += kThreads) {
         const int i = m * Lg + l;
         double s1 = 0.0, s3 = 0.0, sd = 0.0;
         for (int c = 0; c < P.nch; ++c) {
-            s1 += P.dpart[((size_t)c * 3 + 0) * LM + i];
-            s3 += P.dpart[((size_t)c * 3 + 1) * LM + i];
-            sd += P.dpart[((size_t)c * 3 + 2) * LM + i];
+            s1 += gp(P.dpart)[((size_t)c * 3 + 0) * LM + i];
+            s3 += gp(P.dpart)[((size_t)c * 3 + 1) * LM + i];
+            sd += gp(P.dpart)[((size_t)c * 3 + 2) * LM + i];
         }
         v1[l] = (float)(s1 * invN);
         const float v2 = (float)(s3 * invN);
@@ -806,8 +835,8 @@ __global__ __launch_bounds__(kThreads) void k_dict_finalize(const CdlDev *__rest
         if (v2 >= 1.f) g3 = sup ? 1.f : 0.f;
         else if (v2 <= 0.f) g3 = 0.f;
         else g3 = sup ? v2 : 0.f;
-        P.G3[i] = g3;
-        P.meand[i] = (float)(sd * invN);
+        gp(P.G3)[i] = g3;
+        gp(P.meand)[i] = (float)(sd * invN);
     }
     __syncthreads();
     for (int l = tid; l < Lg; l += kThreads) {
@@ -817,15 +846,15 @@ __global__ __launch_bounds__(kThreads) void k_dict_finalize(const CdlDev *__rest
             const float s4 = v1[gi] + v1[gi + 1] + v1[gi + 2] + v1[gi + 3] - 1.0f;
             g = (float)((double)v1[l] - 0.25 * (double)s4);
         }
-        G1new[(size_t)m * Lg + l] = g;
+        gp(G1new)[(size_t)m * Lg + l] = g;
         fa[l] = make_float2(g, 0.f);
     }
     __syncthreads();
     float2 *C = team_fft<false>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{});
-    for (int k = tid; k < Lg; k += kThreads) P.gsq[(size_t)m * Lg + k] = C[k].x * C[k].x + C[k].y * C[k].y;
+    for (int k = tid; k < Lg; k += kThreads) gp(P.gsq)[(size_t)m * Lg + k] = C[k].x * C[k].x + C[k].y * C[k].y;
 }
 
-// |F(G1[:, m])|^2 for the current filters (constructor and reset).
+// |F(gp(G1)[:, m])|^2 for the current filters (constructor and reset).
 __global__ __launch_bounds__(kThreads) void k_g1_spectrum(const CdlDev *__restrict__ tab, const int *__restrict__ starts,
                                                           int np, int LgMax) {
     extern __shared__ float4 smem4[];
@@ -837,12 +866,12 @@ __global__ __launch_bounds__(kThreads) void k_g1_spectrum(const CdlDev *__restri
     const int tid = threadIdx.x, Lg = P.Lg;
     const float *G1 = g1cur_of(P);
     for (int l = tid; l < Lg; l += kThreads) {
-        tw[l] = P.tw[l];
-        fa[l] = make_float2(G1[(size_t)m * Lg + l], 0.f);
+        tw[l] = gp(P.tw)[l];
+        fa[l] = make_float2(gp(G1)[(size_t)m * Lg + l], 0.f);
     }
     __syncthreads();
     float2 *C = team_fft<false>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{});
-    for (int k = tid; k < Lg; k += kThreads) P.gsq[(size_t)m * Lg + k] = C[k].x * C[k].x + C[k].y * C[k].y;
+    for (int k = tid; k < Lg; k += kThreads) gp(P.gsq)[(size_t)m * Lg + k] = C[k].x * C[k].x + C[k].y * C[k].y;
 }
 
 // s_k = sum_m |G1h|^2 (FP32, m ascending; train.jl:18-19), c_k = s_k + sherman_const_csc.
@@ -858,9 +887,9 @@ __global__ __launch_bounds__(kThreads) void k_cs(const CdlDev *__restrict__ tab,
     }
     if (k >= P.Lg) return;
     float a = 0.f;
-    for (int m = 0; m < P.M; ++m) a += P.gsq[(size_t)m * P.Lg + k];
-    P.sS[k] = a;
-    P.cS[k] = a + P.sh_csc;
+    for (int m = 0; m < P.M; ++m) a += gp(P.gsq)[(size_t)m * P.Lg + k];
+    gp(P.sS)[k] = a;
+    gp(P.cS)[k] = a + P.sh_csc;
 }
 
 // nb transforms of length plan.n (table length plan.n * plan.twstride) by the team FFT, for
