@@ -206,6 +206,46 @@ def served_leg(rate: float, n_jobs: int, log, policy: str = "fifo"):
     }
 
 
+def topology_leg(c, log):
+    """Activation record and topology (record_training_result!, get_spatial_info!,
+    src/SEARCH/store.jl) for two trainings at the single-dataset configuration: the timed
+    problem's last training and a second training from another seed; codes over each
+    training's std threshold (wrap.jl:12), filter_size 5 bases.  Kernel time from HIP events;
+    the kernel's work is the pairs of active entries of each sequence."""
+    import ctypes as Cc
+    import numpy as np
+    import cdlm
+    from cdlm import _lib, synth
+    ctx = cdlm.default_context()
+    bases, _ = synth.c2_dataset(seed=2, N=c.N, L=c.L)
+    c2 = cdlm.CDL(synth.onehot(bases), [FLEN] * M_FILT, RHO, SIGMA, LAM, seed=98765)
+    c2.optimize(ITER_NUM, CSC_ITER, CDL_ITER)
+    rec = cdlm.ActivationRecord(c.N, c.L, M_FILT, max_trainings=2)
+    t0 = time.perf_counter()
+    for prob in (c, c2):
+        th, _ = prob.code_thresh(1.0)
+        rec.add(prob, th, [FLEN // 4] * M_FILT)
+    t_add = time.perf_counter() - t0
+    c2.close()
+    t0 = time.perf_counter()
+    over, non, nxt = rec.topology()
+    t_topo = time.perf_counter() - t0
+    v = Cc.c_double()
+    _lib.check(_lib.load().cdlm_topology_last_ms(ctx.h, Cc.byref(v)), ctx.h)
+    K, T, C, N = rec.dims()
+    act = int(rec.start_positions(0, min(N, 2000)).sum()) * N / min(N, 2000)
+    rec.close()
+    # pair visits: the active entries of each sequence against each other (estimated from
+    # the mean per sequence); the reference's kernel visits (C K)^2 per sequence
+    pairs = (act / N) ** 2 * N
+    log("bench: topology K=%d (2 trainings), ~%.0f active entries, kernel %.2f ms (%.2e pair visits/s), record %.2f s"
+        % (K, act, v.value, pairs / (v.value * 1e-3), t_add))
+    return {"workload": "activation topology of 2 trainings at N=%d L=%d M=%d (filter_size %d)" % (N, C, M_FILT, FLEN // 4),
+            "K": K, "active_entries": act, "kernel_ms": v.value, "record_s": t_add,
+            "pair_visits_per_s": pairs / (v.value * 1e-3), "reference_pair_visits": float(C * K) ** 2 * N,
+            "counts": {"overlap": int(over.sum()), "non_overlap": int(non.sum()), "next_to": int(nxt.sum())}}
+
+
 PWM_L, PWM_K = 500, 512
 MFMA_FP32_PEAK_TFLOPS = 157.3
 
@@ -251,6 +291,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-datasets", type=int, default=16, help="datasets in the batched leg (0: skip it)")
+    ap.add_argument("--no-topology", dest="topology", action="store_false", help="skip the activation-topology leg")
     ap.add_argument("--pwm-seqs", type=int, default=100000, help="sequences in the PWM-scan leg (0: skip it)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="job arrival rate of the served leg (jobs/s)")
     ap.add_argument("--small-datasets", type=int, default=64,
@@ -350,6 +391,8 @@ def main():
         "kernels_ms": {"csc": avg_csc, "dict": dict_ms / max(dict_n, 1), "best": best_ms / max(best_n, 1),
                        "dict_reduce": red_ms / max(red_n, 1)},
     }
+    if args.topology:
+        out["topology"] = topology_leg(c, lambda m: print(m, file=sys.stderr, flush=True))
     c.close()
     del c
     if args.batch_datasets > 0:

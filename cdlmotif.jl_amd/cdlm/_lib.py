@@ -115,6 +115,16 @@ def load():
         "cdlm_server_result": ([vp, C.c_uint64, P(cdlm_job_result), vp, vp, C.c_size_t], C.c_int),
         "cdlm_server_release": ([vp, C.c_uint64], C.c_int),
         "cdlm_server_stats": ([vp, P(cdlm_server_stats_t)], C.c_int),
+        "cdlm_record_create": ([vp, i32, i32, i32, i32, P(vp)], C.c_int),
+        "cdlm_record_destroy": ([vp], C.c_int),
+        "cdlm_record_add": ([vp, vp, f64, vp, vp, P(i32)], C.c_int),
+        "cdlm_record_dims": ([vp, P(i32), P(i32), P(i32), P(i32)], C.c_int),
+        "cdlm_record_filters": ([vp, vp, vp, vp], C.c_int),
+        "cdlm_record_start_positions": ([vp, i32, i32, vp], C.c_int),
+        "cdlm_record_topology_dims": ([vp, P(i32), P(i32), P(i32)], C.c_int),
+        "cdlm_record_topology": ([vp, vp, vp, vp], C.c_int),
+        "cdlm_topology": ([vp, vp, vp, vp, vp, i32, i32, i64, i32, i32, i32, vp, vp, vp], C.c_int),
+        "cdlm_topology_last_ms": ([vp, P(f64)], C.c_int),
         "cdlm_selftest_fft": ([vp, i32, i32, i32, i32, i32, vp, vp], C.c_int),
         "cdlm_batch_create": ([vp, P(vp), i32, P(vp)], C.c_int),
         "cdlm_batch_destroy": ([vp], C.c_int),

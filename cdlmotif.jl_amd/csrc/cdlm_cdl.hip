@@ -761,6 +761,17 @@ extern "C" cdlm_status cdlm_get(cdlm_problem *p, int32_t field, void *dst, size_
     return sync_ctx(p->ctx);
 }
 
+cdlm_status cdlm_problem_xbest_device(cdlm_problem *p, const float **slice, int *L, int *M, int *N) {
+    CdlScal sc;
+    cdlm_status st = read_scal(p, &sc);
+    if (st != CDLM_OK) return st;
+    *slice = p->dv.xs[sc.best_idx & 1];
+    *L = p->L;
+    *M = p->M;
+    *N = p->N;
+    return CDLM_OK;
+}
+
 namespace {
 cdlm_status timing_read(cdlm_ctx *ctx, Timer &t, int kernel, double *total_ms, int64_t *launches) {
     CDLM_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -16,10 +16,15 @@ struct cdlm_ctx {
     std::string last_error;
     int num_cu = 0;
     double last_scan_ms = 0.0;   // device time of the last PWM hit-scan kernel
+    double last_topology_ms = 0.0;   // device time of the last activation-topology kernel
 };
 
 // Records an error message on the context and returns the status.
 cdlm_status cdlm_fail(cdlm_ctx *ctx, cdlm_status st, const char *fmt, ...);
+
+// Device pointer of a problem's current X_best slice (L, M, N) and its dimensions; reads the
+// slice index from the device (synchronises the context stream).
+cdlm_status cdlm_problem_xbest_device(cdlm_problem *p, const float **slice, int *L, int *M, int *N);
 
 #define CDLM_HIP(ctx, expr)                                                                        \
     do {                                                                                           \

@@ -260,6 +260,41 @@ cdlm_status cdlm_selftest_fft(cdlm_ctx *ctx, int32_t n, int32_t twstride, int32_
  * A plain streaming kernel of known traffic, used to calibrate HBM byte counters. */
 cdlm_status cdlm_copy_bench(cdlm_ctx *ctx, size_t bytes, int32_t reps, double *ms_per_copy);
 
+/* ---- activation record and topology ------------------------------------------------------ */
+/* activation_record{T,S}(N, C, M) (src/SETUP/SEARCH_setup.jl:6-54) on the device: up to
+ * max_trainings X_best slices (L, M, N) with their thresholds, the filters with any
+ * activation, their training numbers and lengths, and the cover starts.  C = L. */
+typedef struct cdlm_record cdlm_record;
+cdlm_status cdlm_record_create(cdlm_ctx *ctx, int32_t N, int32_t L, int32_t M, int32_t max_trainings, cdlm_record **out);
+cdlm_status cdlm_record_destroy(cdlm_record *r);
+/* record_training_result!(c, thresh, filter_spec, cover_starts, ar, train_num) (src/SEARCH/store.jl:1-40):
+ * copies the problem's X_best slice (device to device), appends the filters m with any code
+ * > thresh (ascending, store.jl:19) with training number T+1 and length filter_spec[m]
+ * (filter_size in bases, wrap.jl:15), and the cover starts (N values, NULL: all 1, wrap.jl:2);
+ * thresh must be > 0 (store.jl:8).  *n_added = number of filters appended. */
+cdlm_status cdlm_record_add(cdlm_record *r, cdlm_problem *p, double thresh, const int32_t *filter_spec,
+                            const int32_t *cover_starts, int32_t *n_added);
+/* K filters recorded, T trainings, cover width C, N sequences. */
+cdlm_status cdlm_record_dims(const cdlm_record *r, int32_t *K, int32_t *T, int32_t *C, int32_t *N);
+/* ar.uinds (1-based), ar.training_num, ar.filter_lengths: K entries each (any may be NULL). */
+cdlm_status cdlm_record_filters(const cdlm_record *r, int32_t *uinds, int32_t *train_num, int32_t *filter_lengths);
+/* ar.start_positions[n] for n in [n0, n0+count): (C, K) Bool each, as bytes, (C, K, count). */
+cdlm_status cdlm_record_start_positions(cdlm_record *r, int32_t n0, int32_t count, uint8_t *out);
+/* Sizes of the topology: K, maximum_filter_length, maximum_apart_length = C - min length - 1
+ * (src/SEARCH/helpers.jl:9-11). */
+cdlm_status cdlm_record_topology_dims(const cdlm_record *r, int32_t *K, int32_t *max_fil_len, int32_t *apart_len);
+/* get_spatial_info!(ar) (src/SEARCH/store.jl:121-144) with topology_at_nth_pt! (store.jl:64-107):
+ * Int32 counts overlap (K, K, max_fil_len), non_overlap (K, K, apart_len), next_to (K, K),
+ * column-major (activations_topology{int_t}, SEARCH_setup.jl:73-90). */
+cdlm_status cdlm_record_topology(cdlm_record *r, int32_t *overlap, int32_t *non_overlap, int32_t *next_to);
+/* topology_at_nth_pt! over host arrays: start_pos (C, K, N) bytes, train_num (K) 1-based,
+ * fil_lens (K), cover_starts (N, T) 1-based, column-major; outputs as cdlm_record_topology. */
+cdlm_status cdlm_topology(cdlm_ctx *ctx, const uint8_t *start_pos, const int32_t *train_num, const int32_t *fil_lens,
+                          const int32_t *cover_starts, int32_t C, int32_t K, int64_t N, int32_t T, int32_t max_fil_len,
+                          int32_t apart_len, int32_t *overlap, int32_t *non_overlap, int32_t *next_to);
+/* Device time (HIP events) of the last topology kernel. */
+cdlm_status cdlm_topology_last_ms(cdlm_ctx *ctx, double *ms);
+
 /* ---- PWM scan ----------------------------------------------------------------------- */
 /* PWMs are packed (K, 4, maxlen) row-major, pwm[(k*4 + a)*maxlen + j], zero padded past
  * lens[k]; rows a = A,C,G,T.  Sequences are base codes 0..3, (N, L) row-major, one byte

@@ -152,3 +152,46 @@ float orc_best_score(const float *pwm, int maxlen, int m) {
     }
     return acc;
 }
+
+/* ------------------------------------------------------------------------------------------
+ * topology_at_nth_pt! (src/SEARCH/store.jl:64-107) over all sequences, index for index.
+ * start_pos (C, K, N) bytes column-major (0/1); train_num (K) 1-based; fil_lens (K);
+ * cover_starts (N, T) column-major, 1-based; counts are Int32 (activations_topology{int_t},
+ * SEARCH_setup.jl:73-90; int_t = Int32, TYPE_setup.jl:4): over (K, K, maxfl),
+ * non_over (K, K, apart), next_to (K, K), column-major, accumulated (not cleared).
+ * Every (r, c) in 1..C*K x 1..C*K is visited as the reference's threads are, including the
+ * else branch that also takes distances beyond apart (store.jl:98-100).
+ */
+void orc_topology(const uint8_t *start_pos, const int *train_num, const int *fil_lens, const int *cover_starts, int C,
+                  int K, long N, int T, int maxfl, int apart, int *over, int *non_over, int *next_to) {
+    (void)T;
+    const long CK = (long)C * K;
+    for (long n = 1; n <= N; ++n)
+        for (long r = 1; r <= CK; ++r)
+            for (long c = 1; c <= CK; ++c) {
+                const long pr_s_cover = (r - 1) % C + 1;
+                const long fr = ((r - 1) - (pr_s_cover - 1)) / C + 1;
+                const int tr = train_num[fr - 1];
+                const long pr_s = pr_s_cover + cover_starts[(n - 1) + N * (tr - 1)] - 1;
+                const long pr_e = pr_s + fil_lens[fr - 1] - 1;
+                const long pc_s_cover = (c - 1) % C + 1;
+                const long fc = ((c - 1) - (pc_s_cover - 1)) / C + 1;
+                const int tc = train_num[fc - 1];
+                const long pc_s = pc_s_cover + cover_starts[(n - 1) + N * (tc - 1)] - 1;
+                const uint8_t ar = start_pos[(pr_s_cover - 1) + C * ((fr - 1) + K * (n - 1))];
+                const uint8_t ac = start_pos[(pc_s_cover - 1) + C * ((fc - 1) + K * (n - 1))];
+                if (fr != fc && pr_s <= pc_s && ar && ac && tr != tc) {
+                    long d = pc_s - pr_e - 1;
+                    if (!(fil_lens[fr - 1] == fil_lens[fc - 1] && pr_s == pc_s)) {
+                        if (d < 0) {
+                            d = -d;
+                            if (d <= maxfl) over[(fr - 1) + K * ((fc - 1) + (long)K * (d - 1))] += 1;
+                        } else if (d > 0 && d <= apart) {
+                            non_over[(fr - 1) + K * ((fc - 1) + (long)K * (d - 1))] += 1;
+                        } else {
+                            next_to[(fr - 1) + K * (fc - 1)] += 1;
+                        }
+                    }
+                }
+            }
+}

@@ -75,6 +75,7 @@ def _declare(L):
                                C.c_void_p]
     L.orc_best_score.restype = C.c_float
     L.orc_best_score.argtypes = [fp, C.c_int, C.c_int]
+    L.orc_topology.argtypes = [u8, ip, ip, ip, C.c_int, C.c_int, C.c_long, C.c_int, C.c_int, C.c_int, ip, ip, ip]
 
 
 def _f32(a):
@@ -239,3 +240,25 @@ def best_score(pwm_km):
     """pwm_km: (4, m) float32."""
     a = _f32(pwm_km)
     return float(lib().orc_best_score(_ptr(a), a.shape[1], a.shape[1]))
+
+
+def topology(start_pos, train_num, fil_lens, cover_starts, maxfl, apart):
+    """topology_at_nth_pt! (src/SEARCH/store.jl:64-107) over all sequences.  start_pos is the
+    reference's (C, K, N) Bool array given as numpy (N, K, C) uint8; train_num (K,) 1-based,
+    fil_lens (K,), cover_starts the reference's (N, T) matrix given as numpy (T, N).  Returns
+    (over, non_over, next_to) as numpy arrays (maxfl, K, K), (apart, K, K), (K, K) of int32
+    (the reference's column-major (K, K, d) with axes reversed: [d, fc, fr])."""
+    sp = np.ascontiguousarray(start_pos, np.uint8)
+    N, K, Cc = sp.shape
+    tn = np.ascontiguousarray(train_num, np.int32)
+    fl = np.ascontiguousarray(fil_lens, np.int32)
+    cs = np.ascontiguousarray(cover_starts, np.int32)
+    T = cs.shape[0]
+    over = np.zeros((maxfl, K, K), np.int32)
+    non = np.zeros((max(apart, 1), K, K), np.int32)
+    nxt = np.zeros((K, K), np.int32)
+    u8 = C.POINTER(C.c_uint8)
+    ip = C.POINTER(C.c_int)
+    lib().orc_topology(sp.ctypes.data_as(u8), tn.ctypes.data_as(ip), fl.ctypes.data_as(ip), cs.ctypes.data_as(ip),
+                       Cc, K, N, T, maxfl, apart, over.ctypes.data_as(ip), non.ctypes.data_as(ip), nxt.ctypes.data_as(ip))
+    return over, non[:apart], nxt
