@@ -125,6 +125,7 @@ def load():
         "cdlm_record_topology": ([vp, vp, vp, vp], C.c_int),
         "cdlm_topology": ([vp, vp, vp, vp, vp, i32, i32, i64, i32, i32, i32, vp, vp, vp], C.c_int),
         "cdlm_topology_last_ms": ([vp, P(f64)], C.c_int),
+        "cdlm_record_pair_pfms": ([vp, vp, i32, vp, i32, vp, vp], C.c_int),
         "cdlm_selftest_fft": ([vp, i32, i32, i32, i32, i32, vp, vp], C.c_int),
         "cdlm_batch_create": ([vp, P(vp), i32, P(vp)], C.c_int),
         "cdlm_batch_destroy": ([vp], C.c_int),

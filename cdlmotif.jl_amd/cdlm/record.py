@@ -68,6 +68,17 @@ class ActivationRecord:
         check(load().cdlm_record_start_positions(self.h, n0, count, out.ctypes.data), self.ctx.h)
         return out
 
+    def pair_pfms(self, data: np.ndarray, pairs: np.ndarray, max_span_len: int):
+        """cdlm_record_pair_pfms: (span_len (P,), pfms (P, 4, max_span_len))."""
+        pr = np.asfortranarray(np.asarray(pairs, np.int32).reshape(-1, 3))      # (P, 3) column-major
+        P = pr.shape[0]
+        dt = np.ascontiguousarray(data, np.float32)                               # (N, 4C) = (4C, N) column-major
+        span = np.zeros(P, np.int32)
+        pfm = np.zeros((max_span_len, 4, P), np.float32)                          # column-major (P, 4, S)
+        check(load().cdlm_record_pair_pfms(self.h, dt.ctypes.data, P, pr.ctypes.data, max_span_len, span.ctypes.data,
+                                           pfm.ctypes.data), self.ctx.h)
+        return span, np.ascontiguousarray(pfm.transpose(2, 1, 0))
+
     def topology(self):
         """get_spatial_info!(ar): (overlap, non_overlap, next_to) as int32 numpy
         (max_fil_len, K, K), (apart, K, K), (K, K)."""
@@ -79,6 +90,24 @@ class ActivationRecord:
         nxt = np.zeros((K, K), np.int32)
         check(load().cdlm_record_topology(self.h, over.ctypes.data, non.ctypes.data, nxt.ctypes.data), self.ctx.h)
         return over, non, nxt
+
+
+def overlap_pairs(overlap: np.ndarray, count_threshold: int) -> np.ndarray:
+    """findall(at.overlap .> count_threshold) (extractions.jl:27-29) as (P, 3) 1-based
+    (fil1, fil2, overlap) in the reference's column-major order; overlap is numpy (d, K, K)."""
+    idx = np.argwhere(np.asarray(overlap) > count_threshold)      # rows (d, fc, fr), C order = Julia order
+    return np.stack([idx[:, 2] + 1, idx[:, 1] + 1, idx[:, 0] + 1], axis=1).astype(np.int32) if len(idx) else \
+        np.zeros((0, 3), np.int32)
+
+
+def get_pair_pfms(ar: "ActivationRecord", overlap: np.ndarray, count_threshold: int, data: np.ndarray,
+                  filter_size: int):
+    """get_pair_pfms!(at, ar, g) up to the PFMs (extractions.jl:18-100): returns the pairs, the
+    span lengths and the list of PFMs (4, span) of the pairs whose overlap count exceeds
+    count_threshold.  data is the (N, 4L) one-hot matrix; max span = 2 filter_size - 1."""
+    pairs = overlap_pairs(overlap, count_threshold)
+    span, pfm = ar.pair_pfms(data, pairs, 2 * filter_size - 1)
+    return pairs, span, [pfm[i, :, :span[i]] for i in range(len(pairs))]
 
 
 def record_training_result(c, thresh, filter_spec, cover_starts, ar: ActivationRecord, train_num: int) -> int:

@@ -108,6 +108,66 @@ __global__ void k_start_positions(TopoArgs A, long n0, long cnt, uint8_t *out) {
         out[i] = active_of(A, i % CK, n0 + i / CK) ? 1 : 0;
 }
 
+// get_pair_pfms! counts (extractions.jl:115-190): one wave per (pair, chunk of sequences).
+// Per sequence, lanes evaluate the pair-start flag of 64 positions at once (get_f_pair_starts:
+// f1 active at a and f2 active at a + len1 - overlap; positions a <= C - span, msa_fill's
+// range), then every lane l < 4 span adds weight x data[4a + l] for the flagged a in ascending
+// order (msa_fill), weight = (code1 + code2) / 2.  The per-sequence rows are summed over the
+// chunk's sequences in ascending order into partial[pair][chunk][l].
+struct PairArgs {
+    int C, M, S, P, nchunk;
+    long N;
+    const int *pairs;      // (P, 3) column-major, 1-based (fil1, fil2, overlap)
+    const int *uind, *train, *flen;
+    const float *codes;    // (C, M, N, T)
+    const double *thr;
+    const float *data;     // (4C, N)
+    float *partial;        // (P, nchunk, 4S)
+};
+
+__global__ __launch_bounds__(256) void k_pair_msa(PairArgs A) {
+    const int lane = threadIdx.x & 63;
+    const long wv = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv >= (long)A.P * A.nchunk) return;
+    const int i = (int)(wv / A.nchunk), ch = (int)(wv % A.nchunk);
+    const int f1 = A.pairs[i] - 1, f2 = A.pairs[i + A.P] - 1, olen = A.pairs[i + 2 * A.P];
+    const int l1 = A.flen[f1], l2 = A.flen[f2];
+    const int span = olen >= l2 ? olen : l1 - olen + l2;      // get_span_len
+    const int t1 = A.train[f1] - 1, t2 = A.train[f2] - 1;
+    const double th1 = A.thr[t1], th2 = A.thr[t2];
+    const long n0 = (long)ch * A.N / A.nchunk, n1 = (long)(ch + 1) * A.N / A.nchunk;
+    const int last = A.C - span;                                // a in [0, C - span]
+    float sum = 0.f;
+    for (long n = n0; n < n1; ++n) {
+        const float *c1 = A.codes + (size_t)A.C * (A.uind[f1] + (size_t)A.M * ((size_t)n + (size_t)A.N * t1));
+        const float *c2 = A.codes + (size_t)A.C * (A.uind[f2] + (size_t)A.M * ((size_t)n + (size_t)A.N * t2));
+        const float *dn = A.data + (size_t)4 * A.C * n;
+        float acc = 0.f;
+        for (int base = 0; base <= last; base += 64) {
+            const int a = base + lane;
+            bool f = false;
+            float w = 0.f;
+            if (a <= last) {
+                const int b = a + l1 - olen;
+                if (b < A.C) {
+                    const float x1 = c1[a], x2 = c2[b];
+                    f = (double)x1 > th1 && (double)x2 > th2;
+                    w = (x1 + x2) / 2.0f;
+                }
+            }
+            unsigned long long mask = __ballot(f);
+            while (mask) {
+                const int j = __ffsll((long long)mask) - 1;
+                mask &= mask - 1;
+                const float wj = __shfl(w, j);
+                if (lane < 4 * span) acc += wj * dn[4 * (base + j) + lane];
+            }
+        }
+        sum += acc;
+    }
+    if (lane < 4 * A.S) A.partial[((size_t)i * A.nchunk + ch) * 4 * A.S + lane] = sum;
+}
+
 struct DevMem {
     std::vector<void *> ptrs;
     cdlm_ctx *ctx;
@@ -375,5 +435,63 @@ extern "C" cdlm_status cdlm_topology(cdlm_ctx *ctx, const uint8_t *start_pos, co
 extern "C" cdlm_status cdlm_topology_last_ms(cdlm_ctx *ctx, double *ms) {
     if (!ctx || !ms) return CDLM_ERR_ARG;
     *ms = ctx->last_topology_ms;
+    return CDLM_OK;
+}
+
+extern "C" cdlm_status cdlm_record_pair_pfms(cdlm_record *r, const float *data, int32_t P, const int32_t *pairs,
+                                             int32_t max_span_len, int32_t *span_len, float *pfms) {
+    if (!r) return CDLM_ERR_ARG;
+    cdlm_ctx *ctx = r->ctx;
+    const int K = (int)r->uinds.size();
+    if (P < 0 || (P > 0 && (!data || !pairs || !span_len || !pfms)) || max_span_len < 1 || 4 * max_span_len > 64)
+        return cdlm_fail(ctx, CDLM_ERR_ARG, "bad pair-PFM arguments (4 x max_span_len must be <= 64)");
+    if (P == 0) return CDLM_OK;
+    std::vector<int> span(P);
+    for (int i = 0; i < P; ++i) {
+        const int f1 = pairs[i], f2 = pairs[i + P], olen = pairs[i + 2 * P];
+        if (f1 < 1 || f1 > K || f2 < 1 || f2 > K || olen < 1 || olen > r->lens[f1 - 1])
+            return cdlm_fail(ctx, CDLM_ERR_ARG, "pair %d out of range", i);
+        const int l1 = r->lens[f1 - 1], l2 = r->lens[f2 - 1];
+        span[i] = olen >= l2 ? olen : l1 - olen + l2;
+        if (span[i] > max_span_len) return cdlm_fail(ctx, CDLM_ERR_ARG, "span %d of pair %d exceeds max_span_len", span[i], i);
+        span_len[i] = span[i];
+    }
+    DevMem m(ctx);
+    TopoArgs T;
+    cdlm_status st = record_args(r, m, &T);
+    if (st != CDLM_OK) return st;
+    PairArgs A;
+    memset(&A, 0, sizeof(A));
+    A.C = r->C; A.M = r->M; A.S = max_span_len; A.P = P; A.N = r->N;
+    A.uind = T.uind; A.train = T.train; A.flen = T.flen; A.thr = T.thr; A.codes = r->codes;
+    long target = (long)(ctx->num_cu > 0 ? ctx->num_cu : 256) * 16;   // waves to fill the device
+    long nch = (target + P - 1) / P;
+    if (nch > r->N) nch = r->N;
+    if (nch < 1) nch = 1;
+    A.nchunk = (int)nch;
+    if ((st = upload(ctx, m, const_cast<int **>(&A.pairs), pairs, (size_t)P * 3)) != CDLM_OK) return st;
+    if ((st = upload(ctx, m, const_cast<float **>(&A.data), data, (size_t)4 * r->C * r->N)) != CDLM_OK) return st;
+    const size_t npart = (size_t)P * nch * 4 * max_span_len;
+    if ((st = m.alloc(&A.partial, npart)) != CDLM_OK) return st;
+    const long waves = (long)P * nch;
+    hipLaunchKernelGGL(k_pair_msa, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, ctx->stream, A);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return cdlm_fail(ctx, CDLM_ERR_HIP, "k_pair_msa: %s", hipGetErrorString(e));
+    std::vector<float> part(npart);
+    CDLM_HIP(ctx, hipMemcpyAsync(part.data(), A.partial, npart * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    CDLM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // sum over the chunks in order, pseudocount on the span (msa_pseudocount), divide by the
+    // first column's total (extractions.jl:70-77)
+    const int S = max_span_len;
+    std::vector<float> msa(4 * S);
+    for (int i = 0; i < P; ++i) {
+        std::fill(msa.begin(), msa.end(), 0.f);
+        for (long c = 0; c < nch; ++c)
+            for (int l = 0; l < 4 * S; ++l) msa[l] += part[((size_t)i * nch + c) * 4 * S + l];
+        for (int l = 0; l < 4 * span[i]; ++l) msa[l] += 0.1f;
+        const float tot = msa[0] + msa[1] + msa[2] + msa[3];
+        for (int a = 0; a < 4; ++a)
+            for (int sp = 0; sp < S; ++sp) pfms[i + (size_t)P * (a + 4 * (size_t)sp)] = msa[a + 4 * sp] / tot;
+    }
     return CDLM_OK;
 }

@@ -292,6 +292,16 @@ cdlm_status cdlm_record_topology(cdlm_record *r, int32_t *overlap, int32_t *non_
 cdlm_status cdlm_topology(cdlm_ctx *ctx, const uint8_t *start_pos, const int32_t *train_num, const int32_t *fil_lens,
                           const int32_t *cover_starts, int32_t C, int32_t K, int64_t N, int32_t T, int32_t max_fil_len,
                           int32_t apart_len, int32_t *overlap, int32_t *non_overlap, int32_t *next_to);
+/* get_pair_pfms! up to the PFMs (src/SEARCH/extractions.jl:18-100 with get_f_pair_starts,
+ * get_span_len, msa_fill, msa_pseudocount, :115-190): for each pair (fil1, fil2, overlap)
+ * (pairs (P, 3) column-major, 1-based indices into the record's filters; the reference takes
+ * the entries of overlap > count_threshold in column-major order), the weighted one-hot
+ * counts of data (4C, N) over the pair's span at every sequence where fil1 starts at a and
+ * fil2 at a + len1 - overlap, summed over sequences, plus 0.1 on the span, divided by the
+ * first column's total.  Outputs span_len (P) and pfms (P, 4, max_span_len) column-major,
+ * zero past each span.  4 x max_span_len <= 64. */
+cdlm_status cdlm_record_pair_pfms(cdlm_record *r, const float *data, int32_t P, const int32_t *pairs,
+                                  int32_t max_span_len, int32_t *span_len, float *pfms);
 /* Device time (HIP events) of the last topology kernel. */
 cdlm_status cdlm_topology_last_ms(cdlm_ctx *ctx, double *ms);
 

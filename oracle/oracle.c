@@ -195,3 +195,56 @@ void orc_topology(const uint8_t *start_pos, const int *train_num, const int *fil
                 }
             }
 }
+
+/* ------------------------------------------------------------------------------------------
+ * get_pair_pfms! up to the PFMs (src/SEARCH/extractions.jl:18-100, kernels :115-190):
+ * get_f_pair_starts, get_span_len, msa_fill (weights = mean of the two code values, c
+ * ascending, Float32), the sum over sequences (here in ascending n, in Float32), the 0.1
+ * pseudocount on the span (msa_pseudocount) and the division by the first column's total.
+ * Inputs: start_pos (C, K, N) bytes; codes (C, M, N, T) Float32 (the reference's ar.codes with
+ * the training index last); uinds (K) 1-based; train_num (K) 1-based; fil_lens (K);
+ * data (4C, N) one-hot Float32; pairs (P, 3) row-major 1-based (fil1, fil2, overlap).
+ * Outputs: span_len (P); pfm (P, 4, S) row-major with S = max_span_len.
+ */
+void orc_pair_pfms(const uint8_t *start_pos, const float *codes, const int *uinds, const int *train_num,
+                   const int *fil_lens, const float *data, int C, int K, int M, long N, int P, const int *pairs, int S,
+                   int *span_len, float *pfm) {
+    float *msa = (float *)calloc((size_t)4 * S, sizeof(float));
+    uint8_t *fps = (uint8_t *)malloc((size_t)C);
+    for (int i = 0; i < P; ++i) {
+        const int f1 = pairs[3 * i], f2 = pairs[3 * i + 1], olen = pairs[3 * i + 2];
+        const int l1 = fil_lens[f1 - 1], l2 = fil_lens[f2 - 1];
+        const int span = olen >= l2 ? olen : l1 - olen + l2;                       /* get_span_len */
+        span_len[i] = span;
+        memset(msa, 0, sizeof(float) * 4 * S);
+        const int t1 = train_num[f1 - 1], t2 = train_num[f2 - 1];
+        const int u1 = uinds[f1 - 1], u2 = uinds[f2 - 1];
+        for (long n = 0; n < N; ++n) {
+            memset(fps, 0, (size_t)C);
+            for (int c = 1; c <= C; ++c) {                                         /* get_f_pair_starts */
+                const int at = olen + c - l1;
+                if (start_pos[(c - 1) + (size_t)C * ((f2 - 1) + (size_t)K * n)] && at > 0 &&
+                    start_pos[(at - 1) + (size_t)C * ((f1 - 1) + (size_t)K * n)])
+                    fps[at - 1] = 1;
+            }
+            for (int l = 1; l <= 4 * S; ++l) {                                     /* msa_fill */
+                float acc = 0.f;
+                for (int c = 1; c <= C - span + 1; ++c)
+                    if (fps[c - 1] && l <= 4 * span) {
+                        const int s2 = c + l1 - olen;
+                        const float w = (codes[(c - 1) + (size_t)C * ((u1 - 1) + (size_t)M * (n + (size_t)N * (t1 - 1)))] +
+                                         codes[(s2 - 1) + (size_t)C * ((u2 - 1) + (size_t)M * (n + (size_t)N * (t2 - 1)))]) /
+                                        2.0f;
+                        acc += w * data[(size_t)(c - 1) * 4 + (l - 1) + (size_t)4 * C * n];
+                    }
+                msa[l - 1] += acc;                                                 /* sum over n */
+            }
+        }
+        for (int l = 1; l <= 4 * span && l <= 4 * S; ++l) msa[l - 1] += 0.1f;       /* msa_pseudocount */
+        const float tot = msa[0] + msa[1] + msa[2] + msa[3];                       /* first column */
+        for (int a = 0; a < 4; ++a)
+            for (int s = 0; s < S; ++s) pfm[((size_t)i * 4 + a) * S + s] = msa[a + 4 * s] / tot;
+    }
+    free(msa);
+    free(fps);
+}

@@ -76,6 +76,7 @@ def _declare(L):
     L.orc_best_score.restype = C.c_float
     L.orc_best_score.argtypes = [fp, C.c_int, C.c_int]
     L.orc_topology.argtypes = [u8, ip, ip, ip, C.c_int, C.c_int, C.c_long, C.c_int, C.c_int, C.c_int, ip, ip, ip]
+    L.orc_pair_pfms.argtypes = [u8, fp, ip, ip, ip, fp, C.c_int, C.c_int, C.c_int, C.c_long, C.c_int, ip, C.c_int, ip, fp]
 
 
 def _f32(a):
@@ -262,3 +263,27 @@ def topology(start_pos, train_num, fil_lens, cover_starts, maxfl, apart):
     lib().orc_topology(sp.ctypes.data_as(u8), tn.ctypes.data_as(ip), fl.ctypes.data_as(ip), cs.ctypes.data_as(ip),
                        Cc, K, N, T, maxfl, apart, over.ctypes.data_as(ip), non.ctypes.data_as(ip), nxt.ctypes.data_as(ip))
     return over, non[:apart], nxt
+
+
+def pair_pfms(start_pos, codes, uinds, train_num, fil_lens, data, pairs, max_span_len):
+    """get_pair_pfms! up to the PFMs (src/SEARCH/extractions.jl:18-100).  start_pos numpy
+    (N, K, C) uint8; codes numpy (T, N, M, C) float32 (the reference's (C, M, N, T) reversed);
+    uinds, train_num, fil_lens (K,) (1-based indices); data numpy (N, 4C) float32; pairs
+    (P, 3) 1-based (fil1, fil2, overlap).  Returns (span_len (P,), pfms (P, 4, S))."""
+    sp = np.ascontiguousarray(start_pos, np.uint8)
+    N, K, Cc = sp.shape
+    cd = np.ascontiguousarray(codes, np.float32)
+    M = cd.shape[2]
+    pr = np.ascontiguousarray(pairs, np.int32).reshape(-1, 3)
+    P = pr.shape[0]
+    dt = np.ascontiguousarray(data, np.float32)
+    span = np.zeros(P, np.int32)
+    pfm = np.zeros((P, 4, max_span_len), np.float32)
+    ip = C.POINTER(C.c_int)
+    fp = C.POINTER(C.c_float)
+    u = np.ascontiguousarray(uinds, np.int32); t = np.ascontiguousarray(train_num, np.int32)
+    f = np.ascontiguousarray(fil_lens, np.int32)
+    lib().orc_pair_pfms(sp.ctypes.data_as(C.POINTER(C.c_uint8)), cd.ctypes.data_as(fp), u.ctypes.data_as(ip),
+                        t.ctypes.data_as(ip), f.ctypes.data_as(ip), dt.ctypes.data_as(fp), Cc, K, M, N, P,
+                        pr.ctypes.data_as(ip), max_span_len, span.ctypes.data_as(ip), pfm.ctypes.data_as(fp))
+    return span, pfm

@@ -35,21 +35,25 @@ def test_topology_hand_case_and_random():
             np.testing.assert_array_equal(g, r)
 
 
-def _train(seed, N, L, M, fl):
+def _train(seed, N, L, M, fl, with_data=False, data_seed=None):
+    """One training (initialisation from seed) on the dataset of data_seed (default: seed);
+    the reference's trainings of one search share the dataset (wrap.jl:4-19)."""
     from cdlm import synth
-    bases, _ = synth.planted_bases(seed, N, L, [6, 8], 0.4)
-    c = cdlm.CDL(synth.onehot(bases), [fl] * M, 1.0, 100.0, 0.05, seed=seed)
+    bases, _ = synth.planted_bases(seed if data_seed is None else data_seed, N, L, [6, 8], 0.4)
+    data = synth.onehot(bases)
+    c = cdlm.CDL(data, [fl] * M, 1.0, 100.0, 0.05, seed=seed)
     c.optimize(4, 6, 1)
-    return c
+    return (c, data) if with_data else c
 
 
 def test_record_two_trainings_matches_oracle():
     N, L, M, fl = 60, 40, 6, 12
     rec = cdlm.ActivationRecord(N, L, M, max_trainings=2)
     spec = [fl // 4] * M                      # filter_size in bases (wrap.jl:15)
-    expect_sp, expect_u, expect_t, expect_f = [], [], [], []
+    expect_sp, expect_u, expect_t, expect_f, expect_xs = [], [], [], [], []
+    data = None
     for t, seed in enumerate((11, 12)):
-        c = _train(seed, N, L, M, fl)
+        c, data = _train(seed, N, L, M, fl, with_data=True, data_seed=11)
         th, _ = c.code_thresh(1.0)
         xs = c.get("Xbest_slice")             # (N, M, L)
         act = xs.astype(np.float64) > th      # Float32 codes against a Float64 threshold
@@ -60,6 +64,7 @@ def test_record_two_trainings_matches_oracle():
         expect_t += [t + 1] * len(uinds)
         expect_f += [spec[u] for u in uinds]
         expect_sp.append(act[:, uinds, :])    # (N, k_t, L)
+        expect_xs.append(xs)
         c.close()
     u, tn, fls = rec.filters()
     np.testing.assert_array_equal(u, expect_u)
@@ -76,6 +81,18 @@ def test_record_two_trainings_matches_oracle():
     for g, r in zip(got, ref):
         np.testing.assert_array_equal(g, r)
     assert sum(int(r.sum()) for r in ref) > 0
+    # pair PFMs (get_pair_pfms!, extractions.jl:18-100) of every overlapping pair; the
+    # per-sequence rows follow the reference's order, the sum over sequences is chunked, so the
+    # comparison is to 1e-5 relative
+    pairs = cdlm.overlap_pairs(got[0], 0)
+    assert len(pairs) > 0
+    codes = np.stack([xs_t for xs_t in expect_xs])                 # (T, N, M, C)
+    span, pfm = rec.pair_pfms(data, pairs, 2 * (fl // 4) - 1)
+    rspan, rpfm = O.pair_pfms(sp, codes, u, tn, fls, data, pairs, 2 * (fl // 4) - 1)
+    flush("pair PFMs: %d pairs" % len(pairs))
+    np.testing.assert_array_equal(span, rspan)
+    np.testing.assert_allclose(pfm, rpfm, rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose(pfm[:, :, 0].sum(axis=1), 1.0, rtol=1e-5)
     with pytest.raises(cdlm.CdlmError):
         c = _train(13, N, L, M, fl)
         rec.add(c, 0.5, spec)                 # a third training in a two-training record

@@ -92,3 +92,33 @@ def test_topology_matches_sparse_restatement(seed):
     for g, r in zip(got, ref):
         np.testing.assert_array_equal(g, r)
     assert sum(int(g.sum()) for g in got) > 0
+
+
+def _pair_case():
+    """C = 6, f1 (training 1, length 2), f2 (training 2, length 2), pair (1, 2, overlap 1):
+    span = 2 - 1 + 2 = 3 (get_span_len), max span 3.
+    Sequence 1 'ACGTAC': f1 at 2 (code 0.6), f2 at 3 (0.2): weight 0.4 over C,G,T;
+      f1 at 5 (0.9) with f2 at 6 (0.9) is flagged but c = 5 > C - span + 1 = 4 (ignored).
+    Sequence 2 'AAAAAA': f1 at 1 (1.0), f2 at 2 (0.6): weight 0.8 over A,A,A.
+    Counts + 0.1: pos1 A .9 C .5 G .1 T .1 (total 1.6), pos2 A .9 C .1 G .5 T .1,
+    pos3 A .9 C .1 G .1 T .5; divided by 1.6."""
+    C, N = 6, 2
+    sp = np.zeros((N, 2, C), np.uint8)
+    codes = np.zeros((2, N, 1, C), np.float32)        # (T, N, M, C)
+    for n, k, p, v in [(0, 0, 2, 0.6), (0, 1, 3, 0.2), (0, 0, 5, 0.9), (0, 1, 6, 0.9), (1, 0, 1, 1.0), (1, 1, 2, 0.6)]:
+        sp[n, k, p - 1] = 1
+        codes[k, n, 0, p - 1] = v
+    seqs = ["ACGTAC", "AAAAAA"]
+    data = np.zeros((N, 4 * C), np.float32)
+    for n, sq in enumerate(seqs):
+        for i, b in enumerate(sq):
+            data[n, 4 * i + "ACGT".index(b)] = 1.0
+    return sp, codes, np.array([1, 1]), np.array([1, 2]), np.array([2, 2]), data, np.array([[1, 2, 1]]), 3
+
+
+def test_pair_pfms_hand_case():
+    sp, codes, uinds, train, lens, data, pairs, S = _pair_case()
+    span, pfm = O.pair_pfms(sp, codes, uinds, train, lens, data, pairs, S)
+    assert span.tolist() == [3]
+    e = np.array([[0.9, 0.9, 0.9], [0.5, 0.1, 0.1], [0.1, 0.5, 0.1], [0.1, 0.1, 0.5]], np.float32) / np.float32(1.6)
+    np.testing.assert_allclose(pfm[0], e, rtol=1e-6)
