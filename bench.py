@@ -234,16 +234,24 @@ def topology_leg(c, log):
     _lib.check(_lib.load().cdlm_topology_last_ms(ctx.h, Cc.byref(v)), ctx.h)
     K, T, C, N = rec.dims()
     act = int(rec.start_positions(0, min(N, 2000)).sum()) * N / min(N, 2000)
+    # pair PFMs of the pairs overlapping in more than count_threshold = 50 sequences
+    # (extractions.jl:27; WRAPPER_setup.jl default)
+    data = synth.onehot(bases)
+    t0 = time.perf_counter()
+    pairs, spans, pfms = cdlm.get_pair_pfms(rec, over, 50, data, FLEN // 4)
+    t_pfm = time.perf_counter() - t0
     rec.close()
     # pair visits: the active entries of each sequence against each other (estimated from
     # the mean per sequence); the reference's kernel visits (C K)^2 per sequence
-    pairs = (act / N) ** 2 * N
-    log("bench: topology K=%d (2 trainings), ~%.0f active entries, kernel %.2f ms (%.2e pair visits/s), record %.2f s"
-        % (K, act, v.value, pairs / (v.value * 1e-3), t_add))
+    npairs = (act / N) ** 2 * N
+    log("bench: topology K=%d (2 trainings), ~%.0f active entries, kernel %.2f ms (%.2e pair visits/s), record %.2f s;"
+        " %d pair PFMs in %.3f s" % (K, act, v.value, npairs / (v.value * 1e-3), t_add, len(pairs), t_pfm))
     return {"workload": "activation topology of 2 trainings at N=%d L=%d M=%d (filter_size %d)" % (N, C, M_FILT, FLEN // 4),
             "K": K, "active_entries": act, "kernel_ms": v.value, "record_s": t_add,
-            "pair_visits_per_s": pairs / (v.value * 1e-3), "reference_pair_visits": float(C * K) ** 2 * N,
-            "counts": {"overlap": int(over.sum()), "non_overlap": int(non.sum()), "next_to": int(nxt.sum())}}
+            "pair_visits_per_s": npairs / (v.value * 1e-3), "reference_pair_visits": float(C * K) ** 2 * N,
+            "counts": {"overlap": int(over.sum()), "non_overlap": int(non.sum()), "next_to": int(nxt.sum())},
+            "pair_pfms": {"pairs": int(len(pairs)), "seconds": t_pfm,
+                          "note": "wall time of the call, including the 64 MB data upload"}}
 
 
 PWM_L, PWM_K = 500, 512
