@@ -75,6 +75,7 @@ __global__ __launch_bounds__(256) void k_pwm_hits_mfma(const uint8_t *__restrict
         sth[gi] = ok ? P.slot_th[sl] : 0.f;
     }
     const int arow = lane & 15, abase = lane >> 4;
+    const int jsteps = (gmax + 3) / 4;   // uniform: the tiles of this work-group
     const long nstride = (long)gridDim.y * 4;
     for (long n = (long)blockIdx.y * 4 + w; n < N; n += nstride) {
         const uint8_t *src = bases + n * (long)L;
@@ -84,14 +85,31 @@ __global__ __launch_bounds__(256) void k_pwm_hits_mfma(const uint8_t *__restrict
             floatx4 acc[G];
 #pragma unroll
             for (int gi = 0; gi < G; ++gi) acc[gi] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int j = 0; j < MAXJ; ++j) {
-                if (j < gmax) {
-                    const float a = (sq[w0 + arow + j] == abase) ? 1.0f : 0.0f;
-#pragma unroll
-                    for (int gi = 0; gi < G; ++gi) acc[gi] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bv[gi][j], acc[gi], 0, 0, 0);
-                }
+            // the group's positions in whole steps of 4 (B is zero past each PWM's length); one
+            // uniform switch per chunk keeps every case branch-free, so the accumulators stay in
+            // place across the MFMA chain
+#define CDLM_MFMA_STEPS(JS)                                                                          \
+    case JS / 4:                                                                                     \
+        _Pragma("unroll") for (int j = 0; j < JS; ++j) {                                             \
+            const float a = (sq[w0 + arow + j] == abase) ? 1.0f : 0.0f;                             \
+            _Pragma("unroll") for (int gi = 0; gi < G; ++gi)                                         \
+                acc[gi] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bv[gi][j], acc[gi], 0, 0, 0);      \
+        }                                                                                            \
+        break;
+            switch (jsteps) {
+                CDLM_MFMA_STEPS(4) CDLM_MFMA_STEPS(8) CDLM_MFMA_STEPS(12) CDLM_MFMA_STEPS(16)
+                CDLM_MFMA_STEPS(20) CDLM_MFMA_STEPS(24) CDLM_MFMA_STEPS(28) CDLM_MFMA_STEPS(32)
+                default:
+                    if constexpr (MAXJ > 32) {
+                        switch (jsteps) {
+                            CDLM_MFMA_STEPS(36) CDLM_MFMA_STEPS(40) CDLM_MFMA_STEPS(44) CDLM_MFMA_STEPS(48)
+                            CDLM_MFMA_STEPS(52) CDLM_MFMA_STEPS(56) CDLM_MFMA_STEPS(60) CDLM_MFMA_STEPS(64)
+                            default: break;
+                        }
+                    }
+                    break;
             }
+#undef CDLM_MFMA_STEPS
             int myc = 0;
             unsigned hitmask = 0;
 #pragma unroll
