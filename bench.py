@@ -258,6 +258,30 @@ PWM_L, PWM_K = 500, 512
 MFMA_FP32_PEAK_TFLOPS = 157.3
 
 
+def ingest_leg(bases, s, fl, D0, log):
+    """Problem creation from the host one-hot matrix (4 bytes x 4L per sequence over the bus)
+    against creation from 1-byte base codes expanded on the device; both must give the same
+    objective."""
+    import cdlm
+    out = {}
+    for kind in ("onehot", "bases"):
+        t0 = time.perf_counter()
+        if kind == "onehot":
+            c = cdlm.CDL(s, fl, RHO, SIGMA, LAM, D0=D0, seed=1234)
+        else:
+            c = cdlm.CDL(None, fl, RHO, SIGMA, LAM, D0=D0, seed=1234, bases=bases)
+        dt = time.perf_counter() - t0
+        out[kind] = {"create_ms": 1000.0 * dt, "host_bytes": int(s.nbytes if kind == "onehot" else bases.size),
+                     "objective": c.objective()}
+        c.close()
+        del c
+    out["identical"] = out["onehot"]["objective"] == out["bases"]["objective"]
+    log("bench: ingest: create from one-hot %.1f ms (%d B), from base codes %.1f ms (%d B), objectives equal %s"
+        % (out["onehot"]["create_ms"], out["onehot"]["host_bytes"], out["bases"]["create_ms"],
+           out["bases"]["host_bytes"], out["identical"]))
+    return out
+
+
 def pwm_leg(n_seq: int, log):
     """PWM scan sample: n_seq sequences of L=500 iid bases (GC 0.42) against 512 log-odds
     PWMs of length 8-24 (the PWM-scan configuration at a bounded N), both strands,
@@ -300,6 +324,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-datasets", type=int, default=16, help="datasets in the batched leg (0: skip it)")
     ap.add_argument("--no-topology", dest="topology", action="store_false", help="skip the activation-topology leg")
+    ap.add_argument("--no-ingest", dest="ingest", action="store_false", help="skip the problem-creation (ingest) leg")
     ap.add_argument("--pwm-seqs", type=int, default=100000, help="sequences in the PWM-scan leg (0: skip it)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="job arrival rate of the served leg (jobs/s)")
     ap.add_argument("--small-datasets", type=int, default=64,
@@ -403,6 +428,8 @@ def main():
         out["topology"] = topology_leg(c, lambda m: print(m, file=sys.stderr, flush=True))
     c.close()
     del c
+    if args.ingest:
+        out["ingest"] = ingest_leg(bases, s, fl, D0, lambda m: print(m, file=sys.stderr, flush=True))
     if args.batch_datasets > 0:
         out["batched"] = batched_leg(args.batch_datasets, lambda m: print(m, file=sys.stderr, flush=True))
         if args.small_datasets > 0:

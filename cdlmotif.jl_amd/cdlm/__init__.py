@@ -11,6 +11,7 @@ from ._lib import CdlmError, CdlmOOM, load  # noqa: F401
 from .device import Context, default_context  # noqa: F401
 from .cdl import CDL, Batch, footprint_bytes, pack_by_footprint, autotune, tuning_set, tuning_get, CDL_setup, init_cdl, code_update, dict_update, update_best, objective, cdl_optimize  # noqa: F401
 from .serve import Server  # noqa: F401
+from . import io  # noqa: F401
 from .record import (ActivationRecord, record_training_result, get_spatial_info, topology,  # noqa: F401
                      overlap_pairs, get_pair_pfms)
 from .scan import (pfm2pwm, pwm_complement, best_score, pack_pwms, scan_hits, scan_w_gpu,  # noqa: F401

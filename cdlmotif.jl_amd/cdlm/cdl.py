@@ -57,13 +57,20 @@ def _f32c(a):
 class CDL:
     """Device-resident state of one CDL problem (the reference's mutable struct `cdl`)."""
 
-    def __init__(self, data: np.ndarray, filter_len: Sequence[int], rho: float, sigma: float, lam: float,
+    def __init__(self, data: Optional[np.ndarray], filter_len: Sequence[int], rho: float, sigma: float, lam: float,
                  D0: Optional[np.ndarray] = None, X0: Optional[np.ndarray] = None, seed: int = 0,
-                 sherman_csc: float = 2.0, sherman_cdl: float = 2.0, ctx: Optional[Context] = None):
+                 sherman_csc: float = 2.0, sherman_cdl: float = 2.0, ctx: Optional[Context] = None,
+                 bases: Optional[np.ndarray] = None):
+        """data: (N, 4L) one-hot float32; or bases: (N, L) uint8 codes 0..3 (one-hot built on the
+        device, cdlm_problem_create_from_bases)."""
         self.ctx = ctx or default_context()
         L = load()
-        data = _f32c(data)                     # (N, 4L)
-        N, L4 = data.shape
+        if bases is not None:
+            bases = np.ascontiguousarray(bases, np.uint8)
+            N, L4 = bases.shape[0], 4 * bases.shape[1]
+        else:
+            data = _f32c(data)                 # (N, 4L)
+            N, L4 = data.shape
         fl = np.ascontiguousarray(filter_len, dtype=np.int32)
         self._fl = fl
         prm = cdlm_params(len(fl), fl.ctypes.data_as(C.POINTER(C.c_int32)), rho, sigma, lam, sherman_csc, sherman_cdl)
@@ -76,9 +83,14 @@ class CDL:
         if x0 is not None and x0.shape != (N, len(fl), Lg):
             raise ValueError("X0 must be (N, M, Lg) = %s, got %s" % ((N, len(fl), Lg), x0.shape))
         h = C.c_void_p()
-        st = L.cdlm_problem_create(self.ctx.h, C.byref(prm), data.ctypes.data, L4, N,
-                                   None if d0 is None else d0.ctypes.data, None if x0 is None else x0.ctypes.data,
-                                   seed & 0xFFFFFFFFFFFFFFFF, C.byref(h))
+        d0p = None if d0 is None else d0.ctypes.data
+        x0p = None if x0 is None else x0.ctypes.data
+        if bases is not None:
+            st = L.cdlm_problem_create_from_bases(self.ctx.h, C.byref(prm), bases.ctypes.data, L4 // 4, N, d0p, x0p,
+                                                  seed & 0xFFFFFFFFFFFFFFFF, C.byref(h))
+        else:
+            st = L.cdlm_problem_create(self.ctx.h, C.byref(prm), data.ctypes.data, L4, N, d0p, x0p,
+                                       seed & 0xFFFFFFFFFFFFFFFF, C.byref(h))
         check(st, self.ctx.h)
         self.h = h
         dims = [C.c_int32() for _ in range(5)]

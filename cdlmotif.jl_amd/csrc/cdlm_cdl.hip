@@ -499,10 +499,42 @@ extern "C" size_t cdlm_footprint_bytes(int32_t L4, int32_t N, int32_t M, int32_t
     return plan_total(buf_plan(L4, N, M, gamma, keep_x0 != 0));
 }
 
+namespace {
+// One-hot data (4L, N) column-major from base codes (N, L) row-major: s[4l + a, n] = 1 iff
+// code(n, l) == a (data_2_dummy, to_dummy.jl:2-38); a code > 3 gives an all-zero group.
+__global__ void k_bases_to_onehot(const uint8_t *__restrict__ codes, int L, long N, float *__restrict__ s) {
+    const long tot = (long)L * N;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (long)gridDim.x * blockDim.x) {
+        const int b = codes[i];
+        float4 v = make_float4(b == 0 ? 1.f : 0.f, b == 1 ? 1.f : 0.f, b == 2 ? 1.f : 0.f, b == 3 ? 1.f : 0.f);
+        reinterpret_cast<float4 *>(s)[i] = v;   // (n, l) row-major = column n, rows 4l..4l+3
+    }
+}
+
+cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const float *s, const uint8_t *codes, int32_t L4,
+                                int32_t N, const float *D0, const float *X0, uint64_t seed, cdlm_problem **out);
+}  // namespace
+
 extern "C" cdlm_status cdlm_problem_create(cdlm_ctx *ctx, const cdlm_params *prm, const float *s, int32_t L4, int32_t N,
                                            const float *D0, const float *X0, uint64_t seed, cdlm_problem **out) {
     if (!ctx) return CDLM_ERR_ARG;
-    if (!prm || !s || !out || !prm->filter_len) return cdlm_fail(ctx, CDLM_ERR_ARG, "null argument");
+    if (!s) return cdlm_fail(ctx, CDLM_ERR_ARG, "null argument");
+    return problem_create_impl(ctx, prm, s, nullptr, L4, N, D0, X0, seed, out);
+}
+
+extern "C" cdlm_status cdlm_problem_create_from_bases(cdlm_ctx *ctx, const cdlm_params *prm, const uint8_t *bases,
+                                                      int32_t L, int32_t N, const float *D0, const float *X0, uint64_t seed,
+                                                      cdlm_problem **out) {
+    if (!ctx) return CDLM_ERR_ARG;
+    if (!bases) return cdlm_fail(ctx, CDLM_ERR_ARG, "null argument");
+    if (L <= 0 || L > (1 << 28)) return cdlm_fail(ctx, CDLM_ERR_ARG, "bad sequence length %d", L);
+    return problem_create_impl(ctx, prm, nullptr, bases, 4 * L, N, D0, X0, seed, out);
+}
+
+namespace {
+cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const float *s, const uint8_t *codes, int32_t L4,
+                                int32_t N, const float *D0, const float *X0, uint64_t seed, cdlm_problem **out) {
+    if (!prm || !out || !prm->filter_len) return cdlm_fail(ctx, CDLM_ERR_ARG, "null argument");
     *out = nullptr;
     if (L4 <= 0 || L4 % 4 != 0) return cdlm_fail(ctx, CDLM_ERR_ARG, "first dimension of dat must be a positive multiple of 4 (got %d)", L4);
     if (N <= 0) return cdlm_fail(ctx, CDLM_ERR_ARG, "N must be positive (got %d)", N);
@@ -571,7 +603,24 @@ extern "C" cdlm_status cdlm_problem_create(cdlm_ctx *ctx, const cdlm_params *prm
         const double a = -2.0 * M_PI * (double)t / (double)Lg;
         tw[t] = make_float2((float)cos(a), (float)sin(a));
     }
-    hipError_t e = hipMemcpyAsync(s_dev, s, b.s, hipMemcpyHostToDevice, hs);
+    hipError_t e = hipSuccess;
+    if (s) {
+        e = hipMemcpyAsync(s_dev, s, b.s, hipMemcpyHostToDevice, hs);
+    } else {
+        // base codes: N*L bytes over the bus, the one-hot expansion on the device
+        uint8_t *cd = nullptr;
+        const size_t nb = (size_t)(L4 / 4) * N;
+        e = hipMalloc(&cd, nb);
+        if (e == hipSuccess) e = hipMemcpyAsync(cd, codes, nb, hipMemcpyHostToDevice, hs);
+        if (e == hipSuccess) {
+            long grid = ((long)nb + 255) / 256;
+            if (grid > 65536) grid = 65536;
+            hipLaunchKernelGGL(k_bases_to_onehot, dim3((unsigned)grid), dim3(256), 0, hs, cd, L4 / 4, (long)N, s_dev);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(hs);
+        if (cd) (void)hipFree(cd);
+    }
     if (e == hipSuccess && X0) e = hipMemcpyAsync(X0_dev, X0, b.Ex0, hipMemcpyHostToDevice, hs);
     if (e == hipSuccess) e = hipMemcpyAsync(D0_dev, D0m.data(), b.LM, hipMemcpyHostToDevice, hs);
     if (e == hipSuccess) e = hipMemcpyAsync(fl_dev, p->fl.data(), b.M, hipMemcpyHostToDevice, hs);
@@ -592,6 +641,7 @@ extern "C" cdlm_status cdlm_problem_create(cdlm_ctx *ctx, const cdlm_params *prm
     *out = p.release();
     return CDLM_OK;
 }
+}  // namespace
 
 extern "C" cdlm_status cdlm_problem_destroy(cdlm_problem *p) {
     if (!p) return CDLM_OK;

@@ -72,6 +72,12 @@ cdlm_status cdlm_mem_info(cdlm_ctx *ctx, size_t *free_bytes, size_t *total_bytes
  * CUDA.randn of CDL_setup.jl:191).  The problem state stays resident on the device. */
 cdlm_status cdlm_problem_create(cdlm_ctx *ctx, const cdlm_params *params, const float *s, int32_t L4, int32_t N,
                                 const float *D0, const float *X0, uint64_t seed, cdlm_problem **out);
+/* As cdlm_problem_create, from base codes instead of the one-hot matrix: bases (N, L)
+ * row-major, codes 0..3 = A,C,G,T (any other code: an all-zero 4-group); the one-hot data
+ * (data_2_dummy, src/LOAD/to_dummy.jl:2-38) is built on the device, so N*L bytes cross the bus
+ * instead of 16*N*L. */
+cdlm_status cdlm_problem_create_from_bases(cdlm_ctx *ctx, const cdlm_params *params, const uint8_t *bases, int32_t L,
+                                           int32_t N, const float *D0, const float *X0, uint64_t seed, cdlm_problem **out);
 cdlm_status cdlm_problem_destroy(cdlm_problem *p);
 /* Re-runs the initialisation from the same D0 / X0 / seed without re-uploading s. */
 cdlm_status cdlm_problem_reset(cdlm_problem *p);
