@@ -91,8 +91,7 @@ def batched_leg(n_datasets: int, log, N_range=(200, 20000), cache=True):
     # launch choices per shape (radix order, dictionary batch, CSC slots), outside the timing
     shapes = sorted({(sp["L"], sp["M"], sp["filter_len"]) for sp in specs})
     t0 = time.perf_counter()
-    for (L, M, f) in shapes:
-        cdlm.autotune(L, M, f, ctx=ctx)
+    tunings = {shape: cdlm.autotune(*shape, ctx=ctx) for shape in shapes}
     log("bench: autotuned %d shapes in %.2f s" % (len(shapes), time.perf_counter() - t0))
     # first use of every kernel variant outside the timed regions
     warm = [cdlm.CDL(np.zeros((4, 4 * 8), np.float32), [f] * 2, RHO, SIGMA, LAM, seed=1, ctx=ctx) for f in (16, 20, 24)]
@@ -134,7 +133,7 @@ def batched_leg(n_datasets: int, log, N_range=(200, 20000), cache=True):
     log("bench: batched workload, batched: %d waves, %.2f s" % (len(waves), t_batch))
     seq_iters = sum(sp["N"] for sp in specs) * C3_ITER_NUM * C3_CSC_ITER
     if cache:
-        batched_leg.cache = (specs, data, t_each, th_plain)
+        batched_leg.cache = (specs, data, t_each, th_plain, tunings)
     return {
         "workload": "%d datasets of the batched configuration (seeds 3..%d, N in [%d, %d]), 150 ADMM iterations each"
                     % (len(specs), 2 + len(specs), N_range[0], N_range[1]),
@@ -155,8 +154,12 @@ def served_leg(rate: float, n_jobs: int, log, policy: str = "fifo"):
     import threading
     import numpy as np
     import cdlm
-    specs, data, t_each, th_plain = batched_leg.cache
+    specs, data, t_each, th_plain, tunings = batched_leg.cache
     n = min(n_jobs, len(specs))
+    # the launch choices the plain results were computed with (a later leg may have re-tuned
+    # a shape; the FFT stage order changes rounding, so another choice changes result bits)
+    for shape, choice in tunings.items():
+        cdlm.tuning_set(*shape, choice)
     gaps = np.random.default_rng(4).exponential(1.0 / rate, n)
     arrive = np.cumsum(gaps) - gaps[0]
     srv = cdlm.Server(csc_iter=C3_CSC_ITER, cdl_iter=C3_CDL_ITER, policy=policy)
@@ -308,8 +311,21 @@ def pwm_leg(n_seq: int, log):
     tf = flops / (k_ms * 1e-3) / 1e12
     log("bench: PWM scan %d x %d, %d PWMs: kernel %.2f ms (%.1f TFLOP/s), %d hits, call %.2f s"
         % (n_seq, PWM_L, PWM_K, k_ms, tf, len(h), wall))
+    # score thresholds of the same PWMs for the reference's p-value (get_thresh, pvalue 2.7e-4
+    # as Float32): exact score distributions, one work-group per PWM
+    pv = float(np.float32(2.7e-4))
+    th_ms, th_wall = [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        th = cdlm.get_thresh((pwms, lens), pv)
+        th_wall.append(1000.0 * (time.perf_counter() - t0))
+        th_ms.append(cdlm.pvalue_last_ms())
+    log("bench: p-value thresholds of %d PWMs: kernels %.3f ms, call %.2f ms, mean threshold %.3f"
+        % (PWM_K, float(np.median(th_ms)), float(np.median(th_wall)), float(th.mean())))
     return {"workload": "PWM scan: %d sequences x L=%d, %d PWMs of length 8-24, both strands, hit list" % (n_seq, PWM_L, PWM_K),
             "kernel_ms": k_ms, "windows_per_s": windows / (k_ms * 1e-3), "hits": int(len(h)),
+            "pvalue_thresholds": {"pwms": PWM_K, "pvalue": pv, "kernel_ms": float(np.median(th_ms)),
+                                  "call_ms": float(np.median(th_wall))},
             "roofline": {"bound": "mfma", "achieved": tf, "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": tf / MFMA_FP32_PEAK_TFLOPS, "algorithmic_flops": flops}}
 

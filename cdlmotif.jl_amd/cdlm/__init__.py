@@ -15,4 +15,5 @@ from . import io  # noqa: F401
 from .record import (ActivationRecord, record_training_result, get_spatial_info, topology,  # noqa: F401
                      overlap_pairs, get_pair_pfms)
 from .scan import (pfm2pwm, pwm_complement, best_score, pack_pwms, scan_hits, scan_w_gpu,  # noqa: F401
-                   overlapping_scan, onehot_to_bases)
+                   overlapping_scan, onehot_to_bases, get_thresh, pvalue_last_ms)
+from .evalue import fisher_pvalue, evalues, filter_using_evalue, sequences_with_hits  # noqa: F401

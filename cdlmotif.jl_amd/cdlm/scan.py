@@ -122,6 +122,33 @@ def scan_w_gpu_dicts(bases, pwms: List[np.ndarray], thresh, ctx: Optional[Contex
     return [{int(n) + 1: int(p) for n, p in enumerate(row) if p > 0} for row in pos]
 
 
+def get_thresh(pwms, pval: float, eps: float = 0.1, ctx: Optional[Context] = None) -> np.ndarray:
+    """get_thresh(pwms, pval, Float32) (helpers.jl:99) on the device: Float32 of
+    pvalue2score(pwm, pval) per PWM (PWM_Touzet.jl:199-217).  pwms: a list of (4, m) arrays or
+    a packed (K, 4, maxlen) array with lens.  pval is taken as given; the reference passes a
+    Float32 (wrap.jl:125), so pass float(np.float32(p)) for its exact value."""
+    ctx = ctx or default_context()
+    if isinstance(pwms, tuple):
+        packed, lens = pwms
+    else:
+        packed, lens = pack_pwms(pwms)
+    packed = np.ascontiguousarray(packed, np.float32)
+    lens = np.ascontiguousarray(lens, np.int32)
+    K, maxlen = packed.shape[0], packed.shape[2]
+    out = np.empty(K, np.float32)
+    check(load().cdlm_pwm_pvalue_thresh(ctx.h, packed.ctypes.data, lens.ctypes.data, K, maxlen, float(pval),
+                                        float(eps), out.ctypes.data), ctx.h)
+    return out
+
+
+def pvalue_last_ms(ctx: Optional[Context] = None) -> float:
+    """Device time of the last get_thresh call's kernels."""
+    ctx = ctx or default_context()
+    ms = C.c_double()
+    check(load().cdlm_pwm_pvalue_last_ms(ctx.h, C.byref(ms)), ctx.h)
+    return ms.value
+
+
 def overlapping_scan(bases, packed, lens, thresh, ctx: Optional[Context] = None):
     """overlapping_scan! best hits: (positions (K,N) 1-based or 0, use_comp (K,N) bool,
     scores (K,N) float32)."""

@@ -17,6 +17,7 @@ struct cdlm_ctx {
     int num_cu = 0;
     double last_scan_ms = 0.0;   // device time of the last PWM hit-scan kernel
     double last_topology_ms = 0.0;   // device time of the last activation-topology kernel
+    double last_pvalue_ms = 0.0;     // device time of the last p-value threshold kernels
 };
 
 // Records an error message on the context and returns the status.

@@ -340,6 +340,18 @@ cdlm_status cdlm_pwm_best(cdlm_ctx *ctx, const uint8_t *bases, int32_t L, int64_
                           int8_t *out_strand, float *out_score);
 /* Device time (HIP events on the context stream) of the last hit-list scan kernel. */
 cdlm_status cdlm_pwm_last_ms(cdlm_ctx *ctx, double *ms);
+/* get_thresh(pwms, pval, Float32) (src/SEARCH/helpers.jl:99): per PWM the Float32 of
+ * pvalue2score(pwm, pval) (src/SEARCH/PWM_Touzet.jl:199-217; granularity eps, 0.1 in the
+ * reference, TYPE_setup.jl:52; uniform background): the PWM rounded down to multiples of eps,
+ * its score distribution over all 4^m sequences, and the lowest attainable score whose upper
+ * tail probability is below pval (the highest attainable score when there is none).
+ * Computed with exact integer counts, which equals the reference's Double64 arithmetic for
+ * lens <= 53.  PWMs packed as for the scans; pval in [0, 1] (the reference passes a Float32,
+ * pass its exact value); entries finite with |x| <= 1e5; eps >= 1e-3. */
+cdlm_status cdlm_pwm_pvalue_thresh(cdlm_ctx *ctx, const float *pwms, const int32_t *lens, int32_t K, int32_t maxlen,
+                                   double pval, double eps, float *thresh);
+/* Device time (HIP events) of the last cdlm_pwm_pvalue_thresh kernels. */
+cdlm_status cdlm_pwm_pvalue_last_ms(cdlm_ctx *ctx, double *ms);
 /* One-hot (4L, N) FP32 columns to base codes (N, L); a column with no 1 in a 4-group
  * gets code 4 (never matches a PWM row).  Replaces the device side of data_2_dummy. */
 cdlm_status cdlm_onehot_to_bases(cdlm_ctx *ctx, const float *onehot, int32_t L4, int64_t N, uint8_t *bases);
