@@ -68,7 +68,7 @@ struct TimedLaunch {
 // Launch geometry of a set of problems that run with one launch per kernel.
 struct LaunchSet {
     int np = 0;
-    int taps = 0, nsl = 4, B = 2, LgMax = 0;
+    int taps = 0, nsl = 4, B = 2, LgMax = 0, waves = 4;
     int blk_seq = 0, blk_filt = 0, blk_cs = 0, blk_red = 0;
     void *mem = nullptr;  // descriptors followed by the prefix arrays
     CdlDev *tab = nullptr;
@@ -86,8 +86,19 @@ struct LaunchSet {
 constexpr size_t kLdsLimit = 160 * 1024;
 constexpr int kTraceCap = 4096;
 
-size_t lds_bytes_csc(int Lg, int taps) {
-    return sizeof(float) * (2 * (size_t)Lg + (size_t)(Lg + taps + 4) + kWaves * (size_t)(Lg + taps) + kWaves * (size_t)Lg);
+size_t lds_bytes_csc(int Lg, int taps, int waves = kWaves) {
+    return sizeof(float) * (2 * (size_t)Lg + (size_t)(Lg + taps + 4) + waves * (size_t)(Lg + taps) + waves * (size_t)Lg +
+                            4 * (size_t)Lg);
+}
+// CSC waves per work-group: 16 (one sequence per CU, no register prefetch) where the slots
+// fit 4 per lane and the LDS fits, else 4; a tuning of 4 or 16 overrides when it fits.
+int pick_csc_waves(int Lg, int taps, int nsl, int tuned) {
+    const bool ok8 = nsl == 4 && lds_bytes_csc(Lg, taps, 8) <= kLdsLimit;
+    const bool ok16 = nsl == 4 && lds_bytes_csc(Lg, taps, 16) <= kLdsLimit;
+    if (tuned == 8) return ok8 ? 8 : 4;
+    if (tuned == 16) return ok16 ? 16 : 4;
+    if (tuned == 4) return 4;
+    return ok16 ? 16 : 4;
 }
 size_t lds_bytes_init(int Lg, int taps) { return sizeof(float) * (kWaves * (size_t)(Lg + taps) + kWaves * (size_t)Lg); }
 size_t lds_bytes_best(int Lg, int taps) { return sizeof(float) * (size_t)(Lg + taps + 4); }
@@ -145,6 +156,7 @@ struct Tuning {
     int order4 = kFftSmallFirst;  // ... of the length-Lg/4 transforms (dictionary step)
     int B = 0;                    // dictionary columns per pass (0: largest-occupancy choice)
     int nsl = 0;                  // CSC float4 slots per lane (0: the fewest that cover Lg)
+    int waves = 0;                // CSC waves per work-group (4 or 16; 0: the default for the shape)
 };
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int, int>, Tuning> g_tune;
@@ -280,7 +292,8 @@ cdlm_status set_build(cdlm_ctx *ctx, cdlm_problem *const *ps, int np, LaunchSet 
     const int tuned_B = ps[lead]->tune.B;
     if ((tuned_B == 2 || tuned_B == 4) && lds_bytes_dict(Lg, taps, tuned_B) <= kLdsLimit - kThreads * sizeof(double)) T.B = tuned_B;
     if (T.B == 0) return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "padded length %d: dictionary step exceeds LDS", Lg);
-    T.lds_csc = lds_bytes_csc(Lg, taps);
+    T.waves = pick_csc_waves(Lg, taps, T.nsl, ps[lead]->tune.waves);
+    T.lds_csc = lds_bytes_csc(Lg, taps, T.waves);
     T.lds_init = lds_bytes_init(Lg, taps);
     T.lds_best = lds_bytes_best(Lg, taps);
     T.lds_dict = lds_bytes_dict(Lg, taps, T.B);
@@ -295,8 +308,10 @@ cdlm_status set_build(cdlm_ctx *ctx, cdlm_problem *const *ps, int np, LaunchSet 
 #define SETA(TT)                                                                                   \
     do {                                                                                           \
         if ((st = ensure_lds(ctx, (const void *)k_init<TT>, T.lds_init)) != CDLM_OK) return st;    \
-        if ((st = ensure_lds(ctx, (const void *)k_csc<TT, 4>, T.lds_csc)) != CDLM_OK) return st;   \
-        if ((st = ensure_lds(ctx, (const void *)k_csc<TT, 8>, T.lds_csc)) != CDLM_OK) return st;   \
+        if ((st = ensure_lds(ctx, (const void *)k_csc<TT, 4, 4>, T.lds_csc)) != CDLM_OK) return st;   \
+        if ((st = ensure_lds(ctx, (const void *)k_csc<TT, 8, 4>, T.lds_csc)) != CDLM_OK) return st;   \
+        if ((st = ensure_lds(ctx, (const void *)k_csc<TT, 4, 16>, T.lds_csc)) != CDLM_OK) return st;  \
+        if ((st = ensure_lds(ctx, (const void *)k_csc<TT, 4, 8>, T.lds_csc)) != CDLM_OK) return st;   \
         if ((st = ensure_lds(ctx, (const void *)k_best<TT>, T.lds_best)) != CDLM_OK) return st;    \
         if ((st = ensure_lds(ctx, (const void *)k_dict<TT>, T.lds_dict)) != CDLM_OK) return st;    \
     } while (0)
@@ -388,15 +403,25 @@ cdlm_status run_best(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm, int force, in
     return lcheck(ctx, "k_best/k_decide");
 }
 
-cdlm_status run_csc(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm) {
+// iters CSC iterations in one launch (the timer counts iterations).
+cdlm_status run_csc(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm, int iters) {
+    if (iters <= 0) return CDLM_OK;
     hipStream_t st = set_stream(ctx, S);
     {
-        TimedLaunch tl(tm, st, 0);
+        TimedLaunch tl(tm, st, 0, iters);
 #define L_CSC(T)                                                                                                     \
-    if (S.nsl == 4)                                                                                                  \
-        hipLaunchKernelGGL((k_csc<T, 4>), dim3(S.blk_seq), dim3(kThreads), S.lds_csc, st, S.tab, S.st_seq, S.np);    \
+    if (S.waves == 16)                                                                                               \
+        hipLaunchKernelGGL((k_csc<T, 4, 16>), dim3(S.blk_seq), dim3(64 * 16), S.lds_csc, st, S.tab, S.st_seq, S.np,  \
+                           iters);                                                                                   \
+    else if (S.waves == 8)                                                                                           \
+        hipLaunchKernelGGL((k_csc<T, 4, 8>), dim3(S.blk_seq), dim3(64 * 8), S.lds_csc, st, S.tab, S.st_seq, S.np,    \
+                           iters);                                                                                   \
+    else if (S.nsl == 4)                                                                                             \
+        hipLaunchKernelGGL((k_csc<T, 4, 4>), dim3(S.blk_seq), dim3(kThreads), S.lds_csc, st, S.tab, S.st_seq, S.np,  \
+                           iters);                                                                                   \
     else                                                                                                             \
-        hipLaunchKernelGGL((k_csc<T, 8>), dim3(S.blk_seq), dim3(kThreads), S.lds_csc, st, S.tab, S.st_seq, S.np)
+        hipLaunchKernelGGL((k_csc<T, 8, 4>), dim3(S.blk_seq), dim3(kThreads), S.lds_csc, st, S.tab, S.st_seq, S.np,  \
+                           iters)
         CDLM_TAPS_SWITCH(S.taps, L_CSC);
 #undef L_CSC
     }
@@ -441,8 +466,7 @@ cdlm_status run_reset(cdlm_ctx *ctx, const LaunchSet &S) {
 cdlm_status run_schedule(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm, int iter_num, int csc_iter, int cdl_iter) {
     cdlm_status st;
     for (int it = 0; it < iter_num; ++it) {
-        for (int i = 0; i < csc_iter; ++i)
-            if ((st = run_csc(ctx, S, tm)) != CDLM_OK) return st;
+        if ((st = run_csc(ctx, S, tm, csc_iter)) != CDLM_OK) return st;
         if ((st = run_best(ctx, S, tm, 0, 1, it, nullptr)) != CDLM_OK) return st;
         for (int i = 0; i < cdl_iter; ++i)
             if ((st = run_dict(ctx, S, tm)) != CDLM_OK) return st;
@@ -675,10 +699,8 @@ extern "C" size_t cdlm_problem_bytes(const cdlm_problem *p) { return p ? p->tota
 extern "C" cdlm_status cdlm_code_update(cdlm_problem *p, int32_t num_iter) {
     if (!p) return CDLM_ERR_ARG;
     if (num_iter < 0) return cdlm_fail(p->ctx, CDLM_ERR_ARG, "num_iter < 0");
-    for (int i = 0; i < num_iter; ++i) {
-        cdlm_status st = run_csc(p->ctx, p->set, &p->timer);
-        if (st != CDLM_OK) return st;
-    }
+    cdlm_status st = run_csc(p->ctx, p->set, &p->timer, num_iter);
+    if (st != CDLM_OK) return st;
     return sync_ctx(p->ctx);
 }
 
@@ -858,9 +880,10 @@ namespace {
 // Shape class of a problem run alone: members of one class share a launch set without
 // lowering each other's occupancy.
 struct ShapeClass {
-    int nsl, B, taps, occ_csc, occ_dict;
+    int nsl, B, taps, waves, occ_csc, occ_dict;
     bool operator<(const ShapeClass &o) const {
-        return std::tie(nsl, B, taps, occ_csc, occ_dict) < std::tie(o.nsl, o.B, o.taps, o.occ_csc, o.occ_dict);
+        return std::tie(nsl, B, taps, waves, occ_csc, occ_dict) <
+               std::tie(o.nsl, o.B, o.taps, o.waves, o.occ_csc, o.occ_dict);
     }
 };
 
@@ -871,7 +894,10 @@ ShapeClass shape_class(const cdlm_problem *p) {
     if ((p->tune.B == 2 || p->tune.B == 4) && lds_bytes_dict(p->Lg, p->gamma, p->tune.B) <= kLdsLimit - kThreads * sizeof(double))
         c.B = p->tune.B;
     c.taps = p->gamma;
-    c.occ_csc = std::min<int>(c.nsl == 4 ? 3 : 2, (int)(kLdsLimit / lds_bytes_csc(p->Lg, p->gamma)));
+    c.waves = pick_csc_waves(p->Lg, p->gamma, c.nsl, p->tune.waves);
+    c.occ_csc = c.waves >= 8 ? 1
+                              : std::min<int>(c.nsl == 4 && c.taps <= 20 ? 3 : 2,
+                                              (int)(kLdsLimit / lds_bytes_csc(p->Lg, p->gamma, c.waves)));
     c.occ_dict = std::min<int>(8, (int)(kLdsLimit / lds_bytes_dict(p->Lg, p->gamma, c.B)));
     return c;
 }
@@ -1161,13 +1187,15 @@ extern "C" cdlm_status cdlm_tuning_set(cdlm_ctx *ctx, int32_t L4, int32_t M, int
     }
     if (t->fft_order < 0 || t->fft_order >= kFftOrders || t->fft4_order < 0 || t->fft4_order >= kFftOrders ||
         (t->dict_batch != 0 && t->dict_batch != 2 && t->dict_batch != 4) ||
-        (t->csc_slots != 0 && t->csc_slots != 4 && t->csc_slots != 8))
+        (t->csc_slots != 0 && t->csc_slots != 4 && t->csc_slots != 8) ||
+        (t->csc_waves != 0 && t->csc_waves != 4 && t->csc_waves != 8 && t->csc_waves != 16))
         return cdlm_fail(ctx, CDLM_ERR_ARG, "bad tuning values");
     Tuning tu;
     tu.order = t->fft_order;
     tu.order4 = t->fft4_order;
     tu.B = t->dict_batch;
     tu.nsl = t->csc_slots;
+    tu.waves = t->csc_waves;
     g_tune[key] = tu;
     return CDLM_OK;
 }
@@ -1181,6 +1209,7 @@ extern "C" cdlm_status cdlm_tuning_get(cdlm_ctx *ctx, int32_t L4, int32_t M, int
     out->fft4_order = tu.order4;
     out->dict_batch = tu.B;
     out->csc_slots = tu.nsl;
+    out->csc_waves = tu.waves;
     return CDLM_OK;
 }
 
@@ -1214,10 +1243,13 @@ extern "C" cdlm_status cdlm_autotune(cdlm_ctx *ctx, int32_t L4, int32_t M, int32
         return cdlm_fail(ctx, CDLM_ERR_HIP, "event creation failed");
     }
     auto timed = [&](bool dict, int reps, float *ms) -> cdlm_status {
-        cdlm_status r = dict ? run_dict(ctx, p->set, nullptr) : run_csc(ctx, p->set, nullptr);   // warm-up
+        cdlm_status r = dict ? run_dict(ctx, p->set, nullptr) : run_csc(ctx, p->set, nullptr, 1);   // warm-up
         if (r != CDLM_OK) return r;
         CDLM_HIP(ctx, hipEventRecord(a, ctx->stream));
-        for (int i = 0; i < reps && r == CDLM_OK; ++i) r = dict ? run_dict(ctx, p->set, nullptr) : run_csc(ctx, p->set, nullptr);
+        if (dict)
+            for (int i = 0; i < reps && r == CDLM_OK; ++i) r = run_dict(ctx, p->set, nullptr);
+        else
+            r = run_csc(ctx, p->set, nullptr, reps);
         if (r != CDLM_OK) return r;
         CDLM_HIP(ctx, hipEventRecord(b, ctx->stream));
         CDLM_HIP(ctx, hipEventSynchronize(b));
@@ -1234,18 +1266,21 @@ extern "C" cdlm_status cdlm_autotune(cdlm_ctx *ctx, int32_t L4, int32_t M, int32
         return set_build(ctx, &p, 1, &p->set);
     };
     const int need = (Lg / 4 <= 256) ? 4 : 8;
-    float best_csc[kFftOrders][2], best_dict[kFftOrders][kFftOrders][2];
+    // CSC candidates: (slots per lane, waves per work-group)
+    const int kCsc = 4;
+    const int cand_nsl[kCsc] = {4, 4, 4, 8}, cand_waves[kCsc] = {4, 8, 16, 4};
+    float best_csc[kFftOrders][kCsc], best_dict[kFftOrders][kFftOrders][2];
     Tuning best;
     float best_t = 0.f;
     bool have = false;
     for (int o = 0; o < kFftOrders && st == CDLM_OK; ++o) {
-        for (int si = 0; si < 2 && st == CDLM_OK; ++si) {
-            const int nsl = si ? 8 : 4;
-            best_csc[o][si] = -1.f;
-            if (nsl < need) continue;
+        for (int ci = 0; ci < kCsc && st == CDLM_OK; ++ci) {
+            const int nsl = cand_nsl[ci];
+            best_csc[o][ci] = -1.f;
+            if (nsl < need || pick_csc_waves(Lg, gamma, nsl, cand_waves[ci]) != cand_waves[ci]) continue;
             Tuning tu;
-            tu.order = o; tu.nsl = nsl;
-            if ((st = apply(tu)) == CDLM_OK) st = timed(false, 3, &best_csc[o][si]);
+            tu.order = o; tu.nsl = nsl; tu.waves = cand_waves[ci];
+            if ((st = apply(tu)) == CDLM_OK) st = timed(false, 5, &best_csc[o][ci]);
         }
         for (int o4 = 0; o4 < kFftOrders && st == CDLM_OK; ++o4)
             for (int bi = 0; bi < 2 && st == CDLM_OK; ++bi) {
@@ -1258,16 +1293,17 @@ extern "C" cdlm_status cdlm_autotune(cdlm_ctx *ctx, int32_t L4, int32_t M, int32
             }
     }
     for (int o = 0; o < kFftOrders && st == CDLM_OK; ++o)
-        for (int si = 0; si < 2; ++si) {
-            if (best_csc[o][si] < 0) continue;
+        for (int ci = 0; ci < kCsc; ++ci) {
+            if (best_csc[o][ci] < 0) continue;
             for (int o4 = 0; o4 < kFftOrders; ++o4)
                 for (int bi = 0; bi < 2; ++bi) {
                     if (best_dict[o][o4][bi] < 0) continue;
-                    const float t = 15.f * best_csc[o][si] + best_dict[o][o4][bi];
+                    const float t = 15.f * best_csc[o][ci] + best_dict[o][o4][bi];
                     if (!have || t < best_t) {
                         have = true;
                         best_t = t;
-                        best.order = o; best.order4 = o4; best.nsl = si ? 8 : 4; best.B = bi ? 4 : 2;
+                        best.order = o; best.order4 = o4; best.nsl = cand_nsl[ci]; best.waves = cand_waves[ci];
+                        best.B = bi ? 4 : 2;
                     }
                 }
         }
@@ -1285,6 +1321,7 @@ extern "C" cdlm_status cdlm_autotune(cdlm_ctx *ctx, int32_t L4, int32_t M, int32
         out->fft4_order = best.order4;
         out->dict_batch = best.B;
         out->csc_slots = best.nsl;
+        out->csc_waves = best.waves;
     }
     return CDLM_OK;
 }

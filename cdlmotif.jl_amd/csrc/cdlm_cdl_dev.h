@@ -104,7 +104,6 @@ __device__ __forceinline__ int find_problem(const int *__restrict__ starts, int 
 #endif
 template <typename T>
 __device__ __forceinline__ CDLM_GLOBAL T *gp(T *p) { return (CDLM_GLOBAL T *)p; }
-
 __device__ __forceinline__ const float *g1cur_of(const CdlDev &P) { return P.G1b[P.sc->g1cur & 1]; }
 
 __device__ __forceinline__ float4 ld4(const float *p, int s) { return ((const CDLM_GLOBAL float4 *)p)[s]; }
@@ -312,89 +311,133 @@ __global__ __launch_bounds__(kThreads) void k_init(const CdlDev *__restrict__ ta
 }
 
 // ---------------------------------------------------------------------------- CSC
-// One iteration of code_update! (train.jl:5-43) for one sequence.  Each wave owns the
-// columns m = w, w+4, ...; a lane owns the float4 slots s = lane + 64 i (i < NSL) of a
-// column.  Column m+4's loads are issued before column m is processed (two register sets).
-// LDS (float4 units): twiddles (Lg/2) | h with wrapped tail | 4 x v column with wrapped
-// head | 4 x z partial; the FFT ping-pong buffers reuse the v columns after the loop.
-template <int TAPS, int NSL>
-__global__ __launch_bounds__(kThreads, (NSL <= 4 ? 3 : 2)) void k_csc(const CdlDev *__restrict__ tab,
-                                                                      const int *__restrict__ starts, int np) {
+// `iters` iterations of code_update! (train.jl:5-43) for one sequence.  The iterations of
+// different sequences are independent (the dictionary spectrum is fixed during code_update!),
+// so a work-group runs all of its sequence's iterations back to back and h stays in LDS
+// between them (written to HBM after the last).  WAVES waves per work-group; wave w owns the
+// columns m = w, w + WAVES, ...; a lane owns the float4 slots s = lane + 64 i (i < NSL) of a
+// column, the same in every iteration.  With WAVES = 4 (three work-groups per CU) column
+// m + WAVES's loads are issued before column m is processed (two register sets), and so
+// with WAVES = 8 (one work-group per CU by registers); with WAVES = 16 (one work-group per
+// CU, at most 128 VGPRs) there is no prefetch and the 16 waves hide the latency instead.
+// With one sequence per CU resident, a sequence's state (Xt, y2, y3: 0.6 MB at Lg = 840,
+// M = 64) is largely re-read from the last-level cache in the next iteration rather than
+// from HBM.
+// The (Lg) vectors y1, Y1, DX and the data window stay in LDS for the whole launch, and the
+// next iteration's first column is loaded before the per-sequence transforms.
+// LDS (float4 units): twiddles (Lg/2) | h with wrapped tail | WAVES x v column with wrapped
+// head | WAVES x z partial | y1, Y1, DX, data window (Lg each); the FFT ping-pong buffers
+// reuse the v columns after the column loop.
+template <int WAVES, int NSL, int TAPS>
+struct CscShape {
+    static constexpr int kThr = 64 * WAVES;
+    static constexpr bool kPrefetch = WAVES <= 8;
+    static constexpr int kWavesPerSimd = WAVES <= 4 ? (NSL <= 4 && TAPS <= 20 ? 3 : 2) : WAVES / 4;
+};
+
+template <int TAPS, int NSL, int WAVES>
+__global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerSimd)) void k_csc(
+    const CdlDev *__restrict__ tab, const int *__restrict__ starts, int np, int iters) {
+    constexpr int kThr = CscShape<WAVES, NSL, TAPS>::kThr;
+    constexpr bool kPf = CscShape<WAVES, NSL, TAPS>::kPrefetch;
+    constexpr int kSg = kPf ? NSL : 2;     // slots loaded together without prefetch (4: no faster)
     extern __shared__ float4 smem4[];
     int n;
     const CdlDev &P = tab[find_problem(starts, np, blockIdx.x, n)];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int Lg = P.Lg, M = P.M, nslot = Lg >> 2;
     const int hb4Len = (Lg + TAPS + 4) / 4, vb4Len = (Lg + TAPS) / 4;
     float2 *tw = reinterpret_cast<float2 *>(smem4);
     float4 *hb4 = smem4 + Lg / 2;
     float4 *vbase4 = hb4 + hb4Len;
-    float4 *zbase4 = vbase4 + kWaves * vb4Len;
+    float4 *zbase4 = vbase4 + WAVES * vb4Len;
     float *hb = reinterpret_cast<float *>(hb4);
     float *zbase = reinterpret_cast<float *>(zbase4);
+    float *y1s = zbase + WAVES * Lg;                 // y1, Y1, DX and the data window, kept in LDS
+    float *Y1s = y1s + Lg;
+    float *DXs = Y1s + Lg;
+    float *bs = DXs + Lg;
     float2 *fa = reinterpret_cast<float2 *>(vbase4);
     float2 *fb = fa + Lg;
     const size_t ro = (size_t)n * Lg;
+    for (int l = tid; l < Lg; l += kThr) {
+        tw[l] = gp(P.tw)[l];
+        y1s[l] = gp(P.y1)[ro + l];
+        Y1s[l] = gp(P.Y1)[ro + l];
+        DXs[l] = gp(P.DX)[ro + l];
+        const bool win = (l >= P.gamma) && (l < P.gamma + P.L4);
+        bs[l] = win ? gp(P.s)[(size_t)n * P.L4 + (l - P.gamma)] : 0.f;
+    }
+    load_h<TAPS>(P, ro, hb, tid, kThr);
+    if (n == 0 && tid == 0) P.sc->xmat = 0;   // this launch leaves h != 0
     const float *G1 = g1cur_of(P);
-
-    for (int l = tid; l < Lg; l += kThreads) tw[l] = gp(P.tw)[l];
-    load_h<TAPS>(P, ro, hb, tid, kThreads);
-    for (int i = tid; i < kWaves * Lg; i += kThreads) zbase[i] = 0.f;
-    if (n == 0 && tid == 0) P.sc->xmat = 0;   // this iteration leaves h != 0
-    __syncthreads();
-
     const float lam = P.lambda;
     const float inv_c0 = 1.0f / P.sh_csc;
     float4 *vb4 = vbase4 + w * vb4Len;
     float4 *zacc4 = zbase4 + w * (Lg / 4);
-    float4 nx[NSL], n2[NSL], n3[NSL];
-    if (w < M) {
-        const size_t col = ((size_t)n * M + w) * Lg, col3 = col3_of(n, w, M, nslot);
+    float4 nx[kPf ? NSL : 1], n2[kPf ? NSL : 1], n3[kPf ? NSL : 1];
+    auto load_col = [&](int m, float4 *a, float4 *b, float4 *c) {
+        const size_t col = ((size_t)n * M + m) * Lg, col3 = col3_of(n, m, M, nslot);
 #pragma unroll
         for (int i = 0; i < NSL; ++i) {
             const int s = lane + 64 * i;
-            if (s < nslot) { nx[i] = ld4(P.Xt + col, s); n2[i] = ld4(P.y2 + col, s); n3[i] = ld3(P.y3 + col3, s); }
+            if (s < nslot) { a[i] = ld4(P.Xt + col, s); b[i] = ld4(P.y2 + col, s); c[i] = ld3(P.y3 + col3, s); }
         }
-    }
-    for (int m = w; m < M; m += kWaves) {
+    };
+    if (kPf && w < M) load_col(w, nx, n2, n3);
+#pragma nounroll
+  for (int it = 0; it < iters; ++it) {
+    for (int i = tid; i < WAVES * Lg; i += kThr) zbase[i] = 0.f;
+    __syncthreads();
+    for (int m = w; m < M; m += WAVES) {
         const int mm = __builtin_amdgcn_readfirstlane(m);
         const size_t col = ((size_t)n * M + mm) * Lg, col3 = col3_of(n, mm, M, nslot);
-        float4 xt[NSL], a2[NSL], a3[NSL];
+        float4 xt[kPf ? NSL : 1], a2[kPf ? NSL : 1], a3[kPf ? NSL : 1];
+        if constexpr (kPf) {
 #pragma unroll
-        for (int i = 0; i < NSL; ++i) { xt[i] = nx[i]; a2[i] = n2[i]; a3[i] = n3[i]; }
-        if (mm + kWaves < M) {
-            const size_t coln = ((size_t)n * M + mm + kWaves) * Lg, coln3 = col3_of(n, mm + kWaves, M, nslot);
-#pragma unroll
-            for (int i = 0; i < NSL; ++i) {
-                const int s = lane + 64 * i;
-                if (s < nslot) { nx[i] = ld4(P.Xt + coln, s); n2[i] = ld4(P.y2 + coln, s); n3[i] = ld3(P.y3 + coln3, s); }
-            }
+            for (int i = 0; i < NSL; ++i) { xt[i] = nx[i]; a2[i] = n2[i]; a3[i] = n3[i]; }
+            if (mm + WAVES < M) load_col(mm + WAVES, nx, n2, n3);
         }
         float g[TAPS];
         load_taps<TAPS>(G1, mm, Lg, g);
 #pragma unroll
-        for (int i = 0; i < NSL; ++i) {
-            const int s = lane + 64 * i;
-            if (s < nslot) {
-                float X[4];
-                recon4<TAPS>(xt[i], hb4, s, g, X);
-                float y2n[4], y3n[4], v[4], xn[4];
+        for (int i0 = 0; i0 < NSL; i0 += kSg) {
+            float4 lx[kSg], l2[kSg], l3[kSg];
+            if constexpr (!kPf) {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const float x = X[q];
-                    const float t = x + f4(a2[i], q);                   // shrink! (csc_kernels.jl:58-59)
-                    const float at = fabsf(t) - lam;
-                    const float Y2 = at > 0.f ? copysignf(at, t) : 0.f;
-                    const float Y3 = (q == 0) ? x : 0.f;                  // projectionX! (:73-77), y3 = 0 there
-                    y2n[q] = f4(a2[i], q) + x - Y2;                       // train.jl:13
-                    y3n[q] = (q == 0) ? 0.f : f4(a3[i], q) + x;           // train.jl:14
-                    v[q] = (Y2 - y2n[q]) + (Y3 - y3n[q]);
-                    xn[q] = v[q] * inv_c0;
+                for (int u = 0; u < kSg; ++u) {
+                    const int s = lane + 64 * (i0 + u);
+                    if (s < nslot) { lx[u] = ld4(P.Xt + col, s); l2[u] = ld4(P.y2 + col, s); l3[u] = ld3(P.y3 + col3, s); }
                 }
-                st4(P.y2 + col, s, mk4(y2n));
-                st3(P.y3 + col3, s, y3n);
-                st4(P.Xt + col, s, mk4(xn));
-                vb4[TAPS / 4 + s] = mk4(v);
+            }
+#pragma unroll
+            for (int u = 0; u < kSg; ++u) {
+                const int i = i0 + u;
+                const int s = lane + 64 * i;
+                if (s < nslot) {
+                    const float4 xv = kPf ? xt[kPf ? i : 0] : lx[u];
+                    const float4 b2 = kPf ? a2[kPf ? i : 0] : l2[u];
+                    const float4 b3 = kPf ? a3[kPf ? i : 0] : l3[u];
+                    float X[4];
+                    recon4<TAPS>(xv, hb4, s, g, X);
+                    float y2n[4], y3n[4], v[4], xn[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float x = X[q];
+                        const float t = x + f4(b2, q);                      // shrink! (csc_kernels.jl:58-59)
+                        const float at = fabsf(t) - lam;
+                        const float Y2 = at > 0.f ? copysignf(at, t) : 0.f;
+                        const float Y3 = (q == 0) ? x : 0.f;                  // projectionX! (:73-77), y3 = 0 there
+                        y2n[q] = f4(b2, q) + x - Y2;                          // train.jl:13
+                        y3n[q] = (q == 0) ? 0.f : f4(b3, q) + x;              // train.jl:14
+                        v[q] = (Y2 - y2n[q]) + (Y3 - y3n[q]);
+                        xn[q] = v[q] * inv_c0;
+                    }
+                    st4(P.y2 + col, s, mk4(y2n));
+                    st3(P.y3 + col3, s, y3n);
+                    st4(P.Xt + col, s, mk4(xn));
+                    vb4[TAPS / 4 + s] = mk4(v);
+                }
             }
         }
         wave_lds_sync();
@@ -407,23 +450,25 @@ __global__ __launch_bounds__(kThreads, (NSL <= 4 ? 3 : 2)) void k_csc(const CdlD
         }
         wave_lds_sync();
     }
+    // the next iteration's first column does not depend on what follows: load it now
+    if (kPf && it + 1 < iters && w < M) load_col(w, nx, n2, n3);
     __syncthreads();
 
-    // z = sum over waves; u = Y1 - y1' with y1' = y1 + DX - Y1 (train.jl:12)
-    for (int l = tid; l < Lg; l += kThreads) {
+    // z = sum over waves in wave order; u = Y1 - y1' with y1' = y1 + DX - Y1 (train.jl:12)
+    for (int l = tid; l < Lg; l += kThr) {
         float z = zbase[l];
 #pragma unroll
-        for (int ww = 1; ww < kWaves; ++ww) z += zbase[ww * Lg + l];
-        const float Y1o = gp(P.Y1)[ro + l];
-        const float y1n = gp(P.y1)[ro + l] + gp(P.DX)[ro + l] - Y1o;
-        zbase[l] = y1n;
+        for (int ww = 1; ww < WAVES; ++ww) z += zbase[ww * Lg + l];
+        const float Y1o = Y1s[l];
+        const float y1n = y1s[l] + DXs[l] - Y1o;
+        y1s[l] = y1n;
         fa[l] = make_float2(z, Y1o - y1n);
     }
     __syncthreads();
-    float2 *C = team_fft<false>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{});
+    float2 *C = team_fft<false>(fa, fb, P.plan, tw, tid, kThr, BlockSync{});
     float2 *O = (C == fa) ? fb : fa;
     const float c0 = P.sh_csc;
-    for (int k = tid; k < Lg; k += kThreads) {
+    for (int k = tid; k < Lg; k += kThr) {
         const float2 Ck = C[k], Cm = C[k == 0 ? 0 : Lg - k];
         const float2 Z = make_float2(0.5f * (Ck.x + Cm.x), 0.5f * (Ck.y - Cm.y));   // F(z)
         const float2 U = make_float2(0.5f * (Ck.y + Cm.y), -0.5f * (Ck.x - Cm.x));  // F(u)
@@ -434,21 +479,31 @@ __global__ __launch_bounds__(kThreads, (NSL <= 4 ? 3 : 2)) void k_csc(const CdlD
         O[k] = make_float2(hh.x - dx.y, hh.y + dx.x);   // h_hat + i DX_hat
     }
     __syncthreads();
-    float2 *R = team_fft<true>(O, C, P.plan, tw, tid, kThreads, BlockSync{});
+    float2 *R = team_fft<true>(O, C, P.plan, tw, tid, kThr, BlockSync{});
     const float scale = 1.0f / (float)Lg;
     const float rho = P.rho;
-    for (int l = tid; l < Lg; l += kThreads) {
+    const bool last = it + 1 == iters;
+    for (int l = tid; l < Lg; l += kThr) {
         const float hn = R[l].x * scale;
         const float DXn = R[l].y * scale;
         const bool win = (l >= P.gamma) && (l < P.gamma + P.L4);
-        const float b = win ? gp(P.s)[(size_t)n * P.L4 + (l - P.gamma)] : 0.f;
         const float wl = win ? 1.f : 0.f;
-        const float y1n = zbase[l];
-        gp(P.h)[ro + l] = hn;
-        gp(P.DX)[ro + l] = DXn;
-        gp(P.y1)[ro + l] = y1n;
-        gp(P.Y1)[ro + l] = (b + rho * (DXn + y1n)) / (wl + rho);   // update_Y1! (csc_kernels.jl:91)
+        const float y1n = y1s[l];
+        const float Y1n = (bs[l] + rho * (DXn + y1n)) / (wl + rho);   // update_Y1! (csc_kernels.jl:91)
+        hb[l] = hn;
+        DXs[l] = DXn;
+        Y1s[l] = Y1n;
+        if (last) {
+            gp(P.h)[ro + l] = hn;
+            gp(P.DX)[ro + l] = DXn;
+            gp(P.y1)[ro + l] = y1n;
+            gp(P.Y1)[ro + l] = Y1n;
+        }
     }
+    __syncthreads();
+    // wrapped tail of the new h for the next iteration's reconstruction
+    for (int t = tid; t < TAPS + 4; t += kThr) hb[Lg + t] = hb[t % Lg];
+  }
 }
 
 // ---------------------------------------------------------------------------- best / objective

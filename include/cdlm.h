@@ -132,9 +132,10 @@ cdlm_status cdlm_get(cdlm_problem *p, int32_t field, void *dst, size_t bytes);
 
 /* Kernel timing with HIP events recorded on the library's stream around each launch (the
  * reference only prints @time wall clocks, src/WRAPPER/wrap.jl:30-137).  enable != 0 starts
- * a fresh accumulation; read returns the summed device time and launch count of one
- * kernel: 0 = fused CSC iteration, 1 = dictionary iteration (per-sequence kernel),
- * 2 = best/objective pass, 3 = dictionary mean reduction. */
+ * a fresh accumulation; read returns the summed device time and count of one kernel:
+ * 0 = CSC iterations (one launch runs all iterations of a code_update!; the count is the
+ * number of iterations), 1 = dictionary iteration (per-sequence kernel), 2 = best/objective
+ * pass, 3 = dictionary mean reduction (counts are launches). */
 cdlm_status cdlm_timing_enable(cdlm_problem *p, int32_t enable);
 cdlm_status cdlm_timing_read(cdlm_problem *p, int32_t kernel, double *total_ms, int64_t *launches);
 /* ---- batches of problems ----------------------------------------------------------------- */
@@ -241,12 +242,17 @@ cdlm_status cdlm_server_stats(cdlm_server *srv, cdlm_server_stats_t *out);
  * fft_order / fft4_order: stage order of the length-Lg and length-Lg/4 transforms
  * (0 = radix-4/2 stages first, 1 = odd primes first; the only choices that change
  * rounding); dict_batch: dictionary columns per pass (2 or 4; 0 = by occupancy);
- * csc_slots: CSC float4 slots per lane (4 or 8; 0 = the fewest that cover Lg). */
+ * csc_slots: CSC float4 slots per lane (4 or 8; 0 = the fewest that cover Lg);
+ * csc_waves: CSC waves per work-group (4: three sequences per CU with register prefetch;
+ * 8 or 16: one sequence per CU, which keeps the resident sequences' state in the last-level
+ * cache between the CSC iterations of one launch; 0 = 16 where it fits, else 4; it changes
+ * the order of the sum over waves, hence rounding). */
 typedef struct {
     int32_t fft_order;
     int32_t fft4_order;
     int32_t dict_batch;
     int32_t csc_slots;
+    int32_t csc_waves;
 } cdlm_tuning;
 /* Times every candidate on a scratch problem of n_seq sequences (<= 0: three per CU),
  * keeps the fastest for 15 CSC + 1 dictionary iterations, and returns it in *out (may be NULL). */
