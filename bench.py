@@ -380,6 +380,11 @@ def main():
     D0 = synth.rand_probvec_filters(7, M_FILT, Lg, fl)
     print("bench: building problem N=%d L=%d M=%d filter_len=%d" % (N, L_BASES, M_FILT, FLEN), file=sys.stderr,
           flush=True)
+    # launch choices for the shape (FFT stage orders, dictionary batch, CSC work-group shape),
+    # measured once before any timing, as the reference builds its FFT plans up front
+    t0 = time.perf_counter()
+    tuning = cdlm.autotune(L_BASES, M_FILT, FLEN)
+    print("bench: autotuned %s in %.2f s" % (tuning, time.perf_counter() - t0), file=sys.stderr, flush=True)
     c = cdlm.CDL(s, fl, RHO, SIGMA, LAM, D0=D0, X0=None, seed=1234)
 
     def step():
@@ -435,7 +440,8 @@ def main():
         "config": {"workload": "single-dataset CDL: N=%d L=%d M=%d filter_len=%d Lg=%d lambda=%g rho=%g sigma=%g, "
                                "%dx(%d CSC + update_best + %d dict) = %d ADMM iterations per step"
                                % (N, L_BASES, M_FILT, FLEN, Lg, LAM, RHO, SIGMA, ITER_NUM, CSC_ITER, CDL_ITER, iters),
-                   "N": N, "L": L_BASES, "M": M_FILT, "filter_len": FLEN, "iterations": iters},
+                   "N": N, "L": L_BASES, "M": M_FILT, "filter_len": FLEN, "iterations": iters,
+                   "launch_choices": tuning},
         "roofline": {"bound": "hbm", "kernel": "k_csc (%d CSC iterations per launch)" % CSC_ITER,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "algorithmic_bytes_per_launch": csc_bytes(N, Lg, M_FILT, CSC_ITER),

@@ -88,7 +88,7 @@ constexpr int kTraceCap = 4096;
 
 size_t lds_bytes_csc(int Lg, int taps, int waves = kWaves) {
     return sizeof(float) * (2 * (size_t)Lg + (size_t)(Lg + taps + 4) + waves * (size_t)(Lg + taps) + waves * (size_t)Lg +
-                            4 * (size_t)Lg);
+                            6 * (size_t)Lg);
 }
 // CSC waves per work-group: 16 (one sequence per CU, no register prefetch) where the slots
 // fit 4 per lane and the LDS fits, else 4; a tuning of 4 or 16 overrides when it fits.
@@ -115,10 +115,10 @@ size_t lds_bytes_g1(int Lg) { return sizeof(float) * 6 * (size_t)Lg; }
 int pick_dict_batch(int Lg, int taps) {
     int best = 0;
     long bestv = 0;
-    for (int B : {2, 4}) {
+    for (int B : {2, 4, 8}) {
         const size_t b = lds_bytes_dict(Lg, taps, B);
-        if (b > kLdsLimit) continue;
-        const long v = (long)B * (long)(kLdsLimit / b);
+        if (b > kLdsLimit - kThreads * sizeof(double)) continue;
+        const long v = (long)B * (long)(B == 8 ? 1 : kLdsLimit / b);   // B = 8: one 1024-thread work-group per CU
         if (v > bestv) { bestv = v; best = B; }
     }
     return best;
@@ -290,7 +290,8 @@ cdlm_status set_build(cdlm_ctx *ctx, cdlm_problem *const *ps, int np, LaunchSet 
     }
     T.B = pick_dict_batch(Lg, taps);
     const int tuned_B = ps[lead]->tune.B;
-    if ((tuned_B == 2 || tuned_B == 4) && lds_bytes_dict(Lg, taps, tuned_B) <= kLdsLimit - kThreads * sizeof(double)) T.B = tuned_B;
+    if ((tuned_B == 2 || tuned_B == 4 || tuned_B == 8) && lds_bytes_dict(Lg, taps, tuned_B) <= kLdsLimit - kThreads * sizeof(double))
+        T.B = tuned_B;
     if (T.B == 0) return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "padded length %d: dictionary step exceeds LDS", Lg);
     T.waves = pick_csc_waves(Lg, taps, T.nsl, ps[lead]->tune.waves);
     T.lds_csc = lds_bytes_csc(Lg, taps, T.waves);
@@ -313,7 +314,8 @@ cdlm_status set_build(cdlm_ctx *ctx, cdlm_problem *const *ps, int np, LaunchSet 
         if ((st = ensure_lds(ctx, (const void *)k_csc<TT, 4, 16>, T.lds_csc)) != CDLM_OK) return st;  \
         if ((st = ensure_lds(ctx, (const void *)k_csc<TT, 4, 8>, T.lds_csc)) != CDLM_OK) return st;   \
         if ((st = ensure_lds(ctx, (const void *)k_best<TT>, T.lds_best)) != CDLM_OK) return st;    \
-        if ((st = ensure_lds(ctx, (const void *)k_dict<TT>, T.lds_dict)) != CDLM_OK) return st;    \
+        if ((st = ensure_lds(ctx, (const void *)k_dict<TT, kThreads>, T.lds_dict)) != CDLM_OK) return st;   \
+        if ((st = ensure_lds(ctx, (const void *)k_dict<TT, 1024>, T.lds_dict)) != CDLM_OK) return st;    \
     } while (0)
     switch (taps) {
     case 4: SETA(4); break;
@@ -432,7 +434,13 @@ cdlm_status run_dict(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm) {
     hipStream_t st = set_stream(ctx, S);
     {
         TimedLaunch tl(tm, st, 1);
-#define L_DICT(T) hipLaunchKernelGGL(k_dict<T>, dim3(S.blk_seq), dim3(kThreads), S.lds_dict, st, S.tab, S.st_seq, S.np, S.B, S.LgMax)
+#define L_DICT(T)                                                                                                    \
+    if (S.B == 8)                                                                                                    \
+        hipLaunchKernelGGL((k_dict<T, 1024>), dim3(S.blk_seq), dim3(1024), S.lds_dict, st, S.tab, S.st_seq, S.np, S.B, \
+                           S.LgMax);                                                                                 \
+    else                                                                                                             \
+        hipLaunchKernelGGL((k_dict<T, kThreads>), dim3(S.blk_seq), dim3(kThreads), S.lds_dict, st, S.tab, S.st_seq,  \
+                           S.np, S.B, S.LgMax)
         CDLM_TAPS_SWITCH(S.taps, L_DICT);
 #undef L_DICT
     }
@@ -891,14 +899,15 @@ ShapeClass shape_class(const cdlm_problem *p) {
     ShapeClass c;
     c.nsl = std::max((p->Lg / 4 <= 256) ? 4 : 8, p->tune.nsl);
     c.B = pick_dict_batch(p->Lg, p->gamma);
-    if ((p->tune.B == 2 || p->tune.B == 4) && lds_bytes_dict(p->Lg, p->gamma, p->tune.B) <= kLdsLimit - kThreads * sizeof(double))
+    if ((p->tune.B == 2 || p->tune.B == 4 || p->tune.B == 8) &&
+        lds_bytes_dict(p->Lg, p->gamma, p->tune.B) <= kLdsLimit - kThreads * sizeof(double))
         c.B = p->tune.B;
     c.taps = p->gamma;
     c.waves = pick_csc_waves(p->Lg, p->gamma, c.nsl, p->tune.waves);
     c.occ_csc = c.waves >= 8 ? 1
                               : std::min<int>(c.nsl == 4 && c.taps <= 20 ? 3 : 2,
                                               (int)(kLdsLimit / lds_bytes_csc(p->Lg, p->gamma, c.waves)));
-    c.occ_dict = std::min<int>(8, (int)(kLdsLimit / lds_bytes_dict(p->Lg, p->gamma, c.B)));
+    c.occ_dict = c.B == 8 ? 1 : std::min<int>(8, (int)(kLdsLimit / lds_bytes_dict(p->Lg, p->gamma, c.B)));
     return c;
 }
 
@@ -1186,7 +1195,7 @@ extern "C" cdlm_status cdlm_tuning_set(cdlm_ctx *ctx, int32_t L4, int32_t M, int
         return CDLM_OK;
     }
     if (t->fft_order < 0 || t->fft_order >= kFftOrders || t->fft4_order < 0 || t->fft4_order >= kFftOrders ||
-        (t->dict_batch != 0 && t->dict_batch != 2 && t->dict_batch != 4) ||
+        (t->dict_batch != 0 && t->dict_batch != 2 && t->dict_batch != 4 && t->dict_batch != 8) ||
         (t->csc_slots != 0 && t->csc_slots != 4 && t->csc_slots != 8) ||
         (t->csc_waves != 0 && t->csc_waves != 4 && t->csc_waves != 8 && t->csc_waves != 16))
         return cdlm_fail(ctx, CDLM_ERR_ARG, "bad tuning values");
@@ -1269,7 +1278,9 @@ extern "C" cdlm_status cdlm_autotune(cdlm_ctx *ctx, int32_t L4, int32_t M, int32
     // CSC candidates: (slots per lane, waves per work-group)
     const int kCsc = 4;
     const int cand_nsl[kCsc] = {4, 4, 4, 8}, cand_waves[kCsc] = {4, 8, 16, 4};
-    float best_csc[kFftOrders][kCsc], best_dict[kFftOrders][kFftOrders][2];
+    const int kDict = 3;
+    const int cand_B[kDict] = {2, 4, 8};
+    float best_csc[kFftOrders][kCsc], best_dict[kFftOrders][kFftOrders][kDict];
     Tuning best;
     float best_t = 0.f;
     bool have = false;
@@ -1283,8 +1294,8 @@ extern "C" cdlm_status cdlm_autotune(cdlm_ctx *ctx, int32_t L4, int32_t M, int32
             if ((st = apply(tu)) == CDLM_OK) st = timed(false, 5, &best_csc[o][ci]);
         }
         for (int o4 = 0; o4 < kFftOrders && st == CDLM_OK; ++o4)
-            for (int bi = 0; bi < 2 && st == CDLM_OK; ++bi) {
-                const int B = bi ? 4 : 2;
+            for (int bi = 0; bi < kDict && st == CDLM_OK; ++bi) {
+                const int B = cand_B[bi];
                 best_dict[o][o4][bi] = -1.f;
                 if (lds_bytes_dict(Lg, gamma, B) > kLdsLimit - kThreads * sizeof(double)) continue;
                 Tuning tu;
@@ -1296,14 +1307,14 @@ extern "C" cdlm_status cdlm_autotune(cdlm_ctx *ctx, int32_t L4, int32_t M, int32
         for (int ci = 0; ci < kCsc; ++ci) {
             if (best_csc[o][ci] < 0) continue;
             for (int o4 = 0; o4 < kFftOrders; ++o4)
-                for (int bi = 0; bi < 2; ++bi) {
+                for (int bi = 0; bi < kDict; ++bi) {
                     if (best_dict[o][o4][bi] < 0) continue;
                     const float t = 15.f * best_csc[o][ci] + best_dict[o][o4][bi];
                     if (!have || t < best_t) {
                         have = true;
                         best_t = t;
                         best.order = o; best.order4 = o4; best.nsl = cand_nsl[ci]; best.waves = cand_waves[ci];
-                        best.B = bi ? 4 : 2;
+                        best.B = cand_B[bi];
                     }
                 }
         }

@@ -326,7 +326,7 @@ __global__ __launch_bounds__(kThreads) void k_init(const CdlDev *__restrict__ ta
 // The (Lg) vectors y1, Y1, DX and the data window stay in LDS for the whole launch, and the
 // next iteration's first column is loaded before the per-sequence transforms.
 // LDS (float4 units): twiddles (Lg/2) | h with wrapped tail | WAVES x v column with wrapped
-// head | WAVES x z partial | y1, Y1, DX, data window (Lg each); the FFT ping-pong buffers
+// head | WAVES x z partial | y1, Y1, DX, data window, s_k, c_k (Lg each); the FFT ping-pong buffers
 // reuse the v columns after the column loop.
 template <int WAVES, int NSL, int TAPS>
 struct CscShape {
@@ -357,6 +357,8 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
     float *Y1s = y1s + Lg;
     float *DXs = Y1s + Lg;
     float *bs = DXs + Lg;
+    float *sSs = bs + Lg;                           // the spectral constants s_k, c_k
+    float *cSs = sSs + Lg;
     float2 *fa = reinterpret_cast<float2 *>(vbase4);
     float2 *fb = fa + Lg;
     const size_t ro = (size_t)n * Lg;
@@ -367,6 +369,8 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
         DXs[l] = gp(P.DX)[ro + l];
         const bool win = (l >= P.gamma) && (l < P.gamma + P.L4);
         bs[l] = win ? gp(P.s)[(size_t)n * P.L4 + (l - P.gamma)] : 0.f;
+        sSs[l] = gp(P.sS)[l];
+        cSs[l] = gp(P.cS)[l];
     }
     load_h<TAPS>(P, ro, hb, tid, kThr);
     if (n == 0 && tid == 0) P.sc->xmat = 0;   // this launch leaves h != 0
@@ -375,7 +379,8 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
     const float inv_c0 = 1.0f / P.sh_csc;
     float4 *vb4 = vbase4 + w * vb4Len;
     float4 *zacc4 = zbase4 + w * (Lg / 4);
-    float4 nx[kPf ? NSL : 1], n2[kPf ? NSL : 1], n3[kPf ? NSL : 1];
+    float4 nx[kPf ? NSL : 2], n2[kPf ? NSL : 2], n3[kPf ? NSL : 2];
+    bool have_next = false;       // without prefetch: the first two slots of column w are loaded
     auto load_col = [&](int m, float4 *a, float4 *b, float4 *c) {
         const size_t col = ((size_t)n * M + m) * Lg, col3 = col3_of(n, m, M, nslot);
 #pragma unroll
@@ -404,10 +409,15 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
         for (int i0 = 0; i0 < NSL; i0 += kSg) {
             float4 lx[kSg], l2[kSg], l3[kSg];
             if constexpr (!kPf) {
+                if (i0 == 0 && have_next) {
 #pragma unroll
-                for (int u = 0; u < kSg; ++u) {
-                    const int s = lane + 64 * (i0 + u);
-                    if (s < nslot) { lx[u] = ld4(P.Xt + col, s); l2[u] = ld4(P.y2 + col, s); l3[u] = ld3(P.y3 + col3, s); }
+                    for (int u = 0; u < kSg; ++u) { lx[u] = nx[u]; l2[u] = n2[u]; l3[u] = n3[u]; }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < kSg; ++u) {
+                        const int s = lane + 64 * (i0 + u);
+                        if (s < nslot) { lx[u] = ld4(P.Xt + col, s); l2[u] = ld4(P.y2 + col, s); l3[u] = ld3(P.y3 + col3, s); }
+                    }
                 }
             }
 #pragma unroll
@@ -449,9 +459,23 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
             if (s < nslot) conv_acc4<TAPS>(vb4, s, g, zacc4);
         }
         wave_lds_sync();
+        if constexpr (!kPf) have_next = false;
     }
-    // the next iteration's first column does not depend on what follows: load it now
-    if (kPf && it + 1 < iters && w < M) load_col(w, nx, n2, n3);
+    // the next iteration's first column does not depend on what follows: load it now (all of
+    // it with prefetch, its first two slots without)
+    if (it + 1 < iters && w < M) {
+        if constexpr (kPf) {
+            load_col(w, nx, n2, n3);
+        } else {
+            const size_t col = ((size_t)n * M + w) * Lg, col3 = col3_of(n, w, M, nslot);
+#pragma unroll
+            for (int u = 0; u < kSg; ++u) {
+                const int s = lane + 64 * u;
+                if (s < nslot) { nx[u] = ld4(P.Xt + col, s); n2[u] = ld4(P.y2 + col, s); n3[u] = ld3(P.y3 + col3, s); }
+            }
+            have_next = true;
+        }
+    }
     __syncthreads();
 
     // z = sum over waves in wave order; u = Y1 - y1' with y1' = y1 + DX - Y1 (train.jl:12)
@@ -472,8 +496,8 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
         const float2 Ck = C[k], Cm = C[k == 0 ? 0 : Lg - k];
         const float2 Z = make_float2(0.5f * (Ck.x + Cm.x), 0.5f * (Ck.y - Cm.y));   // F(z)
         const float2 U = make_float2(0.5f * (Ck.y + Cm.y), -0.5f * (Ck.x - Cm.x));  // F(u)
-        const float ss = gp(P.sS)[k];
-        const float den = c0 * gp(P.cS)[k];
+        const float ss = sSs[k];
+        const float den = c0 * cSs[k];
         const float2 hh = make_float2((c0 * U.x - Z.x) / den, (c0 * U.y - Z.y) / den);
         const float2 dx = make_float2(Z.x * inv_c0 + ss * hh.x, Z.y * inv_c0 + ss * hh.y);
         O[k] = make_float2(hh.x - dx.y, hh.y + dx.x);   // h_hat + i DX_hat
@@ -631,7 +655,9 @@ __device__ __forceinline__ float recon1(float xt0, const float *hb, int s, const
 }
 
 // ---------------------------------------------------------------------------- dict
-// One iteration of dict_update! (train.jl:50-72) for one sequence; the means over n
+// One iteration of dict_update! (train.jl:50-72) for one sequence (THR threads: 256 with 2 or
+// 4 columns per pass and several sequences per CU, or 1024 with 8 columns per pass and one
+// sequence per CU, which needs a quarter of the passes and so of the barrier phases); the means over n
 // (:76,:83) follow in k_dict_reduce / k_dict_finalize.  Columns go through the transforms B
 // at a time (B even): pass 1 packs two columns' Y3 into one complex length-Lg/4 FFT, stores
 // each column's half spectrum in y3s and, unless X is held materialised already, writes it
@@ -640,8 +666,8 @@ __device__ __forceinline__ float recon1(float xt0, const float *hb, int s, const
 // stay in ascending m.
 // LDS: twiddles | h with wrapped tail | taps of the batch | fa, fb (B x Lg) | ya, yb
 // (B/2 x P) | ys (B x P) | accS, accT, accP (Lg/2+1 bins), with P = Lg/4.
-template <int TAPS>
-__global__ __launch_bounds__(kThreads) void k_dict(const CdlDev *__restrict__ tab, const int *__restrict__ starts, int np,
+template <int TAPS, int THR>
+__global__ __launch_bounds__(THR) void k_dict(const CdlDev *__restrict__ tab, const int *__restrict__ starts, int np,
                                                    int B, int LgMax) {
     extern __shared__ float4 smem4[];
     int n;
@@ -669,23 +695,23 @@ __global__ __launch_bounds__(kThreads) void k_dict(const CdlDev *__restrict__ ta
     const float *G1 = g1cur_of(P);
     const bool mat = !P.sc->xmat;   // set again only by k_cs after this step
 
-    for (int l = tid; l < Lg; l += kThreads) tw[l] = gp(P.tw)[l];
-    if (mat) load_h<TAPS>(P, ro, hb, tid, kThreads);
-    for (int k = tid; k < HpX; k += kThreads) { accS[k] = 0.f; accT[k] = make_float2(0.f, 0.f); accP[k] = make_float2(0.f, 0.f); }
+    for (int l = tid; l < Lg; l += THR) tw[l] = gp(P.tw)[l];
+    if (mat) load_h<TAPS>(P, ro, hb, tid, THR);
+    for (int k = tid; k < HpX; k += THR) { accS[k] = 0.f; accT[k] = make_float2(0.f, 0.f); accP[k] = make_float2(0.f, 0.f); }
 
     // ---- pass 1: spectra sums over m; X materialised
     for (int m0 = 0; m0 < M; m0 += B) {
         const int nb = (M - m0) < B ? (M - m0) : B;
         const int npair = (nb + 1) / 2;
         if (mat)
-            for (int i = tid; i < nb * TAPS; i += kThreads) {
+            for (int i = tid; i < nb * TAPS; i += THR) {
                 const int j = i % TAPS;
                 gtap[i] = (j < Lg) ? gp(G1)[(size_t)(m0 + i / TAPS) * Lg + j] : 0.f;
             }
         if (nb & 1)
-            for (int s = tid; s < Pq; s += kThreads) ya[(npair - 1) * Pq + s].y = 0.f;
+            for (int s = tid; s < Pq; s += THR) ya[(npair - 1) * Pq + s].y = 0.f;
         __syncthreads();
-        for (int idx = tid; idx < nb * nslot; idx += kThreads) {
+        for (int idx = tid; idx < nb * nslot; idx += THR) {
             int s;
             const int bb = fdivmod(idx, nslot, rslot, s), m = m0 + bb;
             const size_t col = ((size_t)n * M + m) * Lg;
@@ -711,10 +737,10 @@ __global__ __launch_bounds__(kThreads) void k_dict(const CdlDev *__restrict__ ta
             }
         }
         __syncthreads();
-        float2 *Y = team_fft<false>(ya, yb, P.plan4, tw, tid, kThreads, BlockSync{}, npair);
-        float2 *C = team_fft<false>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{}, nb);
+        float2 *Y = team_fft<false>(ya, yb, P.plan4, tw, tid, THR, BlockSync{}, npair);
+        float2 *C = team_fft<false>(fa, fb, P.plan, tw, tid, THR, BlockSync{}, nb);
         // unpack the column pairs: Y3h of column 2p is the Hermitian part, of 2p+1 the other
-        for (int idx = tid; idx < nb * Pq; idx += kThreads) {
+        for (int idx = tid; idx < nb * Pq; idx += THR) {
             int k;
             const int bb = fdivmod(idx, Pq, rPq, k);
             const float2 Yk = Y[(bb >> 1) * Pq + k], Ym = Y[(bb >> 1) * Pq + (k == 0 ? 0 : Pq - k)];
@@ -724,7 +750,7 @@ __global__ __launch_bounds__(kThreads) void k_dict(const CdlDev *__restrict__ ta
             if (k < Ph) gp(y3s)[(size_t)(m0 + bb) * Ph + k] = v;
         }
         __syncthreads();
-        for (int k = tid; k < H; k += kThreads) {
+        for (int k = tid; k < H; k += THR) {
             int kq;
             (void)fdivmod(k, Pq, rPq, kq);
             float aS = accS[k];
@@ -743,31 +769,31 @@ __global__ __launch_bounds__(kThreads) void k_dict(const CdlDev *__restrict__ ta
         __syncthreads();
     }
     if (mat)
-        for (int l = tid; l < Lg; l += kThreads) gp(P.h)[ro + l] = 0.f;
+        for (int l = tid; l < Lg; l += THR) gp(P.h)[ro + l] = 0.f;
 
     // ---- per-sequence spectral step: Td, G2/g2 (update_g2!, train.jl:68-71), kappa
-    for (int k = tid; k < Lg; k += kThreads) fa[k] = (k < H) ? accT[k] : cconj(accT[Lg - k]);
+    for (int k = tid; k < Lg; k += THR) fa[k] = (k < H) ? accT[k] : cconj(accT[Lg - k]);
     __syncthreads();
-    float2 *R2 = team_fft<true>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{}, 1);
+    float2 *R2 = team_fft<true>(fa, fb, P.plan, tw, tid, THR, BlockSync{}, 1);
     float *tdb = reinterpret_cast<float *>(ys);      // Lg floats fit in ys (B * Lg/4 complex, B >= 2)
     {
         const float scale = 1.0f / (float)Lg;
-        for (int l = tid; l < Lg; l += kThreads) tdb[l] = R2[l].x * scale;
+        for (int l = tid; l < Lg; l += THR) tdb[l] = R2[l].x * scale;
     }
     __syncthreads();
     {
         // F(G2 - g2) of the previous dictionary iteration (sherman_CDL_RHS!, cdl_kernels.jl:8)
-        for (int l = tid; l < Lg; l += kThreads) fa[l] = make_float2(gp(P.G2)[ro + l] - gp(P.g2)[ro + l], 0.f);
+        for (int l = tid; l < Lg; l += THR) fa[l] = make_float2(gp(P.G2)[ro + l] - gp(P.g2)[ro + l], 0.f);
         __syncthreads();
-        float2 *R = team_fft<false>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{}, 1);
+        float2 *R = team_fft<false>(fa, fb, P.plan, tw, tid, THR, BlockSync{}, 1);
         const float c1 = P.sh_cdl;
-        for (int k = tid; k < H; k += kThreads) {
+        for (int k = tid; k < H; k += THR) {
             const float2 Gk = R[k];
             const float den = accS[k] + c1;
             accP[k] = make_float2((c1 * Gk.x - accP[k].x) / den, (c1 * Gk.y - accP[k].y) / den);
         }
     }
-    for (int l = tid; l < Lg; l += kThreads) {
+    for (int l = tid; l < Lg; l += THR) {
         const float Td = tdb[l];
         const bool win = (l >= P.gamma) && (l < P.gamma + P.L4);
         const float b = win ? gp(P.s)[(size_t)n * P.L4 + (l - P.gamma)] : 0.f;
@@ -785,7 +811,7 @@ __global__ __launch_bounds__(kThreads) void k_dict(const CdlDev *__restrict__ ta
     for (int m0 = 0; m0 < M; m0 += B) {
         const int nb = (M - m0) < B ? (M - m0) : B;
         const int npair = (nb + 1) / 2;
-        for (int idx = tid; idx < nb * Ph; idx += kThreads) {
+        for (int idx = tid; idx < nb * Ph; idx += THR) {
             const int bb = idx / Ph;
             const int k = idx - bb * Ph;
             const float2 v = gp(y3s)[(size_t)(m0 + bb) * Ph + k];
@@ -793,7 +819,7 @@ __global__ __launch_bounds__(kThreads) void k_dict(const CdlDev *__restrict__ ta
             if (k > 0 && Pq - k >= Ph) ys[bb * Pq + Pq - k] = cconj(v);
         }
         __syncthreads();
-        for (int idx = tid; idx < npair * Lg; idx += kThreads) {
+        for (int idx = tid; idx < npair * Lg; idx += THR) {
             int k, kq;
             const int pp = fdivmod(idx, Lg, rLg, k);
             (void)fdivmod(k, Pq, rPq, kq);
@@ -803,8 +829,8 @@ __global__ __launch_bounds__(kThreads) void k_dict(const CdlDev *__restrict__ ta
             fa[idx] = make_float2(CA.x - CB.y, CA.y + CB.x);
         }
         __syncthreads();
-        float2 *R = team_fft<true>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{}, npair);
-        for (int idx = tid; idx < nb * nslot; idx += kThreads) {
+        float2 *R = team_fft<true>(fa, fb, P.plan, tw, tid, THR, BlockSync{}, npair);
+        for (int idx = tid; idx < nb * nslot; idx += THR) {
             int s;
             const int bb = fdivmod(idx, nslot, rslot, s), m = m0 + bb;
             const size_t col = ((size_t)n * M + m) * Lg;

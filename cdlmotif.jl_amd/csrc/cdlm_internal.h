@@ -49,7 +49,7 @@ struct FftPlan {
     int radix[kMaxStages] = {0};
 };
 
-// Stage orders: radix 4 and 2 stages first, then odd primes ascending; or odd primes
+// Stage orders: radix 8, 4 and 2 stages first, then odd primes ascending; or odd primes
 // first, descending (the largest prime then runs without input twiddles).
 enum FftOrder { kFftSmallFirst = 0, kFftPrimesFirst = 1, kFftOrders = 2 };
 
@@ -59,8 +59,16 @@ inline bool make_fft_plan(int n, int twstride, FftPlan *p, int order = kFftSmall
     p->nst = 0;
     int m = n;
     int pw[kMaxStages], npw = 0, od[kMaxStages], nod = 0;
-    while (m % 4 == 0) { if (npw >= kMaxStages) return false; pw[npw++] = 4; m /= 4; }
-    while (m % 2 == 0) { if (npw >= kMaxStages) return false; pw[npw++] = 2; m /= 2; }
+    // powers of two: radix 8 stages, then a 4 or a 2; 2^(3q+1) uses two 4s instead of 8 * 2
+    int e = 0;
+    while (m % 2 == 0) { ++e; m /= 2; }
+    int q8 = e / 3, n4 = 0, n2 = 0;
+    if (e % 3 == 2) n4 = 1;
+    else if (e % 3 == 1) { if (q8 > 0) { --q8; n4 = 2; } else n2 = 1; }
+    if (q8 + n4 + n2 > kMaxStages) return false;
+    for (int i = 0; i < q8; ++i) pw[npw++] = 8;
+    for (int i = 0; i < n4; ++i) pw[npw++] = 4;
+    for (int i = 0; i < n2; ++i) pw[npw++] = 2;
     for (int q = 3; m > 1; q += 2)
         while (m % q == 0) { if (nod >= kMaxStages) return false; od[nod++] = q; m /= q; }
     if (npw + nod > kMaxStages) return false;
@@ -109,17 +117,19 @@ __device__ __forceinline__ int fdivmod(int x, int d, float rcp, int &r) {
 }
 
 // One radix-R butterfly j (position k = j mod Ns) of a Stockham stage with sub-length Ns:
-// inputs a[j + r T] times W_{Ns R}^{r k} (table step st), outputs b[(j - k) R + k + q Ns].
-// rs is the table step of the radix-R roots W_R.
-template <int R, bool INV>
-__device__ __forceinline__ void fft_bfly(const float2 *__restrict__ a, float2 *__restrict__ b, int j, int k, int T,
-                                         int Ns, int st, int rs, const float2 *__restrict__ tw) {
+// inputs a[j + r T] times W_{Ns R}^{r k} (twiddle table index r * kst, kst = k * table step;
+// TW = false in the first stage, where k = 0), outputs b[(j - k) R + k + q Ns].
+// rs is the table step of the radix-R roots W_R (odd R).
+template <int R, bool INV, bool TW>
+__device__ __forceinline__ void fft_bfly(const float2 *__restrict__ a, float2 *__restrict__ b, int j, int k, int kst, int T,
+                                         int Ns, int rs, const float2 *__restrict__ tw) {
     const int ob = (j - k) * R + k;
     float2 v[R];
+    v[0] = a[j];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        float2 x = a[j + r * T];
-        v[r] = (r == 0) ? x : cmul(x, twid<INV>(tw, r * k * st));
+    for (int r = 1; r < R; ++r) {
+        const float2 x = a[j + r * T];
+        v[r] = TW ? cmul(x, twid<INV>(tw, r * kst)) : x;
     }
     if constexpr (R == 2) {
         b[ob] = cadd(v[0], v[1]);
@@ -133,6 +143,36 @@ __device__ __forceinline__ void fft_bfly(const float2 *__restrict__ a, float2 *_
         b[ob + Ns] = cadd(d02, rot);
         b[ob + 2 * Ns] = csub(s02, s13);
         b[ob + 3 * Ns] = csub(d02, rot);
+    } else if constexpr (R == 8) {
+        // decimation in frequency: y_{2m} = DFT4(v_n + v_{n+4})_m,
+        // y_{2m+1} = DFT4((v_n - v_{n+4}) W_8^n)_m
+        constexpr float c = 0.70710678118654752f;
+        float2 e[4], o[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            e[q] = cadd(v[q], v[q + 4]);
+            o[q] = csub(v[q], v[q + 4]);
+        }
+        if (INV) {
+            o[1] = make_float2(c * (o[1].x - o[1].y), c * (o[1].x + o[1].y));
+            o[2] = make_float2(-o[2].y, o[2].x);
+            o[3] = make_float2(-c * (o[3].x + o[3].y), c * (o[3].x - o[3].y));
+        } else {
+            o[1] = make_float2(c * (o[1].x + o[1].y), c * (o[1].y - o[1].x));
+            o[2] = make_float2(o[2].y, -o[2].x);
+            o[3] = make_float2(c * (o[3].y - o[3].x), -c * (o[3].x + o[3].y));
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const float2 *u = h ? o : e;
+            const float2 s02 = cadd(u[0], u[2]), d02 = csub(u[0], u[2]);
+            const float2 s13 = cadd(u[1], u[3]), d13 = csub(u[1], u[3]);
+            const float2 rot = INV ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);
+            b[ob + h * Ns] = cadd(s02, s13);
+            b[ob + (2 + h) * Ns] = cadd(d02, rot);
+            b[ob + (4 + h) * Ns] = csub(s02, s13);
+            b[ob + (6 + h) * Ns] = csub(d02, rot);
+        }
     } else {
         // odd R: outputs q and R - q from a_r = v_r + v_{R-r} and b_r = v_r - v_{R-r}:
         //   y_q = v_0 + sum_r a_r cos(2 pi r q / R) -/+ i sum_r b_r sin(2 pi r q / R)
@@ -171,6 +211,44 @@ __device__ __forceinline__ void fft_bfly(const float2 *__restrict__ a, float2 *_
             const float2 pi = make_float2(re.x - im.y, re.y + im.x);
             b[ob + q * Ns] = INV ? pi : mi;
             b[ob + (R - q) * Ns] = INV ? mi : pi;
+        }
+    }
+}
+
+// One Stockham stage of radix R over nb transforms of length n (T = n / R butterflies each):
+// thread tid takes butterflies tid, tid + nthr, ...; its (transform, j, k = j mod Ns, k * st)
+// advance by additions (nthr = dB T + dT and T is a multiple of Ns), so the loop has no
+// division or integer multiply.
+template <int R, bool INV>
+__device__ __forceinline__ void fft_stage(const float2 *__restrict__ a, float2 *__restrict__ b, int n, int nb, int T,
+                                          int Ns, int st, int rs, const float2 *__restrict__ tw, int tid, int nthr) {
+    const int tot = nb * T;
+    if (tid >= tot) return;
+    const float rT = 1.0f / (float)T, rNs = 1.0f / (float)Ns;
+    int j, k, dT, dK;
+    const int bb = fdivmod(tid, T, rT, j);
+    (void)fdivmod(j, Ns, rNs, k);
+    const int dB = fdivmod(nthr, T, rT, dT);
+    (void)fdivmod(dT, Ns, rNs, dK);
+    int kst = k * st;
+    const int dKst = dK * st, Nst = Ns * st, dOff = dB * n;
+    int off = bb * n;
+    if (Ns == 1) {
+        for (int jj = tid; jj < tot; jj += nthr) {
+            fft_bfly<R, INV, false>(a + off, b + off, j, 0, 0, T, 1, rs, tw);
+            j += dT;
+            off += dOff;
+            if (j >= T) { j -= T; off += n; }
+        }
+    } else {
+        for (int jj = tid; jj < tot; jj += nthr) {
+            fft_bfly<R, INV, true>(a + off, b + off, j, k, kst, T, Ns, rs, tw);
+            j += dT;
+            off += dOff;
+            k += dK;
+            kst += dKst;
+            if (j >= T) { j -= T; off += n; }
+            if (k >= Ns) { k -= Ns; kst -= Nst; }
         }
     }
 }
@@ -230,23 +308,16 @@ __device__ __forceinline__ float2 *team_fft(float2 *a, float2 *b, const FftPlan 
     for (int st = 0; st < P.nst; ++st) {
         const int R = P.radix[st];
         const int T = n / R;
-        const int tot = nb * T;
         const float rT = 1.0f / (float)T, rNs = 1.0f / (float)Ns;
         const int stw = (n / (Ns * R)) * tws;   // twiddle step of r*k
         const int rs = (n / R) * tws;           // twiddle step of the radix-R roots
-#define CDLM_BFLY_LOOP(RR)                                                                        \
-    for (int jj = tid; jj < tot; jj += nthr) {                                                    \
-        int j, k;                                                                                 \
-        const int bb = fdivmod(jj, T, rT, j);                                                     \
-        (void)fdivmod(j, Ns, rNs, k);                                                             \
-        fft_bfly<RR, INV>(a + bb * n, b + bb * n, j, k, T, Ns, stw, rs, tw);                      \
-    }
         switch (R) {
-        case 2: CDLM_BFLY_LOOP(2) break;
-        case 3: CDLM_BFLY_LOOP(3) break;
-        case 4: CDLM_BFLY_LOOP(4) break;
-        case 5: CDLM_BFLY_LOOP(5) break;
-        case 7: CDLM_BFLY_LOOP(7) break;
+        case 2: fft_stage<2, INV>(a, b, n, nb, T, Ns, stw, rs, tw, tid, nthr); break;
+        case 3: fft_stage<3, INV>(a, b, n, nb, T, Ns, stw, rs, tw, tid, nthr); break;
+        case 4: fft_stage<4, INV>(a, b, n, nb, T, Ns, stw, rs, tw, tid, nthr); break;
+        case 5: fft_stage<5, INV>(a, b, n, nb, T, Ns, stw, rs, tw, tid, nthr); break;
+        case 7: fft_stage<7, INV>(a, b, n, nb, T, Ns, stw, rs, tw, tid, nthr); break;
+        case 8: fft_stage<8, INV>(a, b, n, nb, T, Ns, stw, rs, tw, tid, nthr); break;
         default: {
             // large odd radix: (butterfly, output pair) work items
             const int H = (R + 1) / 2;
@@ -261,7 +332,6 @@ __device__ __forceinline__ float2 *team_fft(float2 *a, float2 *b, const FftPlan 
             }
         } break;
         }
-#undef CDLM_BFLY_LOOP
         sync();
         float2 *t = a; a = b; b = t;
         Ns *= R;

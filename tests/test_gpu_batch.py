@@ -127,7 +127,9 @@ def test_footprint_is_what_a_problem_holds():
 
 
 def test_tuned_launch_choices_keep_parity():
-    """Every stage order and dictionary batch meets the oracle tolerance; the autotuner's
+    """Every stage order, dictionary batch (2 and 4 columns per pass, 8 in 1024-thread
+    work-groups) and CSC shape (slots per lane, waves per work-group) meets the oracle
+    tolerance; the autotuner's
     choice is recorded and used by problems created afterwards; a batch of problems with
     different tunings still equals each problem alone."""
     import oracle as O
@@ -141,22 +143,23 @@ def test_tuned_launch_choices_keep_parity():
     o.update_best()
     o.dict_update(1)
     o.code_update(3)
+    choices = [dict(fft_order=order, fft4_order=order4, dict_batch=B, csc_slots=8, csc_waves=4)
+               for order in (0, 1) for order4 in (0, 1) for B in (2, 4, 8)]
+    choices += [dict(fft_order=0, fft4_order=0, dict_batch=2, csc_slots=4, csc_waves=wv) for wv in (4, 8, 16)]
     try:
-        for order in (0, 1):
-            for order4 in (0, 1):
-                for B in (2, 4):
-                    cdlm.tuning_set(L, M, 20, dict(fft_order=order, fft4_order=order4, dict_batch=B, csc_slots=8))
-                    assert cdlm.tuning_get(L, M, 20)["fft_order"] == order
-                    c = cdlm.CDL(s, fl, RHO, SIGMA, LAM, D0=D0, X0=X0)
-                    c.code_update(6)
-                    c.update_best()
-                    c.dict_update(1)
-                    c.code_update(3)
-                    for f in ("X", "DX", "d", "G1"):
-                        e = rel(c.get(f), o.get(f))
-                        flush("orders %d/%d B=%d %s: %.2e" % (order, order4, B, f, e))
-                        assert e <= 1e-4, (order, order4, B, f, e)
-                    c.close()
+        for ch in choices:
+            cdlm.tuning_set(L, M, 20, ch)
+            assert cdlm.tuning_get(L, M, 20) == ch
+            c = cdlm.CDL(s, fl, RHO, SIGMA, LAM, D0=D0, X0=X0)
+            c.code_update(6)
+            c.update_best()
+            c.dict_update(1)
+            c.code_update(3)
+            for f in ("X", "DX", "d", "G1"):
+                e = rel(c.get(f), o.get(f))
+                flush("%s %s: %.2e" % (ch, f, e))
+                assert e <= 1e-4, (ch, f, e)
+            c.close()
         got = cdlm.autotune(L, M, 20, n_seq=64)
         flush("autotune: %s" % got)
         assert got == cdlm.tuning_get(L, M, 20)
