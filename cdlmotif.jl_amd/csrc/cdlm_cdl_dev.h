@@ -144,14 +144,14 @@ __device__ __forceinline__ void recon4(const float4 xt, const float4 *hb4, int s
 // z[4s..4s+3] += sum_j g[j] * v[(l - j) mod Lg]; vb4 holds v at float4 offset TAPS/4 with
 // the wrapped head of TAPS samples in front.
 template <int TAPS>
-__device__ __forceinline__ void conv_acc4(const float4 *vb4, int s, const float *g, float4 *zacc4) {
+__device__ __forceinline__ void conv_acc4(const float4 *vb4, int s, const float *g, float4 *zacc4, bool first = false) {
     float vw[TAPS + 4];
 #pragma unroll
     for (int u = 0; u <= TAPS / 4; ++u) {
         float4 t = vb4[s + u];
         vw[4 * u] = t.x; vw[4 * u + 1] = t.y; vw[4 * u + 2] = t.z; vw[4 * u + 3] = t.w;
     }
-    float4 z = zacc4[s];
+    float4 z = first ? make_float4(0.f, 0.f, 0.f, 0.f) : zacc4[s];
     float o[4] = {z.x, z.y, z.z, z.w};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -390,9 +390,9 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
         }
     };
     if (kPf && w < M) load_col(w, nx, n2, n3);
+    const int nw = M < WAVES ? M : WAVES;       // waves that own columns (and z partials)
 #pragma nounroll
   for (int it = 0; it < iters; ++it) {
-    for (int i = tid; i < WAVES * Lg; i += kThr) zbase[i] = 0.f;
     __syncthreads();
     for (int m = w; m < M; m += WAVES) {
         const int mm = __builtin_amdgcn_readfirstlane(m);
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
 #pragma unroll
         for (int i = 0; i < NSL; ++i) {
             const int s = lane + 64 * i;
-            if (s < nslot) conv_acc4<TAPS>(vb4, s, g, zacc4);
+            if (s < nslot) conv_acc4<TAPS>(vb4, s, g, zacc4, mm == w);   // the first column starts the partial
         }
         wave_lds_sync();
         if constexpr (!kPf) have_next = false;
@@ -481,8 +481,7 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
     // z = sum over waves in wave order; u = Y1 - y1' with y1' = y1 + DX - Y1 (train.jl:12)
     for (int l = tid; l < Lg; l += kThr) {
         float z = zbase[l];
-#pragma unroll
-        for (int ww = 1; ww < WAVES; ++ww) z += zbase[ww * Lg + l];
+        for (int ww = 1; ww < nw; ++ww) z += zbase[ww * Lg + l];
         const float Y1o = Y1s[l];
         const float y1n = y1s[l] + DXs[l] - Y1o;
         y1s[l] = y1n;
@@ -515,6 +514,7 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
         const float y1n = y1s[l];
         const float Y1n = (bs[l] + rho * (DXn + y1n)) / (wl + rho);   // update_Y1! (csc_kernels.jl:91)
         hb[l] = hn;
+        for (int t = l; t < TAPS + 4; t += Lg) hb[Lg + t] = hn;   // the wrapped tail, h[t mod Lg]
         DXs[l] = DXn;
         Y1s[l] = Y1n;
         if (last) {
@@ -524,9 +524,6 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
             gp(P.Y1)[ro + l] = Y1n;
         }
     }
-    __syncthreads();
-    // wrapped tail of the new h for the next iteration's reconstruction
-    for (int t = tid; t < TAPS + 4; t += kThr) hb[Lg + t] = hb[t % Lg];
   }
 }
 
