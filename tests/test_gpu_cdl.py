@@ -194,3 +194,23 @@ def test_x_materialisation_paths_agree():
         b.dict_update(1)
     for f in ("X", "d", "G1", "G3", "DX", "g1", "g2", "Td"):
         np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
+
+
+@pytest.mark.parametrize("shape", [(1, 2, 1, [4]), (3, 1, 2, [4, 4]), (2, 5, 3, [8, 4, 12]), (5, 3, 17, [8] * 17)])
+def test_tiny_shapes_vs_oracle(shape):
+    """Edge shapes: one sequence, one base, one filter, fewer filters than the CSC kernel has
+    waves (M < 16: waves without columns), an odd filter count across the waves; short
+    schedule, every launch choice at its default."""
+    N, L, M, fl = shape
+    s, D0, X0, _ = _synth(40 + N + M, N, L, M, fl)
+    g, o = _both(s, D0, X0, fl, 1.0, 100.0, 0.05)
+    for _ in range(2):
+        g.code_update(4); o.code_update(4)
+        ob_g, imp_g = g.update_best(); ob_o, imp_o = o.update_best()
+        assert abs(ob_g - ob_o) <= TOL * abs(ob_o) and imp_g == imp_o, (ob_g, ob_o)
+        g.dict_update(1); o.dict_update(1)
+    for f in ("X", "DX", "y1", "y2", "y3", "Y1", "d", "g1", "g3", "G1", "G3"):
+        e = rel(g.get(f), o.get(f))
+        flush("tiny %s %s: %.2e" % (shape, f, e))
+        assert e < TOL, (shape, f, e)
+    g.close()
