@@ -21,6 +21,11 @@
 namespace {
 
 constexpr int kPwmThreads = 256;
+// Hit-list kernel shape: G tiles of 16 strand-PWMs per wave (B operands in registers) and CH
+// chunks of 16 windows per pass sharing them, so a wave runs G x CH independent MFMA chains.
+// Measured at 100000 x 500 bases, 512 PWMs of length 8-24: (G, CH) = (4, 1) 94.2 ms,
+// (2, 1) 89.3, (2, 2) 86.7, (4, 2) 87.5, (2, 4) 92.1, (1, 8) 89.1, (1, 4) 84.6.
+constexpr int kPwmG = 1, kPwmCH = 4;
 constexpr int kMaxPwmLen = 64;
 
 // ---------------------------------------------------------------------------- hit list on MFMA
@@ -46,13 +51,18 @@ struct PwmTiles {
     const float *slot_th;  // threshold
 };
 
-template <int G, int MAXJ>
+// Sequence bytes per wave in LDS: chunks of 16 * CH windows, each reading MAXJ bases on.
+__host__ __device__ constexpr int pwm_lpad(int L, int MAXJ, int CH) {
+    return ((L + 16 * CH - 1) / (16 * CH)) * (16 * CH) + MAXJ + 16;
+}
+
+template <int G, int MAXJ, int CH>
 __global__ __launch_bounds__(256) void k_pwm_hits_mfma(const uint8_t *__restrict__ bases, int L, long N, PwmTiles P,
                                                        int *hk, long long *hn, int *hp, signed char *hs, long long cap,
                                                        unsigned long long *count) {
     extern __shared__ uint8_t seqs_lds[];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int Lpad = ((L + 15) & ~15) + MAXJ + 16;
+    const int Lpad = pwm_lpad(L, MAXJ, CH);
     uint8_t *sq = seqs_lds + w * Lpad;
     const int g = blockIdx.x;
     const int t0 = g * G;
@@ -75,51 +85,60 @@ __global__ __launch_bounds__(256) void k_pwm_hits_mfma(const uint8_t *__restrict
         sth[gi] = ok ? P.slot_th[sl] : 0.f;
     }
     const int arow = lane & 15, abase = lane >> 4;
-    const int jsteps = (gmax + 3) / 4;   // uniform: the tiles of this work-group
+    const int jsteps = gmax;   // uniform: the tiles of this work-group
     const long nstride = (long)gridDim.y * 4;
     for (long n = (long)blockIdx.y * 4 + w; n < N; n += nstride) {
         const uint8_t *src = bases + n * (long)L;
         for (int i = lane; i < Lpad; i += 64) sq[i] = i < L ? src[i] : (uint8_t)4;
         wave_lds_sync();
-        for (int w0 = 0; w0 < L; w0 += 16) {
-            floatx4 acc[G];
+        for (int w0 = 0; w0 < L; w0 += 16 * CH) {
+            // CH chunks of 16 windows share the B operands: G x CH independent accumulators
+            floatx4 acc[CH][G];
 #pragma unroll
-            for (int gi = 0; gi < G; ++gi) acc[gi] = floatx4{0.f, 0.f, 0.f, 0.f};
-            // the group's positions in whole steps of 4 (B is zero past each PWM's length); one
-            // uniform switch per chunk keeps every case branch-free, so the accumulators stay in
-            // place across the MFMA chain
+            for (int c = 0; c < CH; ++c)
+#pragma unroll
+                for (int gi = 0; gi < G; ++gi) acc[c][gi] = floatx4{0.f, 0.f, 0.f, 0.f};
+            // the group's positions (B is zero past each PWM's length); one uniform switch per
+            // chunk keeps every case branch-free, so the accumulators stay in place across the
+            // MFMA chain
 #define CDLM_MFMA_STEPS(JS)                                                                          \
-    case JS / 4:                                                                                     \
+    case JS:                                                                                         \
         _Pragma("unroll") for (int j = 0; j < JS; ++j) {                                             \
-            const float a = (sq[w0 + arow + j] == abase) ? 1.0f : 0.0f;                             \
-            _Pragma("unroll") for (int gi = 0; gi < G; ++gi)                                         \
-                acc[gi] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bv[gi][j], acc[gi], 0, 0, 0);      \
+            _Pragma("unroll") for (int c = 0; c < CH; ++c) {                                         \
+                const float a = (sq[w0 + 16 * c + arow + j] == abase) ? 1.0f : 0.0f;                \
+                _Pragma("unroll") for (int gi = 0; gi < G; ++gi)                                     \
+                    acc[c][gi] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bv[gi][j], acc[c][gi], 0, 0, 0); \
+            }                                                                                        \
         }                                                                                            \
         break;
+#define CDLM_MFMA_STEPS8(J0)                                                                         \
+    CDLM_MFMA_STEPS(J0 + 1) CDLM_MFMA_STEPS(J0 + 2) CDLM_MFMA_STEPS(J0 + 3) CDLM_MFMA_STEPS(J0 + 4)  \
+    CDLM_MFMA_STEPS(J0 + 5) CDLM_MFMA_STEPS(J0 + 6) CDLM_MFMA_STEPS(J0 + 7) CDLM_MFMA_STEPS(J0 + 8)
             switch (jsteps) {
-                CDLM_MFMA_STEPS(4) CDLM_MFMA_STEPS(8) CDLM_MFMA_STEPS(12) CDLM_MFMA_STEPS(16)
-                CDLM_MFMA_STEPS(20) CDLM_MFMA_STEPS(24) CDLM_MFMA_STEPS(28) CDLM_MFMA_STEPS(32)
+                CDLM_MFMA_STEPS8(0) CDLM_MFMA_STEPS8(8) CDLM_MFMA_STEPS8(16) CDLM_MFMA_STEPS8(24)
                 default:
                     if constexpr (MAXJ > 32) {
                         switch (jsteps) {
-                            CDLM_MFMA_STEPS(36) CDLM_MFMA_STEPS(40) CDLM_MFMA_STEPS(44) CDLM_MFMA_STEPS(48)
-                            CDLM_MFMA_STEPS(52) CDLM_MFMA_STEPS(56) CDLM_MFMA_STEPS(60) CDLM_MFMA_STEPS(64)
+                            CDLM_MFMA_STEPS8(32) CDLM_MFMA_STEPS8(40) CDLM_MFMA_STEPS8(48) CDLM_MFMA_STEPS8(56)
                             default: break;
                         }
                     }
                     break;
             }
+#undef CDLM_MFMA_STEPS8
 #undef CDLM_MFMA_STEPS
             int myc = 0;
             unsigned hitmask = 0;
 #pragma unroll
-            for (int gi = 0; gi < G; ++gi)
+            for (int c = 0; c < CH; ++c)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int l0 = w0 + 4 * abase + r;
-                    const bool h = sk[gi] >= 0 && l0 + sm[gi] <= L && acc[gi][r] > sth[gi];
-                    if (h) { hitmask |= 1u << (gi * 4 + r); ++myc; }
-                }
+                for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int l0 = w0 + 16 * c + 4 * abase + r;
+                        const bool h = sk[gi] >= 0 && l0 + sm[gi] <= L && acc[c][gi][r] > sth[gi];
+                        if (h) { hitmask |= 1u << ((c * G + gi) * 4 + r); ++myc; }
+                    }
             if (__ballot(myc > 0)) {
                 int incl = myc;
 #pragma unroll
@@ -133,15 +152,18 @@ __global__ __launch_bounds__(256) void k_pwm_hits_mfma(const uint8_t *__restrict
                 base = __shfl(base, 0);
                 unsigned long long slot = base + (unsigned long long)(incl - myc);
 #pragma unroll
-                for (int gi = 0; gi < G; ++gi)
+                for (int c = 0; c < CH; ++c)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        if (hitmask & (1u << (gi * 4 + r))) {
-                            if ((long long)slot < cap) {
-                                hk[slot] = sk[gi]; hn[slot] = n; hp[slot] = w0 + 4 * abase + r; hs[slot] = (signed char)sst[gi];
+                    for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            if (hitmask & (1u << ((c * G + gi) * 4 + r))) {
+                                if ((long long)slot < cap) {
+                                    hk[slot] = sk[gi]; hn[slot] = n; hp[slot] = w0 + 16 * c + 4 * abase + r;
+                                    hs[slot] = (signed char)sst[gi];
+                                }
+                                ++slot;
                             }
-                            ++slot;
-                        }
             }
         }
         wave_lds_sync();
@@ -388,10 +410,10 @@ cdlm_status launch_hits(cdlm_ctx *ctx, const uint8_t *d_bases, int L, long N, co
         PwmTiles P;
         P.T = ht.T; P.B = dt.B; P.boff = dt.boff; P.tlen = dt.tlen;
         P.slot_k = dt.sk; P.slot_m = dt.sm; P.slot_st = dt.sst; P.slot_th = dt.sth;
-        const int G = maxlen <= 32 ? 4 : 2;
+        const int G = kPwmG;
         const int MAXJ = maxlen <= 32 ? 32 : 64;
         P.groups = (ht.T + G - 1) / G;
-        const int Lpad = ((L + 15) & ~15) + MAXJ + 16;
+        const int Lpad = pwm_lpad(L, MAXJ, kPwmCH);
         const size_t lds = 4 * (size_t)Lpad;
         long yb = (N + 3) / 4;
         const long target = (long)ctx->num_cu * 16 / (P.groups > 0 ? P.groups : 1) + 1;
@@ -405,12 +427,12 @@ cdlm_status launch_hits(cdlm_ctx *ctx, const uint8_t *d_bases, int L, long N, co
         }
         if (e0) (void)hipEventRecord(e0, st);
         if (lds > 64 * 1024) s = cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "sequence length %d too long for the scan", L);
-        else if (G == 4)
-            hipLaunchKernelGGL((k_pwm_hits_mfma<4, 32>), dim3(P.groups, (unsigned)yb), dim3(256), lds, st, d_bases, L, N,
-                               P, hk, hn, hp, hs, cap, d_count);
+        else if (maxlen <= 32)
+            hipLaunchKernelGGL((k_pwm_hits_mfma<kPwmG, 32, kPwmCH>), dim3(P.groups, (unsigned)yb), dim3(256), lds, st,
+                               d_bases, L, N, P, hk, hn, hp, hs, cap, d_count);
         else
-            hipLaunchKernelGGL((k_pwm_hits_mfma<2, 64>), dim3(P.groups, (unsigned)yb), dim3(256), lds, st, d_bases, L, N,
-                               P, hk, hn, hp, hs, cap, d_count);
+            hipLaunchKernelGGL((k_pwm_hits_mfma<kPwmG, 64, kPwmCH>), dim3(P.groups, (unsigned)yb), dim3(256), lds, st,
+                               d_bases, L, N, P, hk, hn, hp, hs, cap, d_count);
         hipError_t e = hipGetLastError();
         if (s == CDLM_OK && e != hipSuccess) s = cdlm_fail(ctx, CDLM_ERR_HIP, "k_pwm_hits_mfma: %s", hipGetErrorString(e));
         if (e1) (void)hipEventRecord(e1, st);

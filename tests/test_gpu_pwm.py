@@ -138,3 +138,18 @@ def test_mfma_layout_and_rounding_probe():
             p = b1[(i % 4) * 16 + j]                  # B[k=i%4][j]
             exp = np.float32(np.float32(c1[4 * l + r]) + p)
             assert d[4 * l + r].view(np.uint32) == exp.view(np.uint32), (l, r)
+
+
+@pytest.mark.parametrize("L,lmin,lmax,frac", [(300, 30, 64, 0.0), (77, 8, 33, 0.35), (64, 1, 16, 0.8)])
+def test_hits_long_pwms_and_ragged_lengths_vs_oracle(L, lmin, lmax, frac):
+    """PWMs longer than 32 columns (the 64-column kernel), sequence lengths that are not a
+    multiple of the kernel's window chunk, single-column PWMs; thresholds frac x best score."""
+    bases, pwms, lens, _ = _random_case(L + lmax, 90, L, 20, lmin, lmax)
+    thresh = np.array([np.float32(frac) * O.best_score(pwms[k, :, :lens[k]]) for k in range(len(lens))], np.float32)
+    k, n, p, st, _ = O.pwm_hits(bases, pwms, lens, thresh, both_strands=True)
+    exp = np.stack([k, n, p, st], 1).astype(np.int64)
+    exp = exp[np.lexsort((exp[:, 3], exp[:, 2], exp[:, 1], exp[:, 0]))]
+    got = cdlm.scan_hits(bases, pwms, lens, thresh, both_strands=True)
+    flush("L=%d lens %d-%d: %d hits" % (L, lmin, lmax, len(exp)))
+    assert len(exp) > 0
+    np.testing.assert_array_equal(got, exp)
