@@ -210,6 +210,81 @@ def served_leg(rate: float, n_jobs: int, log, policy: str = "fifo"):
     }
 
 
+def reference_loop_leg(s, fl, D0, t_fused_step, n_batch, csc_sample, log):
+    """The plain per-dataset GPU loop, as the reference runs it, at the single-dataset
+    configuration, against the batched form on the same shape.
+
+    Plain loop: cdlm.refloop (the reference's code_update!/dict_update!/update_best!
+    statement for statement on torch + hipFFT, a device synchronisation at each CUDA.@sync)
+    on the full N=20000 problem.  Bounded sample: after one untimed outer iteration (FFT
+    plan creation), csc_sample CSC iterations, update_best and one dictionary iteration are
+    timed; a training is 10 x (20 CSC + update_best + 1 dict), extrapolated from those
+    per-call times.  Initialisation and the code threshold are left out, which favours the
+    plain loop.  Batched: n_batch datasets of the same shape (other seeds) trained in one
+    batch (a captured HIP graph), problem creation and teardown included."""
+    import torch
+    import cdlm
+    from cdlm import synth
+    from cdlm.refloop import ReferenceStructuredCDL
+    N = s.shape[0]
+    Lg = 4 * L_BASES + 2 * FLEN
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    X0 = torch.randn((N, M_FILT, Lg), generator=g, device="cuda", dtype=torch.float32)
+    t0 = time.perf_counter()
+    r = ReferenceStructuredCDL(s, fl, RHO, SIGMA, LAM, D0, X0)
+    t_init = time.perf_counter() - t0
+    del X0
+    r.code_update(1)
+    r.update_best()
+    r.dict_update(1)
+    r.sync()
+    t0 = time.perf_counter()
+    r.code_update(csc_sample)
+    r.sync()
+    t_csc = (time.perf_counter() - t0) / csc_sample
+    t0 = time.perf_counter()
+    r.update_best()
+    r.sync()
+    t_best = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r.dict_update(CDL_ITER)
+    r.sync()
+    t_dict = time.perf_counter() - t0
+    peak = torch.cuda.max_memory_allocated() / 1e9
+    del r
+    torch.cuda.empty_cache()
+    t_train = ITER_NUM * (CSC_ITER * t_csc + t_best + CDL_ITER * t_dict)
+    log("bench: reference-structured loop at N=%d: CSC %.1f ms, update_best %.1f ms, dict %.1f ms per call "
+        "(init %.2f s, peak %.0f GB) -> %.2f s per training"
+        % (N, 1e3 * t_csc, 1e3 * t_best, 1e3 * t_dict, t_init, peak, t_train))
+    out = {"workload": "single-dataset configuration N=%d L=%d M=%d filter_len=%d, 10 x (20 CSC + update_best + "
+                       "1 dict) per training" % (N, L_BASES, M_FILT, FLEN),
+           "plain_loop": {"csc_ms": 1e3 * t_csc, "update_best_ms": 1e3 * t_best, "dict_ms": 1e3 * t_dict,
+                          "init_s": t_init, "peak_device_GB": peak, "training_s": t_train,
+                          "datasets_per_s": 1.0 / t_train,
+                          "sample": "%d CSC iterations + update_best + %d dict timed after one untimed outer "
+                                    "iteration; training time extrapolated, initialisation excluded"
+                                    % (csc_sample, CDL_ITER)},
+           "fused_single": {"training_s": t_fused_step, "datasets_per_s": 1.0 / t_fused_step,
+                            "vs_plain_loop": t_train / t_fused_step}}
+    if n_batch > 0:
+        ctx = cdlm.default_context()
+        sets = [synth.onehot(synth.c2_dataset(seed=100 + i, N=N, L=L_BASES)[0]) for i in range(n_batch)]
+        t0 = time.perf_counter()
+        probs = [cdlm.CDL(sets[i], fl, RHO, SIGMA, LAM, D0=D0, X0=None, seed=200 + i, ctx=ctx) for i in range(n_batch)]
+        b = cdlm.Batch(probs)
+        _tr, th, _cnt = b.train(ITER_NUM, CSC_ITER, CDL_ITER, 1.0)
+        b.close()
+        for p in probs:
+            p.close()
+        t_b = time.perf_counter() - t0
+        log("bench: batched %d datasets of the same shape in %.2f s (%.3f datasets/s)" % (n_batch, t_b, n_batch / t_b))
+        out["batched"] = {"datasets": n_batch, "seconds": t_b, "datasets_per_s": n_batch / t_b,
+                          "vs_plain_loop": (n_batch / t_b) * t_train}
+    return out
+
+
 def topology_leg(c, log):
     """Activation record and topology (record_training_result!, get_spatial_info!,
     src/SEARCH/store.jl) for two trainings at the single-dataset configuration: the timed
@@ -348,6 +423,10 @@ def main():
                     help="datasets with N in [200, 2000] in the small-dataset batched leg (0: skip it)")
     ap.add_argument("--serve-policy", default="fifo", choices=["fifo", "shortest"], help="server admission order")
     ap.add_argument("--serve-jobs", type=int, default=16, help="jobs in the served leg (0: skip it)")
+    ap.add_argument("--reference-loop", type=int, default=2, metavar="CSC_ITERS",
+                    help="CSC iterations timed in the reference-structured plain-loop leg (0: skip it)")
+    ap.add_argument("--reference-batch", type=int, default=4,
+                    help="datasets of the single-dataset shape in that leg's batched comparison (0: skip it)")
     ap.add_argument("--outer", type=int, default=ITER_NUM, help="outer iterations per step (profiling runs only)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB GiB (HBM counter calibration) and exit")
@@ -372,6 +451,11 @@ def main():
                           "GBps": 2 * nbytes / (ms.value * 1e-3) / 1e9, "reps": 5, "warmup_launches": 1}))
         return
 
+    if args.reference_loop > 0:
+        # torch carries its own HIP runtime; it has to open the device before libcdlm's
+        # runtime does, or it finds no device later in the process
+        import torch
+        torch.cuda.init()
     N = args.n
     Lg = 4 * L_BASES + 2 * FLEN
     fl = [FLEN] * M_FILT
@@ -454,6 +538,10 @@ def main():
         out["topology"] = topology_leg(c, lambda m: print(m, file=sys.stderr, flush=True))
     c.close()
     del c
+    if args.reference_loop > 0:
+        out["reference_loop"] = reference_loop_leg(s, fl, D0, dt / args.steps, args.reference_batch,
+                                                   args.reference_loop,
+                                                   lambda m: print(m, file=sys.stderr, flush=True))
     if args.ingest:
         out["ingest"] = ingest_leg(bases, s, fl, D0, lambda m: print(m, file=sys.stderr, flush=True))
     if args.batch_datasets > 0:
