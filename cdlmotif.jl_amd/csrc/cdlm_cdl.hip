@@ -102,24 +102,32 @@ int pick_csc_waves(int Lg, int taps, int nsl, int tuned) {
 }
 size_t lds_bytes_init(int Lg, int taps) { return sizeof(float) * (kWaves * (size_t)(Lg + taps) + kWaves * (size_t)Lg); }
 size_t lds_bytes_best(int Lg, int taps) { return sizeof(float) * (size_t)(Lg + taps + 4); }
-size_t lds_bytes_dict(int Lg, int taps, int B) {
-    const int H = Lg / 2 + 1, Hp = (H + 3) & ~3;
-    const size_t Pq = (size_t)Lg / 4;
-    return sizeof(float) * (2 * (size_t)Lg + (size_t)(Lg + taps + 4) + (size_t)((B * taps + 3) & ~3) + 4 * (size_t)B * Lg +
-                            2 * 2 * (size_t)(B / 2) * Pq + 2 * (size_t)B * Pq + 5 * (size_t)Hp);
+// k_dict with W waves: twiddles, h with its tail, three spectrum accumulators, two length-Lg
+// complex buffers, and per wave two of length Lg/4.
+size_t lds_bytes_dict(int Lg, int taps, int W) {
+    const size_t Hp = (size_t)(((Lg / 2 + 1) + 3) & ~3);
+    return sizeof(float) * (2 * (size_t)Lg + (size_t)((Lg + taps + 4 + 3) & ~3) + 5 * Hp + 4 * (size_t)Lg +
+                            (size_t)W * (size_t)Lg);
 }
+size_t lds_bytes_half(int Lg) { return sizeof(float2) * 3 * (size_t)Lg; }
 size_t lds_bytes_fin(int Lg) { return sizeof(float) * (((size_t)Lg + 3) / 4 * 4 + 6 * (size_t)Lg); }
 size_t lds_bytes_g1(int Lg) { return sizeof(float) * 6 * (size_t)Lg; }
 
-// Dictionary columns per batched FFT pass: the most columns in flight per CU.
+// Work-groups of W dictionary waves that fit one CU: by registers (k_dict keeps 3 waves per
+// SIMD for padded lengths up to 1020 and filters up to 24, else 2) and by LDS.
+int dict_wg_per_cu(int Lg, int taps, int W) {
+    const size_t b = lds_bytes_dict(Lg, taps, W);
+    if (b > kLdsLimit - kThreads * sizeof(double)) return 0;
+    const int wps = (Lg <= 1020 && taps <= 24) ? 3 : 2;
+    return std::min<int>(4 * wps / W, (int)(kLdsLimit / b));
+}
+// Dictionary waves per work-group (columns in flight, one per wave): the most waves per CU;
+// on a tie 4, then 8.
 int pick_dict_batch(int Lg, int taps) {
-    int best = 0;
-    long bestv = 0;
-    for (int B : {2, 4, 8}) {
-        const size_t b = lds_bytes_dict(Lg, taps, B);
-        if (b > kLdsLimit - kThreads * sizeof(double)) continue;
-        const long v = (long)B * (long)(B == 8 ? 1 : kLdsLimit / b);   // B = 8: one 1024-thread work-group per CU
-        if (v > bestv) { bestv = v; best = B; }
+    int best = 0, bestv = 0;
+    for (int W : {4, 8, 2}) {
+        const int v = W * dict_wg_per_cu(Lg, taps, W);
+        if (v > bestv) { bestv = v; best = W; }
     }
     return best;
 }
@@ -154,7 +162,7 @@ cdlm_status ensure_lds(cdlm_ctx *ctx, const void *fn, size_t bytes) {
 struct Tuning {
     int order = kFftSmallFirst;   // stage order of the length-Lg transforms
     int order4 = kFftSmallFirst;  // ... of the length-Lg/4 transforms (dictionary step)
-    int B = 0;                    // dictionary columns per pass (0: largest-occupancy choice)
+    int B = 0;                    // dictionary waves per work-group, one column each (0: by occupancy)
     int nsl = 0;                  // CSC float4 slots per lane (0: the fewest that cover Lg)
     int waves = 0;                // CSC waves per work-group (4 or 16; 0: the default for the shape)
 };
@@ -213,7 +221,7 @@ namespace {
 // Buffer plan of one problem: sizes in order of allocation.  The footprint query and the
 // allocator both use it, so cdlm_problem_bytes == cdlm_footprint_bytes.
 struct BufPlan {
-    size_t s, E, E3, Ex0, LN, LM, Lg, xs, y3s, M, partials, dpart, trace, sc;
+    size_t s, E, Eh, E3, Ex0, LN, LM, LMh, Lg, xs, y3s, M, partials, dpart, trace, sc;
 };
 
 BufPlan buf_plan(int L4, int N, int M, int gamma, bool keep_x0) {
@@ -221,24 +229,26 @@ BufPlan buf_plan(int L4, int N, int M, int gamma, bool keep_x0) {
     const size_t Lg = (size_t)L4 + 2 * (size_t)gamma;
     b.s = (size_t)L4 * N * sizeof(float);
     b.E = Lg * M * (size_t)N * sizeof(float);
+    b.Eh = (size_t)half_bins((int)Lg) * M * (size_t)N * sizeof(float2);   // d, g1, g3 as half spectra
     b.E3 = 3 * (Lg / 4) * M * (size_t)N * sizeof(float);   // y3 off the stride-4 support
     b.Ex0 = keep_x0 ? b.E : 0;
     b.LN = Lg * (size_t)N * sizeof(float);
     b.LM = Lg * (size_t)M * sizeof(float);
+    b.LMh = (size_t)half_bins((int)Lg) * M * sizeof(float2);
     b.Lg = Lg * sizeof(float);
     b.xs = (size_t)(L4 / 4) * M * (size_t)N * sizeof(float);
     b.y3s = (size_t)M * N * (Lg / 8 + 1) * sizeof(float2);
     b.M = (size_t)M * sizeof(int);
     b.partials = (size_t)N * kPartStride * sizeof(double);
-    b.dpart = (size_t)pick_nch((int)Lg, M, N) * 3 * (Lg * M) * sizeof(double);
+    b.dpart = (size_t)pick_nch((int)Lg, M, N) * 6 * ((size_t)half_bins((int)Lg) * M) * sizeof(double);
     b.trace = (size_t)kTraceCap * sizeof(double);
     b.sc = sizeof(CdlScal);
     return b;
 }
 
 size_t plan_total(const BufPlan &b) {
-    return b.s + 5 * b.E + b.E3 + b.Ex0 + 7 * b.LN + 7 * b.LM + 2 * b.Lg /* cS, sS */ + 2 * b.Lg /* tw */ + 2 * b.xs + b.y3s +
-           b.M + b.partials + b.dpart + b.trace + b.sc;
+    return b.s + 2 * b.E + 3 * b.Eh + b.E3 + b.Ex0 + 7 * b.LN + 7 * b.LM + 3 * b.LMh + 2 * b.Lg /* cS, sS */ +
+           2 * b.Lg /* tw */ + 2 * b.xs + b.y3s + b.M + b.partials + b.dpart + b.trace + b.sc;
 }
 
 cdlm_status palloc(cdlm_problem *p, void **out, size_t bytes) {
@@ -314,8 +324,8 @@ cdlm_status set_build(cdlm_ctx *ctx, cdlm_problem *const *ps, int np, LaunchSet 
         if ((st = ensure_lds(ctx, (const void *)k_csc<TT, 4, 16>, T.lds_csc)) != CDLM_OK) return st;  \
         if ((st = ensure_lds(ctx, (const void *)k_csc<TT, 4, 8>, T.lds_csc)) != CDLM_OK) return st;   \
         if ((st = ensure_lds(ctx, (const void *)k_best<TT>, T.lds_best)) != CDLM_OK) return st;    \
-        if ((st = ensure_lds(ctx, (const void *)k_dict<TT, kThreads>, T.lds_dict)) != CDLM_OK) return st;   \
-        if ((st = ensure_lds(ctx, (const void *)k_dict<TT, 1024>, T.lds_dict)) != CDLM_OK) return st;    \
+        if ((st = ensure_lds(ctx, (const void *)k_dict<TT, 4>, T.lds_dict)) != CDLM_OK) return st;         \
+        if ((st = ensure_lds(ctx, (const void *)k_dict<TT, 8>, T.lds_dict)) != CDLM_OK) return st;         \
     } while (0)
     switch (taps) {
     case 4: SETA(4); break;
@@ -331,6 +341,7 @@ cdlm_status set_build(cdlm_ctx *ctx, cdlm_problem *const *ps, int np, LaunchSet 
 #undef SETA
     if ((st = ensure_lds(ctx, (const void *)k_dict_finalize, T.lds_fin)) != CDLM_OK) return st;
     if ((st = ensure_lds(ctx, (const void *)k_g1_spectrum, T.lds_g1)) != CDLM_OK) return st;
+    if ((st = ensure_lds(ctx, (const void *)k_half_to_spatial, lds_bytes_half(Lg))) != CDLM_OK) return st;
 
     std::vector<CdlDev> tab(np);
     std::vector<int> pre(5 * (size_t)np);
@@ -343,11 +354,11 @@ cdlm_status set_build(cdlm_ctx *ctx, cdlm_problem *const *ps, int np, LaunchSet 
         pre[2 * np + i] = (int)a_cs;
         pre[3 * np + i] = (int)a_red;
         pre[4 * np + i] = slots ? slots[i] : i;
-        const long LM = (long)p->Lg * p->M;
+        const long HM = (long)half_bins(p->Lg) * p->M;
         a_seq += p->N;
         a_filt += p->M;
         a_cs += (p->Lg + kThreads - 1) / kThreads;
-        a_red += ((LM + kThreads - 1) / kThreads) * p->dv.nch;
+        a_red += ((HM + kThreads - 1) / kThreads) * p->dv.nch;
     }
     if (a_seq > INT32_MAX || a_red > INT32_MAX) return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "batch too large for one grid");
     T.blk_seq = (int)a_seq;
@@ -435,12 +446,12 @@ cdlm_status run_dict(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm) {
     {
         TimedLaunch tl(tm, st, 1);
 #define L_DICT(T)                                                                                                    \
-    if (S.B == 8)                                                                                                    \
-        hipLaunchKernelGGL((k_dict<T, 1024>), dim3(S.blk_seq), dim3(1024), S.lds_dict, st, S.tab, S.st_seq, S.np, S.B, \
+    if (S.LgMax <= 1020)                                                                                             \
+        hipLaunchKernelGGL((k_dict<T, 4>), dim3(S.blk_seq), dim3(64 * S.B), S.lds_dict, st, S.tab, S.st_seq, S.np,   \
                            S.LgMax);                                                                                 \
     else                                                                                                             \
-        hipLaunchKernelGGL((k_dict<T, kThreads>), dim3(S.blk_seq), dim3(kThreads), S.lds_dict, st, S.tab, S.st_seq,  \
-                           S.np, S.B, S.LgMax)
+        hipLaunchKernelGGL((k_dict<T, 8>), dim3(S.blk_seq), dim3(64 * S.B), S.lds_dict, st, S.tab, S.st_seq, S.np,   \
+                           S.LgMax)
         CDLM_TAPS_SWITCH(S.taps, L_DICT);
 #undef L_DICT
     }
@@ -613,11 +624,12 @@ cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const flo
     float2 *tw_dev = nullptr;
 #define A(ptr, bytes) do { st = palloc_t(p.get(), &(ptr), (bytes)); if (st != CDLM_OK) { free_problem(p.get()); return st; } } while (0)
     A(s_dev, b.s);
-    A(d.Xt, b.E); A(d.y2, b.E); A(d.y3, b.E3); A(d.d, b.E); A(d.g1, b.E); A(d.g3, b.E);
+    A(d.Xt, b.E); A(d.y2, b.E); A(d.y3, b.E3); A(d.dh, b.Eh); A(d.g1h, b.Eh); A(d.g3h, b.Eh);
     if (X0) A(X0_dev, b.Ex0);
     A(d.h, b.LN); A(d.y1, b.LN); A(d.Y1, b.LN); A(d.DX, b.LN); A(d.G2, b.LN); A(d.g2, b.LN); A(d.Td, b.LN);
     A(d.G1b[0], b.LM); A(d.G1b[1], b.LM); A(d.G3, b.LM); A(d.meand, b.LM); A(d.Dbest, b.LM); A(d.gsq, b.LM);
     A(D0_dev, b.LM);
+    A(d.G1hb[0], b.LMh); A(d.G1hb[1], b.LMh); A(d.G3h, b.LMh);
     A(d.cS, b.Lg); A(d.sS, b.Lg);
     A(tw_dev, 2 * b.Lg);
     A(d.xs[0], b.xs); A(d.xs[1], b.xs);
@@ -803,7 +815,25 @@ extern "C" cdlm_status cdlm_get(cdlm_problem *p, int32_t field, void *dst, size_
         return CDLM_OK;
     }
     case CDLM_F_DX: src = d.DX; cnt = LN; break;
-    case CDLM_F_D: src = d.d; cnt = E; break;
+    case CDLM_F_D:
+    case CDLM_F_g1:
+    case CDLM_F_g3: {
+        // held as half spectra: back to the spatial domain on the device first
+        if (bytes != E * sizeof(float)) return cdlm_fail(p->ctx, CDLM_ERR_ARG, "field %d needs %zu bytes", field, E * sizeof(float));
+        const float2 *h = field == CDLM_F_D ? d.dh : field == CDLM_F_g1 ? d.g1h : d.g3h;
+        float *tmp = nullptr;
+        hipError_t e = hipMalloc(&tmp, E * sizeof(float));
+        if (e != hipSuccess) { (void)hipGetLastError(); return cdlm_fail(p->ctx, CDLM_ERR_OOM, "no memory for field %d", field); }
+        hipLaunchKernelGGL(k_half_to_spatial, dim3((unsigned)((size_t)p->M * p->N)), dim3(kThreads), lds_bytes_half(p->Lg), st,
+                           p->set.tab, h, tmp);
+        cdlm_status s = lcheck(p->ctx, "k_half_to_spatial");
+        hipError_t e2 = s == CDLM_OK ? hipMemcpyAsync(dst, tmp, bytes, hipMemcpyDeviceToHost, st) : hipSuccess;
+        hipError_t e3 = hipStreamSynchronize(st);
+        (void)hipFree(tmp);
+        if (s != CDLM_OK) return s;
+        if (e2 != hipSuccess || e3 != hipSuccess) return cdlm_fail(p->ctx, CDLM_ERR_HIP, "copy of field %d failed", field);
+        return CDLM_OK;
+    }
     case CDLM_F_G1: src = d.G1b[sc.g1cur & 1]; cnt = LM; break;
     case CDLM_F_G2: src = d.G2; cnt = LN; break;
     case CDLM_F_G3: src = d.G3; cnt = LM; break;
@@ -829,9 +859,7 @@ extern "C" cdlm_status cdlm_get(cdlm_problem *p, int32_t field, void *dst, size_
             }
         return CDLM_OK;
     }
-    case CDLM_F_g1: src = d.g1; cnt = E; break;
     case CDLM_F_g2: src = d.g2; cnt = LN; break;
-    case CDLM_F_g3: src = d.g3; cnt = E; break;
     case CDLM_F_TD: src = d.Td; cnt = LN; break;
     default: return cdlm_fail(p->ctx, CDLM_ERR_ARG, "unknown field %d", field);
     }
@@ -907,7 +935,7 @@ ShapeClass shape_class(const cdlm_problem *p) {
     c.occ_csc = c.waves >= 8 ? 1
                               : std::min<int>(c.nsl == 4 && c.taps <= 20 ? 3 : 2,
                                               (int)(kLdsLimit / lds_bytes_csc(p->Lg, p->gamma, c.waves)));
-    c.occ_dict = c.B == 8 ? 1 : std::min<int>(8, (int)(kLdsLimit / lds_bytes_dict(p->Lg, p->gamma, c.B)));
+    c.occ_dict = c.B > 0 ? dict_wg_per_cu(p->Lg, p->gamma, c.B) : 0;
     return c;
 }
 

@@ -66,10 +66,12 @@ struct CdlDev {
     float rho, sigma, lambda, sh_csc, sh_cdl;
     const float *s;
     const int *fl;
-    float *Xt, *y2, *d, *g1, *g3;                 // (Lg, M, N)
+    float *Xt, *y2;                                // (Lg, M, N)
+    float2 *dh, *g1h, *g3h;                        // (Hs, M, N): half spectra of d, g1, g3
     float *y3;                                     // (3 Lg/4, M, N): y3 off the stride-4 support
     float *h, *y1, *Y1, *DX, *G2, *g2, *Td;       // (Lg, N)
     float *G1b[2], *G3, *meand, *Dbest, *gsq;     // (Lg, M)
+    float2 *G1hb[2], *G3h;                         // (Hs, M): half spectra of G1b[i], G3
     float *cS, *sS;                                // (Lg)
     float *xs[2];                                  // (L, M, N) X_best slices
     float2 *y3s;                                   // (Lg/8+1, M, N) dictionary scratch
@@ -81,6 +83,10 @@ struct CdlDev {
     unsigned long long seed;
     FftPlan plan, plan4;
 };
+
+// Half-spectrum bins kept per column: Lg/2 + 1 rounded up to even, so that a column is a
+// whole number of float4 (two bins each) and columns stay 16-byte aligned.
+__host__ __device__ __forceinline__ int half_bins(int Lg) { return ((Lg / 2 + 1) + 1) & ~1; }
 
 // Block -> (problem, local unit): the last p with starts[p] <= b.
 __device__ __forceinline__ int find_problem(const int *__restrict__ starts, int np, int b, int &local) {
@@ -255,7 +261,8 @@ __global__ void k_gather(const CdlDev *__restrict__ tab, const int *__restrict__
 
 // ---------------------------------------------------------------------------- init
 // CDL_setup.jl:183-230 for one sequence: X = Pi4(X0); DX = sum_m D_m (x) X_m (the
-// spectral product of :201-203 as a circular convolution); d = D; Y1 = y = g = 0; G2 = DX.
+// spectral product of :201-203 as a circular convolution); d = D (as F(D)); Y1 = y = g = 0;
+// G2 = DX.
 template <int TAPS>
 __global__ __launch_bounds__(kThreads) void k_init(const CdlDev *__restrict__ tab, const int *__restrict__ starts,
                                                    int np) {
@@ -283,10 +290,18 @@ __global__ __launch_bounds__(kThreads) void k_init(const CdlDev *__restrict__ ta
             st4(P.Xt + col, s, xp);
             st4(P.y2 + col, s, zero4);
             st3(P.y3 + col3_of(n, mm, M, nslot), s, reinterpret_cast<const float *>(&zero4));
-            st4(P.g1 + col, s, zero4);
-            st4(P.g3 + col, s, zero4);
-            st4(P.d + col, s, ld4(G1 + (size_t)mm * Lg, s));
             vb4[TAPS / 4 + s] = xp;
+        }
+        // d = D (dh = F(D), CDL_setup.jl:207-208), g1 = g3 = 0, as half spectra
+        {
+            const int Hs = half_bins(Lg);
+            const size_t colh = ((size_t)n * M + mm) * Hs;
+            const float2 *G1h = P.G1hb[P.sc->g1cur & 1] + (size_t)mm * Hs;
+            for (int t = lane; t < Hs / 2; t += 64) {
+                st4(reinterpret_cast<float *>(P.g1h + colh), t, zero4);
+                st4(reinterpret_cast<float *>(P.g3h + colh), t, zero4);
+                st4(reinterpret_cast<float *>(P.dh + colh), t, ld4(reinterpret_cast<const float *>(G1h), t));
+            }
         }
         wave_lds_sync();
         v_head<TAPS>(vb4, nslot, lane);
@@ -642,139 +657,176 @@ __global__ __launch_bounds__(kThreads) void k_decide(const CdlDev *__restrict__ 
         for (int i = threadIdx.x; i < P.Lg * P.M; i += kThreads) gp(P.Dbest)[i] = gp(P.meand)[i];
 }
 
-// X on the first sample of slot s only (the stride-4 support of Y3).
-template <int TAPS>
-__device__ __forceinline__ float recon1(float xt0, const float *hb, int s, const float *g) {
-    float acc = xt0;
-#pragma unroll
-    for (int j = 0; j < TAPS; ++j) acc = fmaf(g[j], hb[4 * s + j], acc);
-    return acc;
-}
-
 // ---------------------------------------------------------------------------- dict
-// One iteration of dict_update! (train.jl:50-72) for one sequence (THR threads: 256 with 2 or
-// 4 columns per pass and several sequences per CU, or 1024 with 8 columns per pass and one
-// sequence per CU, which needs a quarter of the passes and so of the barrier phases); the means over n
-// (:76,:83) follow in k_dict_reduce / k_dict_finalize.  Columns go through the transforms B
-// at a time (B even): pass 1 packs two columns' Y3 into one complex length-Lg/4 FFT, stores
-// each column's half spectrum in y3s and, unless X is held materialised already, writes it
-// back materialised (Xt = X, h = 0), since G1 changes after this step; pass 2
-// packs two columns' real inverse transforms into one complex FFT.  Accumulations over m
-// stay in ascending m.
-// LDS: twiddles | h with wrapped tail | taps of the batch | fa, fb (B x Lg) | ya, yb
-// (B/2 x P) | ys (B x P) | accS, accT, accP (Lg/2+1 bins), with P = Lg/4.
-template <int TAPS, int THR>
-__global__ __launch_bounds__(THR) void k_dict(const CdlDev *__restrict__ tab, const int *__restrict__ starts, int np,
-                                                   int B, int LgMax) {
+// One iteration of dict_update! (train.jl:50-72) for one sequence; the means over n
+// (:76,:83) follow in k_dict_reduce / k_dict_finalize.  d, g1 and g3 are held as half
+// spectra (dh, g1h, g3h: bins 0..Lg/2), which the reference keeps too (dh, g1h, g3h), so the
+// only per-column transform left is that of Y3 = Pi4(X + y3) = Pi4(X) (y3 = 0 on the
+// support): one sample per slot, so Y3h is a length-Lg/4 DFT repeated 4 times.
+//   pass 1, columns m = w, w+W, ... of wave w (W = blockDim/64): Y3h by a wave-level DFT;
+//     with Qh = (G1h - g1h) + (G3h - g3h) (cdl_kernels.jl:8), |Y3h|^2, Y3h dh and Y3h Qh add
+//     into the wave's registers in ascending m (:52-59); the wave partials are then summed in
+//     wave order.  Unless X is held materialised already it is written back materialised
+//     (Xt = X, h = 0), since G1 changes after this step.
+//   a block-wide spectral step gives Td = F^-1(sum_m Y3h dh) (:56-57), G2, g2 (update_g2!,
+//     :68-71) and kappa = (c1 F(G2 - g2) - sum_m Y3h Qh) / (|Y3h|^2 + c1).
+//   pass 2, the same columns: dh = Qh + conj(Y3h) kappa (sherman_CDL_combine, :60),
+//     g1h += dh - G1h, g3h += dh - G3h (:70-72 in the transform domain).
+// The partial-sum order depends on W, which belongs to the problem's launch choices (a
+// problem runs with the same W alone and in a batch).  Y3h is taken Hermitian from its half
+// spectrum (what pass 1 leaves in y3s).  A lane owns the float4 slots t = lane + 64 i of a
+// spectral column (bins 2t, 2t+1) and s = lane + 64 i of a spatial one.
+// LDS: twiddles | h with wrapped tail | accS, accT, accP (kappa) | fa, fb (Lg complex, the
+// block transforms) | per wave: ya, yb (LgMax/4 complex each).
+template <int TAPS, int NSL>
+__global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(const CdlDev *__restrict__ tab, const int *__restrict__ starts,
+                                                                  int np, int LgMax) {
+    constexpr bool kPf = NSL <= 4;              // spectral loads in flight across the Y3 transform
+    constexpr int NQ = NSL <= 4 ? 4 : 9;        // spectral float4 slots per lane: Hs/2 <= 64 NQ
     extern __shared__ float4 smem4[];
     int n;
     const CdlDev &P = tab[find_problem(starts, np, blockIdx.x, n)];
     float *sm = reinterpret_cast<float *>(smem4);
-    const int tid = threadIdx.x;
-    const int Lg = P.Lg, M = P.M, nslot = Lg >> 2, Pq = Lg >> 2, Ph = Pq / 2 + 1;
-    const int H = Lg / 2 + 1;
-    // layout sized for the largest problem of the launch
-    const int HpX = ((LgMax / 2 + 1) + 3) & ~3, PqX = LgMax >> 2;
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int THR = blockDim.x, W = THR >> 6;
+    const int Lg = P.Lg, M = P.M, Pq = Lg >> 2, Ph = Pq / 2 + 1, H = Lg / 2 + 1, Hs = half_bins(Lg), nq = Hs >> 1;
+    const int HsX = ((LgMax / 2 + 1) + 3) & ~3, PqX = LgMax >> 2;
     float2 *tw = reinterpret_cast<float2 *>(sm);
     float *hb = sm + 2 * LgMax;
-    float *gtap = hb + LgMax + TAPS + 4;                  // B * TAPS (rounded to 4)
-    float2 *fa = reinterpret_cast<float2 *>(gtap + ((B * TAPS + 3) & ~3));
-    float2 *fb = fa + B * LgMax;
-    float2 *ya = fb + B * LgMax;
-    float2 *yb = ya + (B / 2) * PqX;
-    float2 *ys = yb + (B / 2) * PqX;
-    float *accS = reinterpret_cast<float *>(ys + B * PqX);
-    float2 *accT = reinterpret_cast<float2 *>(accS + HpX);
-    float2 *accP = accT + HpX;
+    float *accS = hb + ((LgMax + TAPS + 4 + 3) & ~3);
+    float2 *accT = reinterpret_cast<float2 *>(accS + HsX);
+    float2 *accP = accT + HsX;
+    float2 *fa = accP + HsX, *fb = fa + LgMax;
+    float2 *ya = fb + LgMax + w * 2 * PqX, *yb = ya + PqX;
     const size_t ro = (size_t)n * Lg;
-    const float rslot = 1.0f / (float)nslot, rLg = 1.0f / (float)Lg, rPq = 1.0f / (float)Pq;
     float2 *y3s = P.y3s + (size_t)n * M * Ph;
     const float *G1 = g1cur_of(P);
+    const float *G1h = reinterpret_cast<const float *>(P.G1hb[P.sc->g1cur & 1]);
+    const float *G3h = reinterpret_cast<const float *>(P.G3h);
     const bool mat = !P.sc->xmat;   // set again only by k_cs after this step
+    const float4 *hb4 = reinterpret_cast<const float4 *>(hb);
 
     for (int l = tid; l < Lg; l += THR) tw[l] = gp(P.tw)[l];
     if (mat) load_h<TAPS>(P, ro, hb, tid, THR);
-    for (int k = tid; k < HpX; k += THR) { accS[k] = 0.f; accT[k] = make_float2(0.f, 0.f); accP[k] = make_float2(0.f, 0.f); }
+    __syncthreads();
 
-    // ---- pass 1: spectra sums over m; X materialised
-    for (int m0 = 0; m0 < M; m0 += B) {
-        const int nb = (M - m0) < B ? (M - m0) : B;
-        const int npair = (nb + 1) / 2;
-        if (mat)
-            for (int i = tid; i < nb * TAPS; i += THR) {
-                const int j = i % TAPS;
-                gtap[i] = (j < Lg) ? gp(G1)[(size_t)(m0 + i / TAPS) * Lg + j] : 0.f;
-            }
-        if (nb & 1)
-            for (int s = tid; s < Pq; s += THR) ya[(npair - 1) * Pq + s].y = 0.f;
-        __syncthreads();
-        for (int idx = tid; idx < nb * nslot; idx += THR) {
-            int s;
-            const int bb = fdivmod(idx, nslot, rslot, s), m = m0 + bb;
-            const size_t col = ((size_t)n * M + m) * Lg;
-            const float4 xt = ld4(P.Xt + col, s);
-            const float4 dv = ld4(P.d + col, s);
-            const float4 a1 = ld4(P.g1 + col, s);
-            const float4 c3 = ld4(P.g3 + col, s);
-            const float4 G1v = ld4(G1 + (size_t)m * Lg, s);
-            const float4 G3v = ld4(P.G3 + (size_t)m * Lg, s);
-            float x0 = xt.x;
-            if (mat) {
-                float X[4];
-                recon4<TAPS>(xt, reinterpret_cast<const float4 *>(hb), s, gtap + bb * TAPS, X);
-                st4(P.Xt + col, s, mk4(X));
-                x0 = X[0];
-            }
-            float *yaf = reinterpret_cast<float *>(ya);
-            yaf[2 * ((bb >> 1) * Pq + s) + (bb & 1)] = x0;    // Y3 = Pi4(X + y3) = Pi4(X): y3 = 0 on the support
+    // Y3h of bin k from the half spectrum in buffer Y
+    auto y3h = [&](const float2 *Y, int k) {
+        int kq = k;
+        while (kq >= Pq) kq -= Pq;
+        return kq < Ph ? Y[kq] : cconj(Y[Pq - kq]);
+    };
+
+    // ---- pass 1
+    float aS[2 * NQ];
+    float2 aT[2 * NQ], aP[2 * NQ];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float qq = (f4(G1v, q) - f4(a1, q)) + (f4(G3v, q) - f4(c3, q));
-                fa[bb * Lg + 4 * s + q] = make_float2(f4(dv, q), qq);
+    for (int i = 0; i < 2 * NQ; ++i) { aS[i] = 0.f; aT[i] = make_float2(0.f, 0.f); aP[i] = make_float2(0.f, 0.f); }
+    float4 nx[NSL];
+    auto loadx = [&](size_t col) {
+#pragma unroll
+        for (int i = 0; i < NSL; ++i) {
+            const int s = lane + 64 * i;
+            if (s < Pq) nx[i] = ld4(P.Xt + col, s);
+        }
+    };
+    if (w < M) loadx(((size_t)n * M + w) * Lg);
+    for (int m = w; m < M; m += W) {
+        const int mm = __builtin_amdgcn_readfirstlane(m);
+        const size_t col = ((size_t)n * M + mm) * Lg;
+        const size_t colh = ((size_t)n * M + mm) * Hs, fcol = (size_t)mm * Hs;
+        // this column's spectra (dh, Qh), in flight during the Y3 transform
+        float4 dq[kPf ? NQ : 1], qq[kPf ? NQ : 1];
+        auto loadq = [&](int i, float4 &D, float4 &Q) {
+            const int t = lane + 64 * i;
+            if (t < nq) {
+                D = ld4(reinterpret_cast<const float *>(P.dh + colh), t);
+                const float4 a1 = ld4(reinterpret_cast<const float *>(P.g1h + colh), t);
+                const float4 a3 = ld4(reinterpret_cast<const float *>(P.g3h + colh), t);
+                const float4 F1 = ld4(G1h + 2 * fcol, t), F3 = ld4(G3h + 2 * fcol, t);
+                Q = make_float4((F1.x - a1.x) + (F3.x - a3.x), (F1.y - a1.y) + (F3.y - a3.y),
+                                (F1.z - a1.z) + (F3.z - a3.z), (F1.w - a1.w) + (F3.w - a3.w));
+            }
+        };
+        if constexpr (kPf) {
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) loadq(i, dq[i], qq[i]);
+        }
+        float g[TAPS];
+        if (mat) load_taps<TAPS>(G1, mm, Lg, g);
+#pragma unroll
+        for (int i = 0; i < NSL; ++i) {
+            const int s = lane + 64 * i;
+            if (s < Pq) {
+                float x0 = nx[i].x;
+                if (mat) {
+                    float X[4];
+                    recon4<TAPS>(nx[i], hb4, s, g, X);
+                    st4(P.Xt + col, s, mk4(X));
+                    x0 = X[0];
+                }
+                ya[s] = make_float2(x0, 0.f);
             }
         }
-        __syncthreads();
-        float2 *Y = team_fft<false>(ya, yb, P.plan4, tw, tid, THR, BlockSync{}, npair);
-        float2 *C = team_fft<false>(fa, fb, P.plan, tw, tid, THR, BlockSync{}, nb);
-        // unpack the column pairs: Y3h of column 2p is the Hermitian part, of 2p+1 the other
-        for (int idx = tid; idx < nb * Pq; idx += THR) {
-            int k;
-            const int bb = fdivmod(idx, Pq, rPq, k);
-            const float2 Yk = Y[(bb >> 1) * Pq + k], Ym = Y[(bb >> 1) * Pq + (k == 0 ? 0 : Pq - k)];
-            const float2 v = (bb & 1) ? make_float2(0.5f * (Yk.y + Ym.y), -0.5f * (Yk.x - Ym.x))
-                                      : make_float2(0.5f * (Yk.x + Ym.x), 0.5f * (Yk.y - Ym.y));
-            ys[bb * Pq + k] = v;
-            if (k < Ph) gp(y3s)[(size_t)(m0 + bb) * Ph + k] = v;
-        }
-        __syncthreads();
-        for (int k = tid; k < H; k += THR) {
-            int kq;
-            (void)fdivmod(k, Pq, rPq, kq);
-            float aS = accS[k];
-            float2 aT = accT[k], aP = accP[k];
-            for (int bb = 0; bb < nb; ++bb) {
-                const float2 Ck = C[bb * Lg + k], Cm = C[bb * Lg + (k == 0 ? 0 : Lg - k)];
-                const float2 Dh = make_float2(0.5f * (Ck.x + Cm.x), 0.5f * (Ck.y - Cm.y));
-                const float2 Qh = make_float2(0.5f * (Ck.y + Cm.y), -0.5f * (Ck.x - Cm.x));
-                const float2 Yk = ys[bb * Pq + kq];
-                aS += Yk.x * Yk.x + Yk.y * Yk.y;
-                aT = cadd(aT, cmul(Yk, Dh));
-                aP = cadd(aP, cmul(Yk, Qh));
+        if (mm + W < M) loadx(col + (size_t)W * Lg);
+        wave_lds_sync();
+        const float2 *Y = team_fft<false>(ya, yb, P.plan4, tw, lane, 64, WaveSync{}, 1);
+        for (int k = lane; k < Ph; k += 64) gp(y3s)[(size_t)mm * Ph + k] = Y[k];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            float4 D, Q;
+            if constexpr (kPf) { D = dq[i]; Q = qq[i]; } else { loadq(i, D, Q); }
+            const int t = lane + 64 * i;
+            if (t < nq) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int k = 2 * t + h;
+                    if (k < H) {
+                        const float2 Yk = y3h(Y, k);
+                        const float2 Dk = h ? make_float2(D.z, D.w) : make_float2(D.x, D.y);
+                        const float2 Qk = h ? make_float2(Q.z, Q.w) : make_float2(Q.x, Q.y);
+                        const int j = 2 * i + h;
+                        aS[j] += Yk.x * Yk.x + Yk.y * Yk.y;
+                        aT[j] = cadd(aT[j], cmul(Yk, Dk));
+                        aP[j] = cadd(aP[j], cmul(Yk, Qk));
+                    }
+                }
             }
-            accS[k] = aS; accT[k] = aT; accP[k] = aP;
         }
-        __syncthreads();
+        wave_lds_sync();
     }
     if (mat)
         for (int l = tid; l < Lg; l += THR) gp(P.h)[ro + l] = 0.f;
+    // wave partials summed in wave order
+    const int nw = M < W ? M : W;
+    for (int ww = 0; ww < nw; ++ww) {
+        if (w == ww) {
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int k = 2 * (lane + 64 * i) + h, j = 2 * i + h;
+                    if (k < H) {
+                        if (ww == 0) { accS[k] = aS[j]; accT[k] = aT[j]; accP[k] = aP[j]; }
+                        else { accS[k] += aS[j]; accT[k] = cadd(accT[k], aT[j]); accP[k] = cadd(accP[k], aP[j]); }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (nw == 0) {   // no columns: sums of nothing
+        for (int k = tid; k < H; k += THR) { accS[k] = 0.f; accT[k] = make_float2(0.f, 0.f); accP[k] = make_float2(0.f, 0.f); }
+        __syncthreads();
+    }
 
     // ---- per-sequence spectral step: Td, G2/g2 (update_g2!, train.jl:68-71), kappa
+    float *tdb = reinterpret_cast<float *>(fb + LgMax);    // Lg floats over wave 0's ya, yb
+    const float scale = 1.0f / (float)Lg;
     for (int k = tid; k < Lg; k += THR) fa[k] = (k < H) ? accT[k] : cconj(accT[Lg - k]);
     __syncthreads();
-    float2 *R2 = team_fft<true>(fa, fb, P.plan, tw, tid, THR, BlockSync{}, 1);
-    float *tdb = reinterpret_cast<float *>(ys);      // Lg floats fit in ys (B * Lg/4 complex, B >= 2)
     {
-        const float scale = 1.0f / (float)Lg;
+        const float2 *R2 = team_fft<true>(fa, fb, P.plan, tw, tid, THR, BlockSync{}, 1);
         for (int l = tid; l < Lg; l += THR) tdb[l] = R2[l].x * scale;
     }
     __syncthreads();
@@ -782,7 +834,7 @@ __global__ __launch_bounds__(THR) void k_dict(const CdlDev *__restrict__ tab, co
         // F(G2 - g2) of the previous dictionary iteration (sherman_CDL_RHS!, cdl_kernels.jl:8)
         for (int l = tid; l < Lg; l += THR) fa[l] = make_float2(gp(P.G2)[ro + l] - gp(P.g2)[ro + l], 0.f);
         __syncthreads();
-        float2 *R = team_fft<false>(fa, fb, P.plan, tw, tid, THR, BlockSync{}, 1);
+        const float2 *R = team_fft<false>(fa, fb, P.plan, tw, tid, THR, BlockSync{}, 1);
         const float c1 = P.sh_cdl;
         for (int k = tid; k < H; k += THR) {
             const float2 Gk = R[k];
@@ -803,86 +855,102 @@ __global__ __launch_bounds__(THR) void k_dict(const CdlDev *__restrict__ tab, co
     }
     __syncthreads();
 
-    // ---- pass 2: d_m = q_m + F^-1[conj(Y3h_m) kappa]; g1 += d - G1; g3 += d - G3
-    const float scale = 1.0f / (float)Lg;
-    for (int m0 = 0; m0 < M; m0 += B) {
-        const int nb = (M - m0) < B ? (M - m0) : B;
-        const int npair = (nb + 1) / 2;
-        for (int idx = tid; idx < nb * Ph; idx += THR) {
-            const int bb = idx / Ph;
-            const int k = idx - bb * Ph;
-            const float2 v = gp(y3s)[(size_t)(m0 + bb) * Ph + k];
-            ys[bb * Pq + k] = v;
-            if (k > 0 && Pq - k >= Ph) ys[bb * Pq + Pq - k] = cconj(v);
-        }
-        __syncthreads();
-        for (int idx = tid; idx < npair * Lg; idx += THR) {
-            int k, kq;
-            const int pp = fdivmod(idx, Lg, rLg, k);
-            (void)fdivmod(k, Pq, rPq, kq);
-            const float2 kap = (k < H) ? accP[k] : cconj(accP[Lg - k]);
-            const float2 CA = cmul(cconj(ys[(2 * pp) * Pq + kq]), kap);
-            const float2 CB = (2 * pp + 1 < nb) ? cmul(cconj(ys[(2 * pp + 1) * Pq + kq]), kap) : make_float2(0.f, 0.f);
-            fa[idx] = make_float2(CA.x - CB.y, CA.y + CB.x);
-        }
-        __syncthreads();
-        float2 *R = team_fft<true>(fa, fb, P.plan, tw, tid, THR, BlockSync{}, npair);
-        for (int idx = tid; idx < nb * nslot; idx += THR) {
-            int s;
-            const int bb = fdivmod(idx, nslot, rslot, s), m = m0 + bb;
-            const size_t col = ((size_t)n * M + m) * Lg;
-            const float4 G1v = ld4(G1 + (size_t)m * Lg, s);
-            const float4 G3v = ld4(P.G3 + (size_t)m * Lg, s);
-            const float4 a1 = ld4(P.g1 + col, s);
-            const float4 c3 = ld4(P.g3 + col, s);
-            const float2 *Rp = R + (bb >> 1) * Lg + 4 * s;
-            float dn[4], g1n[4], g3n[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float qv = (f4(G1v, q) - f4(a1, q)) + (f4(G3v, q) - f4(c3, q));
-                const float cv = (bb & 1) ? Rp[q].y : Rp[q].x;
-                dn[q] = qv + cv * scale;
-                g1n[q] = f4(a1, q) + dn[q] - f4(G1v, q);       // train.jl:70
-                g3n[q] = f4(c3, q) + dn[q] - f4(G3v, q);       // train.jl:72
+    // ---- pass 2: dh = Qh + conj(Y3h) kappa; g1h += dh - G1h; g3h += dh - G3h
+    for (int m = w; m < M; m += W) {
+        const int mm = __builtin_amdgcn_readfirstlane(m);
+        const size_t colh = ((size_t)n * M + mm) * Hs, fcol = (size_t)mm * Hs;
+        float *dcol = reinterpret_cast<float *>(P.dh + colh);
+        float *a1col = reinterpret_cast<float *>(P.g1h + colh);
+        float *a3col = reinterpret_cast<float *>(P.g3h + colh);
+        float4 a1v[kPf ? NQ : 1], a3v[kPf ? NQ : 1], f1v[kPf ? NQ : 1], f3v[kPf ? NQ : 1];
+        auto load2 = [&](int i, float4 &A1, float4 &A3, float4 &F1, float4 &F3) {
+            const int t = lane + 64 * i;
+            if (t < nq) {
+                A1 = ld4(a1col, t);
+                A3 = ld4(a3col, t);
+                F1 = ld4(G1h + 2 * fcol, t);
+                F3 = ld4(G3h + 2 * fcol, t);
             }
-            st4(P.d + col, s, mk4(dn));
-            st4(P.g1 + col, s, mk4(g1n));
-            st4(P.g3 + col, s, mk4(g3n));
+        };
+        if constexpr (kPf) {
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) load2(i, a1v[i], a3v[i], f1v[i], f3v[i]);
         }
-        __syncthreads();
+        for (int k = lane; k < Ph; k += 64) ya[k] = gp(y3s)[(size_t)mm * Ph + k];
+        wave_lds_sync();
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            float4 A1, A3, F1, F3;
+            if constexpr (kPf) { A1 = a1v[i]; A3 = a3v[i]; F1 = f1v[i]; F3 = f3v[i]; } else { load2(i, A1, A3, F1, F3); }
+            const int t = lane + 64 * i;
+            if (t < nq) {
+                float dn[4], g1n[4], g3n[4];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int k = 2 * t + h;
+                    const float q0 = (f4(F1, 2 * h) - f4(A1, 2 * h)) + (f4(F3, 2 * h) - f4(A3, 2 * h));
+                    const float q1 = (f4(F1, 2 * h + 1) - f4(A1, 2 * h + 1)) + (f4(F3, 2 * h + 1) - f4(A3, 2 * h + 1));
+                    float2 dk = make_float2(0.f, 0.f);
+                    if (k < H) {
+                        const float2 c = cmul(cconj(y3h(ya, k)), accP[k]);
+                        dk = make_float2(q0 + c.x, q1 + c.y);
+                    }
+                    dn[2 * h] = dk.x;
+                    dn[2 * h + 1] = dk.y;
+                    g1n[2 * h] = f4(A1, 2 * h) + dk.x - f4(F1, 2 * h);               // train.jl:70
+                    g1n[2 * h + 1] = f4(A1, 2 * h + 1) + dk.y - f4(F1, 2 * h + 1);
+                    g3n[2 * h] = f4(A3, 2 * h) + dk.x - f4(F3, 2 * h);               // train.jl:72
+                    g3n[2 * h + 1] = f4(A3, 2 * h + 1) + dk.y - f4(F3, 2 * h + 1);
+                }
+                st4(dcol, t, mk4(dn));
+                st4(a1col, t, mk4(g1n));
+                st4(a3col, t, mk4(g3n));
+            }
+        }
+        wave_lds_sync();
     }
 }
 
-// Partial sums over a chunk of sequences of (d + g1), (d + g3) and d, per (l, m)
-// (train.jl:76,83, train.jl:95), in double.  Blocks of a problem: (tile, chunk).
+// Partial sums over a chunk of sequences of (dh + g1h), (dh + g3h) and dh, per (bin, m)
+// (train.jl:76,83, train.jl:95: the means of d + g1, d + g3 and d, taken on the spectra), in
+// double, real and imaginary parts.  Blocks of a problem: (tile, chunk).
 __global__ __launch_bounds__(kThreads) void k_dict_reduce(const CdlDev *__restrict__ tab, const int *__restrict__ starts,
                                                           int np) {
     int b;
     const CdlDev &P = tab[find_problem(starts, np, blockIdx.x, b)];
-    const int LM = P.Lg * P.M;
-    const int tiles = (LM + kThreads - 1) / kThreads;
+    const int HM = half_bins(P.Lg) * P.M;
+    const int tiles = (HM + kThreads - 1) / kThreads;
     const int c = b / tiles, tile = b - c * tiles;
     const int i = tile * kThreads + threadIdx.x;
-    if (i >= LM) return;
+    if (i >= HM) return;
     const int N = P.N, nch = P.nch;
     const int n0 = (int)((long)c * N / nch), n1 = (int)((long)(c + 1) * N / nch);
-    double a1 = 0.0, a3 = 0.0, ad = 0.0;
+    double a1x = 0.0, a1y = 0.0, a3x = 0.0, a3y = 0.0, adx = 0.0, ady = 0.0;
     for (int n = n0; n < n1; ++n) {
-        const size_t e = (size_t)n * LM + i;
-        const float dv = gp(P.d)[e];
-        a1 += (double)(dv + gp(P.g1)[e]);
-        a3 += (double)(dv + gp(P.g3)[e]);
-        ad += (double)dv;
+        const size_t e = (size_t)n * HM + i;
+        const float2 dv = gp(P.dh)[e], v1 = gp(P.g1h)[e], v3 = gp(P.g3h)[e];
+        a1x += (double)(dv.x + v1.x);
+        a1y += (double)(dv.y + v1.y);
+        a3x += (double)(dv.x + v3.x);
+        a3y += (double)(dv.y + v3.y);
+        adx += (double)dv.x;
+        ady += (double)dv.y;
     }
-    gp(P.dpart)[((size_t)c * 3 + 0) * LM + i] = a1;
-    gp(P.dpart)[((size_t)c * 3 + 1) * LM + i] = a3;
-    gp(P.dpart)[((size_t)c * 3 + 2) * LM + i] = ad;
+    double *o = gp(P.dpart) + (size_t)c * 6 * HM + i;
+    o[0 * (size_t)HM] = a1x;
+    o[1 * (size_t)HM] = a1y;
+    o[2 * (size_t)HM] = a3x;
+    o[3 * (size_t)HM] = a3y;
+    o[4 * (size_t)HM] = adx;
+    o[5 * (size_t)HM] = ady;
 }
 
-// Column m: v1 = mean(d+g1), v2 = mean(d+g3); G3 = projection2box!(v2) (cdl_kernels.jl:37-51);
-// G1new = prox_simplex_zeros!(v1) (:53-66, FP64 combination, quirk group 4*floor(l/4)) into
-// the non-current G1 buffer; meand = mean(d) (D_best candidate, train.jl:95);
-// gsq = |F(G1new)|^2.
+// Column m: the mean spectra of (d+g1), (d+g3) and d back to the spatial domain (one packed
+// inverse transform for the first two, one for the third); v1 = mean(d+g1), v2 = mean(d+g3);
+// G3 = projection2box!(v2) (cdl_kernels.jl:37-51); G1new = prox_simplex_zeros!(v1) (:53-66,
+// FP64 combination, quirk group 4*floor(l/4)) into the non-current G1 buffer; meand =
+// mean(d) (D_best candidate, train.jl:95); gsq = |F(G1new)|^2 and the half spectra of G1new
+// and G3.
 __global__ __launch_bounds__(kThreads) void k_dict_finalize(const CdlDev *__restrict__ tab, const int *__restrict__ starts,
                                                             int np, int LgMax) {
     extern __shared__ float4 smem4[];
@@ -892,29 +960,55 @@ __global__ __launch_bounds__(kThreads) void k_dict_finalize(const CdlDev *__rest
     float2 *tw = reinterpret_cast<float2 *>(v1 + ((LgMax + 3) & ~3));
     float2 *fa = tw + LgMax;
     float2 *fb = fa + LgMax;
-    const int tid = threadIdx.x, Lg = P.Lg;
-    const int LM = Lg * P.M;
+    const int tid = threadIdx.x, Lg = P.Lg, H = Lg / 2 + 1, Hs = half_bins(Lg);
+    const int HM = Hs * P.M;
     const int flm = gp(P.fl)[m];
     const double invN = 1.0 / (double)P.N;
-    float *G1new = P.G1b[(P.sc->g1cur & 1) ^ 1];
-    for (int l = tid; l < Lg; l += kThreads) tw[l] = gp(P.tw)[l];
-    for (int l = tid; l < Lg; l += kThreads) {
-        const int i = m * Lg + l;
-        double s1 = 0.0, s3 = 0.0, sd = 0.0;
+    const float scale = 1.0f / (float)Lg;
+    const int nxt = (P.sc->g1cur & 1) ^ 1;
+    float *G1new = P.G1b[nxt];
+    float2 *G1hnew = P.G1hb[nxt];
+    // mean spectrum j (0: d+g1, 1: d+g3, 2: d) at bin k < H
+    auto mean = [&](int j, int k) {
+        const size_t i = (size_t)m * Hs + k;
+        double sx = 0.0, sy = 0.0;
         for (int c = 0; c < P.nch; ++c) {
-            s1 += gp(P.dpart)[((size_t)c * 3 + 0) * LM + i];
-            s3 += gp(P.dpart)[((size_t)c * 3 + 1) * LM + i];
-            sd += gp(P.dpart)[((size_t)c * 3 + 2) * LM + i];
+            sx += gp(P.dpart)[((size_t)c * 6 + 2 * j) * HM + i];
+            sy += gp(P.dpart)[((size_t)c * 6 + 2 * j + 1) * HM + i];
         }
-        v1[l] = (float)(s1 * invN);
-        const float v2 = (float)(s3 * invN);
-        const bool sup = l < flm;
-        float g3;
-        if (v2 >= 1.f) g3 = sup ? 1.f : 0.f;
-        else if (v2 <= 0.f) g3 = 0.f;
-        else g3 = sup ? v2 : 0.f;
-        gp(P.G3)[i] = g3;
-        gp(P.meand)[i] = (float)(sd * invN);
+        return make_float2((float)(sx * invN), (float)(sy * invN));
+    };
+    for (int l = tid; l < Lg; l += kThreads) tw[l] = gp(P.tw)[l];
+    // v1 + i v2 = F^-1(V1 + i V2) for Hermitian V1, V2
+    for (int k = tid; k < H; k += kThreads) {
+        const float2 a = mean(0, k), b = mean(1, k);
+        fa[k] = make_float2(a.x - b.y, a.y + b.x);
+        if (k > 0 && Lg - k >= H) fa[Lg - k] = make_float2(a.x + b.y, b.x - a.y);
+    }
+    __syncthreads();
+    {
+        const float2 *R = team_fft<true>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{});
+        for (int l = tid; l < Lg; l += kThreads) {
+            v1[l] = R[l].x * scale;
+            const float v2 = R[l].y * scale;
+            const bool sup = l < flm;
+            float g3;
+            if (v2 >= 1.f) g3 = sup ? 1.f : 0.f;
+            else if (v2 <= 0.f) g3 = 0.f;
+            else g3 = sup ? v2 : 0.f;
+            gp(P.G3)[(size_t)m * Lg + l] = g3;
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < H; k += kThreads) {
+        const float2 a = mean(2, k);
+        fa[k] = a;
+        if (k > 0 && Lg - k >= H) fa[Lg - k] = cconj(a);
+    }
+    __syncthreads();
+    {
+        const float2 *R = team_fft<true>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{});
+        for (int l = tid; l < Lg; l += kThreads) gp(P.meand)[(size_t)m * Lg + l] = R[l].x * scale;
     }
     __syncthreads();
     for (int l = tid; l < Lg; l += kThreads) {
@@ -928,11 +1022,22 @@ __global__ __launch_bounds__(kThreads) void k_dict_finalize(const CdlDev *__rest
         fa[l] = make_float2(g, 0.f);
     }
     __syncthreads();
-    float2 *C = team_fft<false>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{});
-    for (int k = tid; k < Lg; k += kThreads) gp(P.gsq)[(size_t)m * Lg + k] = C[k].x * C[k].x + C[k].y * C[k].y;
+    {
+        const float2 *C = team_fft<false>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{});
+        for (int k = tid; k < Lg; k += kThreads) gp(P.gsq)[(size_t)m * Lg + k] = C[k].x * C[k].x + C[k].y * C[k].y;
+        for (int k = tid; k < Hs; k += kThreads) gp(G1hnew)[(size_t)m * Hs + k] = k < H ? C[k] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    for (int l = tid; l < Lg; l += kThreads) fa[l] = make_float2(gp(P.G3)[(size_t)m * Lg + l], 0.f);
+    __syncthreads();
+    {
+        const float2 *C = team_fft<false>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{});
+        for (int k = tid; k < Hs; k += kThreads) gp(P.G3h)[(size_t)m * Hs + k] = k < H ? C[k] : make_float2(0.f, 0.f);
+    }
 }
 
-// |F(gp(G1)[:, m])|^2 for the current filters (constructor and reset).
+// |F(gp(G1)[:, m])|^2 and the half spectra of the current G1 and of G3 (constructor and
+// reset).
 __global__ __launch_bounds__(kThreads) void k_g1_spectrum(const CdlDev *__restrict__ tab, const int *__restrict__ starts,
                                                           int np, int LgMax) {
     extern __shared__ float4 smem4[];
@@ -941,15 +1046,49 @@ __global__ __launch_bounds__(kThreads) void k_g1_spectrum(const CdlDev *__restri
     float2 *tw = reinterpret_cast<float2 *>(smem4);
     float2 *fa = tw + LgMax;
     float2 *fb = fa + LgMax;
-    const int tid = threadIdx.x, Lg = P.Lg;
-    const float *G1 = g1cur_of(P);
+    const int tid = threadIdx.x, Lg = P.Lg, H = Lg / 2 + 1, Hs = half_bins(Lg);
+    const int cur = P.sc->g1cur & 1;
+    const float *G1 = P.G1b[cur];
     for (int l = tid; l < Lg; l += kThreads) {
         tw[l] = gp(P.tw)[l];
         fa[l] = make_float2(gp(G1)[(size_t)m * Lg + l], 0.f);
     }
     __syncthreads();
-    float2 *C = team_fft<false>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{});
-    for (int k = tid; k < Lg; k += kThreads) gp(P.gsq)[(size_t)m * Lg + k] = C[k].x * C[k].x + C[k].y * C[k].y;
+    {
+        const float2 *C = team_fft<false>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{});
+        for (int k = tid; k < Lg; k += kThreads) gp(P.gsq)[(size_t)m * Lg + k] = C[k].x * C[k].x + C[k].y * C[k].y;
+        for (int k = tid; k < Hs; k += kThreads) gp(P.G1hb[cur])[(size_t)m * Hs + k] = k < H ? C[k] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    for (int l = tid; l < Lg; l += kThreads) fa[l] = make_float2(gp(P.G3)[(size_t)m * Lg + l], 0.f);
+    __syncthreads();
+    {
+        const float2 *C = team_fft<false>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{});
+        for (int k = tid; k < Hs; k += kThreads) gp(P.G3h)[(size_t)m * Hs + k] = k < H ? C[k] : make_float2(0.f, 0.f);
+    }
+}
+
+// One column (n, m) of a half-spectrum array back to the spatial domain (field reads of d,
+// g1, g3): out[(n M + m) Lg + l] = F^-1 of the Hermitian extension.  LDS: twiddles | a | b.
+__global__ __launch_bounds__(kThreads) void k_half_to_spatial(const CdlDev *__restrict__ tab, const float2 *__restrict__ src,
+                                                              float *__restrict__ out) {
+    extern __shared__ float4 smem4[];
+    const CdlDev &P = tab[0];
+    const int Lg = P.Lg, H = Lg / 2 + 1, Hs = half_bins(Lg), tid = threadIdx.x;
+    const size_t c = blockIdx.x;
+    float2 *tw = reinterpret_cast<float2 *>(smem4);
+    float2 *fa = tw + Lg;
+    float2 *fb = fa + Lg;
+    for (int l = tid; l < Lg; l += kThreads) tw[l] = gp(P.tw)[l];
+    for (int k = tid; k < H; k += kThreads) {
+        const float2 v = gp(src)[c * Hs + k];
+        fa[k] = v;
+        if (k > 0 && Lg - k >= H) fa[Lg - k] = cconj(v);
+    }
+    __syncthreads();
+    const float2 *R = team_fft<true>(fa, fb, P.plan, tw, tid, kThreads, BlockSync{});
+    const float scale = 1.0f / (float)Lg;
+    for (int l = tid; l < Lg; l += kThreads) gp(out)[c * Lg + l] = R[l].x * scale;
 }
 
 // s_k = sum_m |G1h|^2 (FP32, m ascending; train.jl:18-19), c_k = s_k + sherman_const_csc.

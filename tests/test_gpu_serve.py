@@ -124,18 +124,20 @@ def test_admission_order_policy(policy):
     srv = cdlm.Server(csc_iter=CSC, cdl_iter=CDL_IT, max_active=1, policy=policy)
     a = srv.submit(_data(301, 60, 200), [16] * 8, RHO, SIGMA, LAM, 400, seed=1)
     time.sleep(0.01)
-    big = srv.submit(_data(302, 60, 200), [16] * 8, RHO, SIGMA, LAM, 20, seed=2)
-    small = srv.submit(_data(303, 10, 20), [8] * 2, RHO, SIGMA, LAM, 2, seed=3)
+    db, ds = _data(302, 60, 200), _data(303, 10, 20)
+    t0 = time.perf_counter()
+    big = srv.submit(db, [16] * 8, RHO, SIGMA, LAM, 20, seed=2)
+    small = srv.submit(ds, [8] * 2, RHO, SIGMA, LAM, 2, seed=3)
+    gap = time.perf_counter() - t0          # >= small's submit time - big's
     assert srv.poll(a) == cdlm.serve.RUNNING
-    first = None
-    while first is None:
-        sb, ss = srv.poll(big), srv.poll(small)
-        if sb != cdlm.serve.QUEUED:
-            first = "big"
-        elif ss != cdlm.serve.QUEUED:
-            first = "small"
-        time.sleep(0.0005)
-    assert first == ("big" if policy == "fifo" else "small"), (policy, first)
     for j in (a, big, small):
         assert srv.wait(j, 120) == cdlm.serve.DONE
+    rb, rs = srv.result(big), srv.result(small)
     srv.close()
+    # one job resident at a time, so the order shows in the server's own clock: a job that ran
+    # first was done before the other was admitted (latency and queue time both count from the
+    # job's own submission, and big was submitted at most `gap` before small)
+    if policy == "fifo":
+        assert rb["latency_s"] <= rs["queue_s"] + gap, (rb, rs, gap)
+    else:
+        assert rs["latency_s"] <= rb["queue_s"], (rb, rs)
