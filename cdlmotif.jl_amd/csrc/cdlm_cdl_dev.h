@@ -18,11 +18,14 @@
 //     next kernel that reads X.  g is always the current G1: before G1 changes (the
 //     dictionary step) X is written back materialised (Xt = X, h = 0) -- by update_best's
 //     pass, which rebuilds all of X anyway, or else by the dictionary step itself.
-//   * the dictionary step keeps d spatial; its solve is
-//       d_m = q_m + F^-1[conj(Y3h_m) kappa],   kappa = (c1 F(G2-g2) - sum_m Y3h_m Q_m)/(|Y3h|^2 + c1)
-//     with q_m = (G1_m - g1_m) + (G3_m - g3_m) (c1 = sherman_const_cdl; no 1/c1, as
-//     sherman_CDL_combine).  Y3 = Pi4(X + y3) is nonzero on every 4th sample only, so its
-//     spectrum is a length Lg/4 DFT repeated 4 times.
+//   * the dictionary step holds d, g1 and g3 as half spectra (dh, g1h, g3h, as the reference
+//     keeps dh, g1h, g3h); its solve is
+//       dh_m = Qh_m + conj(Y3h_m) kappa,   kappa = (c1 F(G2-g2) - sum_m Y3h_m Qh_m)/(|Y3h|^2 + c1)
+//     with Qh_m = (G1h_m - g1h_m) + (G3h_m - g3h_m) (c1 = sherman_const_cdl; no 1/c1, as
+//     sherman_CDL_combine), and g1h, g3h are updated in the same domain.  Y3 = Pi4(X + y3) is
+//     nonzero on every 4th sample only, so its spectrum is a length Lg/4 DFT repeated 4 times:
+//     the only per-column transform of the step.  The means over n of d + g1, d + g3 and d
+//     are taken on the spectra and transformed back once per filter.
 // All quantities are exact rewrites of the reference iteration; only rounding differs.
 //
 // Launch model.  Every kernel reads its problem from a device table of descriptors
@@ -33,7 +36,8 @@
 // wrap modulo the problem's own Lg.
 //
 // HBM layout: the reference's column-major (Lg, M, N) with l fastest; one work-group owns
-// one sequence n and streams its contiguous (Lg x M) slab.  y3 is stored without its
+// one sequence n and streams its contiguous (Lg x M) slab.  dh, g1h, g3h are (Hs, M, N)
+// complex with Hs = Lg/2 + 1 rounded up to even (bins 0..Lg/2 and a zero pad).  y3 is stored without its
 // stride-4 support samples: there y3 + X - Pi4(X + y3) = (y3 + X) - (X + y3) = 0 exactly
 // (IEEE addition commutes) from the first iteration on, and y3 starts at 0, so the
 // support samples are identically zero and only l % 4 != 0 is kept, 3 floats per slot.

@@ -241,8 +241,8 @@ cdlm_status cdlm_server_stats(cdlm_server *srv, cdlm_server_stats_t *out);
  * own algorithms (CDL_setup.jl:158-163); here the choice is explicit and measured.
  * fft_order / fft4_order: stage order of the length-Lg and length-Lg/4 transforms
  * (0 = radix-4/2 stages first, 1 = odd primes first; the only choices that change
- * rounding); dict_batch: dictionary columns per pass (2 or 4 with 256-thread work-groups,
- * 8 with 1024-thread work-groups and one sequence per CU; 0 = by occupancy);
+ * rounding); dict_batch: dictionary waves per work-group, one column in flight per wave
+ * (2, 4 or 8; 0 = by occupancy; it changes the order of the sum over waves, hence rounding);
  * csc_slots: CSC float4 slots per lane (4 or 8; 0 = the fewest that cover Lg);
  * csc_waves: CSC waves per work-group (4: three sequences per CU with register prefetch;
  * 8 or 16: one sequence per CU, which keeps the resident sequences' state in the last-level
