@@ -127,9 +127,8 @@ def test_footprint_is_what_a_problem_holds():
 
 
 def test_tuned_launch_choices_keep_parity():
-    """Every stage order, dictionary batch (2 and 4 columns per pass, 8 in 1024-thread
-    work-groups) and CSC shape (slots per lane, waves per work-group) meets the oracle
-    tolerance; the autotuner's
+    """Every stage order, dictionary work-group width (2, 4 or 8 waves, one column each) and
+    CSC shape (slots per lane, waves per work-group) meets the oracle tolerance; the autotuner's
     choice is recorded and used by problems created afterwards; a batch of problems with
     different tunings still equals each problem alone."""
     import oracle as O

@@ -196,11 +196,14 @@ def test_x_materialisation_paths_agree():
         np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
 
 
-@pytest.mark.parametrize("shape", [(1, 2, 1, [4]), (3, 1, 2, [4, 4]), (2, 5, 3, [8, 4, 12]), (5, 3, 17, [8] * 17)])
+@pytest.mark.parametrize("shape", [(1, 2, 1, [4]), (3, 1, 2, [4, 4]), (2, 5, 3, [8, 4, 12]), (5, 3, 17, [8] * 17),
+                                   (2, 252, 2, [8, 8]), (3, 260, 3, [8, 12, 8]), (1, 502, 2, [20, 20])])
 def test_tiny_shapes_vs_oracle(shape):
     """Edge shapes: one sequence, one base, one filter, fewer filters than the CSC kernel has
-    waves (M < 16: waves without columns), an odd filter count across the waves; short
-    schedule, every launch choice at its default."""
+    waves (M < 16: waves without columns), an odd filter count across the waves; padded
+    lengths 1024 (the first with more than 256 float4 slots of half spectrum, the dictionary
+    kernel's wider variant), 1064 and 2048 (the largest); short schedule, every launch choice
+    at its default."""
     N, L, M, fl = shape
     s, D0, X0, _ = _synth(40 + N + M, N, L, M, fl)
     g, o = _both(s, D0, X0, fl, 1.0, 100.0, 0.05)
