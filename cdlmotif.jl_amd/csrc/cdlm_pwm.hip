@@ -23,13 +23,21 @@ namespace {
 constexpr int kPwmThreads = 256;
 // Hit-list kernel shape: G tiles of 16 strand-PWMs per wave (B operands in registers) and CH
 // chunks of 16 windows per pass sharing them, so a wave runs G x CH independent MFMA chains.
-// Measured at 100000 x 500 bases, 512 PWMs of length 8-24: (G, CH) = (4, 1) 94.2 ms,
-// (2, 1) 89.3, (2, 2) 86.7, (4, 2) 87.5, (2, 4) 92.1, (1, 8) 89.1, (1, 4) 84.6.
-constexpr int kPwmG = 1, kPwmCH = 4;
+// Measured at 100000 x 500 bases, 512 PWMs of length 8-24, A operands built from the base
+// bytes: (G, CH) = (4, 1) 94.2 ms, (2, 1) 89.3, (2, 2) 86.7, (4, 2) 87.5, (2, 4) 92.1, (1, 8) 89.1,
+// (1, 4) 84.6; read as one-hot floats from LDS: (1, 2) 70.6, (1, 4) 71.4, (1, 8) 74.3, (2, 2) 79.0,
+// (2, 4) 84.3.
+constexpr int kPwmG = 1, kPwmCH = 2;
+// LDS per wave position: the sequence expanded to one-hot floats (16 bytes; each A operand
+// is one LDS load) while 4 waves' sequences fit 64 KB, else the base bytes (A operands are
+// compared from them).
+constexpr size_t kPwmLdsMax = 64 * 1024;
 constexpr int kMaxPwmLen = 64;
 
 // ---------------------------------------------------------------------------- hit list on MFMA
-// One-hot im2col x PWM contraction on v_mfma_f32_16x16x4_f32.  A tile of 16 windows x 16
+// One-hot im2col x PWM contraction on v_mfma_f32_16x16x4_f32.  OH: the wave's sequence is held
+// in LDS as one-hot floats (position p, base a at float 4p + a), so each A operand is a
+// single load; otherwise as base bytes, compared per operand.  A tile of 16 windows x 16
 // strand-PWMs accumulates one motif column j per MFMA: A[i][a] = [base(w0+i+j) == a],
 // B[a][c] = pwm_c[a][j].  Each K-block holds exactly one non-zero product per output (the
 // base's row), so every MFMA step is the single FP32 addition acc + pwm[b][j] of the
@@ -56,7 +64,7 @@ __host__ __device__ constexpr int pwm_lpad(int L, int MAXJ, int CH) {
     return ((L + 16 * CH - 1) / (16 * CH)) * (16 * CH) + MAXJ + 16;
 }
 
-template <int G, int MAXJ, int CH>
+template <int G, int MAXJ, int CH, bool OH>
 __global__ __launch_bounds__(256) void k_pwm_hits_mfma(const uint8_t *__restrict__ bases, int L, long N, PwmTiles P,
                                                        int *hk, long long *hn, int *hp, signed char *hs, long long cap,
                                                        unsigned long long *count) {
@@ -64,6 +72,8 @@ __global__ __launch_bounds__(256) void k_pwm_hits_mfma(const uint8_t *__restrict
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int Lpad = pwm_lpad(L, MAXJ, CH);
     uint8_t *sq = seqs_lds + w * Lpad;
+    float4 *oh = reinterpret_cast<float4 *>(seqs_lds) + (size_t)w * Lpad;
+    const float *ohf = reinterpret_cast<const float *>(oh);
     const int g = blockIdx.x;
     const int t0 = g * G;
     float Bv[G][MAXJ];
@@ -89,7 +99,14 @@ __global__ __launch_bounds__(256) void k_pwm_hits_mfma(const uint8_t *__restrict
     const long nstride = (long)gridDim.y * 4;
     for (long n = (long)blockIdx.y * 4 + w; n < N; n += nstride) {
         const uint8_t *src = bases + n * (long)L;
-        for (int i = lane; i < Lpad; i += 64) sq[i] = i < L ? src[i] : (uint8_t)4;
+        if constexpr (OH) {
+            for (int i = lane; i < Lpad; i += 64) {
+                const int b = i < L ? src[i] : 4;
+                oh[i] = make_float4(b == 0 ? 1.f : 0.f, b == 1 ? 1.f : 0.f, b == 2 ? 1.f : 0.f, b == 3 ? 1.f : 0.f);
+            }
+        } else {
+            for (int i = lane; i < Lpad; i += 64) sq[i] = i < L ? src[i] : (uint8_t)4;
+        }
         wave_lds_sync();
         for (int w0 = 0; w0 < L; w0 += 16 * CH) {
             // CH chunks of 16 windows share the B operands: G x CH independent accumulators
@@ -105,7 +122,8 @@ __global__ __launch_bounds__(256) void k_pwm_hits_mfma(const uint8_t *__restrict
     case JS:                                                                                         \
         _Pragma("unroll") for (int j = 0; j < JS; ++j) {                                             \
             _Pragma("unroll") for (int c = 0; c < CH; ++c) {                                         \
-                const float a = (sq[w0 + 16 * c + arow + j] == abase) ? 1.0f : 0.0f;                \
+                const int p = w0 + 16 * c + arow + j;                                                \
+                const float a = OH ? ohf[4 * p + abase] : ((sq[p] == abase) ? 1.0f : 0.0f);          \
                 _Pragma("unroll") for (int gi = 0; gi < G; ++gi)                                     \
                     acc[c][gi] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bv[gi][j], acc[c][gi], 0, 0, 0); \
             }                                                                                        \
@@ -414,7 +432,8 @@ cdlm_status launch_hits(cdlm_ctx *ctx, const uint8_t *d_bases, int L, long N, co
         const int MAXJ = maxlen <= 32 ? 32 : 64;
         P.groups = (ht.T + G - 1) / G;
         const int Lpad = pwm_lpad(L, MAXJ, kPwmCH);
-        const size_t lds = 4 * (size_t)Lpad;
+        const bool oh = 4 * (size_t)Lpad * sizeof(float4) <= kPwmLdsMax;
+        const size_t lds = 4 * (size_t)Lpad * (oh ? sizeof(float4) : 1);
         long yb = (N + 3) / 4;
         const long target = (long)ctx->num_cu * 16 / (P.groups > 0 ? P.groups : 1) + 1;
         if (yb > target) yb = target;
@@ -426,13 +445,15 @@ cdlm_status launch_hits(cdlm_ctx *ctx, const uint8_t *d_bases, int L, long N, co
             e0 = e1 = nullptr;
         }
         if (e0) (void)hipEventRecord(e0, st);
-        if (lds > 64 * 1024) s = cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "sequence length %d too long for the scan", L);
-        else if (maxlen <= 32)
-            hipLaunchKernelGGL((k_pwm_hits_mfma<kPwmG, 32, kPwmCH>), dim3(P.groups, (unsigned)yb), dim3(256), lds, st,
-                               d_bases, L, N, P, hk, hn, hp, hs, cap, d_count);
-        else
-            hipLaunchKernelGGL((k_pwm_hits_mfma<kPwmG, 64, kPwmCH>), dim3(P.groups, (unsigned)yb), dim3(256), lds, st,
-                               d_bases, L, N, P, hk, hn, hp, hs, cap, d_count);
+#define L_HITS(MJ, OHV)                                                                                         \
+    hipLaunchKernelGGL((k_pwm_hits_mfma<kPwmG, MJ, kPwmCH, OHV>), dim3(P.groups, (unsigned)yb), dim3(256), lds, st, \
+                       d_bases, L, N, P, hk, hn, hp, hs, cap, d_count)
+        if (lds > kPwmLdsMax) s = cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "sequence length %d too long for the scan", L);
+        else if (maxlen <= 32 && oh) L_HITS(32, true);
+        else if (maxlen <= 32) L_HITS(32, false);
+        else if (oh) L_HITS(64, true);
+        else L_HITS(64, false);
+#undef L_HITS
         hipError_t e = hipGetLastError();
         if (s == CDLM_OK && e != hipSuccess) s = cdlm_fail(ctx, CDLM_ERR_HIP, "k_pwm_hits_mfma: %s", hipGetErrorString(e));
         if (e1) (void)hipEventRecord(e1, st);

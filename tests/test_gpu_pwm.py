@@ -140,10 +140,14 @@ def test_mfma_layout_and_rounding_probe():
             assert d[4 * l + r].view(np.uint32) == exp.view(np.uint32), (l, r)
 
 
-@pytest.mark.parametrize("L,lmin,lmax,frac", [(300, 30, 64, 0.0), (77, 8, 33, 0.35), (64, 1, 16, 0.8)])
+@pytest.mark.parametrize("L,lmin,lmax,frac", [(300, 30, 64, 0.0), (77, 8, 33, 0.35), (64, 1, 16, 0.8),
+                                              (960, 8, 24, 0.8), (961, 8, 24, 0.8), (1500, 8, 24, 0.8),
+                                              (1200, 30, 64, 0.2)])
 def test_hits_long_pwms_and_ragged_lengths_vs_oracle(L, lmin, lmax, frac):
     """PWMs longer than 32 columns (the 64-column kernel), sequence lengths that are not a
-    multiple of the kernel's window chunk, single-column PWMs; thresholds frac x best score."""
+    multiple of the kernel's window chunk, single-column PWMs; sequences up to 960 bases with
+    PWMs of at most 32 columns are held in LDS as one-hot floats, from 961 on (and 1200 with
+    64-column PWMs) as bytes; thresholds frac x best score."""
     bases, pwms, lens, _ = _random_case(L + lmax, 90, L, 20, lmin, lmax)
     thresh = np.array([np.float32(frac) * O.best_score(pwms[k, :, :lens[k]]) for k in range(len(lens))], np.float32)
     k, n, p, st, _ = O.pwm_hits(bases, pwms, lens, thresh, both_strands=True)
