@@ -860,6 +860,30 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
     __syncthreads();
 
     // ---- pass 2: dh = Qh + conj(Y3h) kappa; g1h += dh - G1h; g3h += dh - G3h
+    // With prefetch, the next column's g1h, g3h and stored Y3h are loaded before this column's
+    // arithmetic (the filter spectra G1h, G3h come from the cache at the column's start).
+    constexpr int NY = 2;                       // stored Y3h bins per lane: Ph <= 64 NY (Lg <= 1020)
+    float4 p1[kPf ? NQ : 1], p3[kPf ? NQ : 1];
+    float2 py[kPf ? NY : 1];
+    auto load_next = [&](int m) {
+        const size_t colh = ((size_t)n * M + m) * Hs;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            const int t = lane + 64 * i;
+            if (t < nq) {
+                p1[i] = ld4(reinterpret_cast<const float *>(P.g1h + colh), t);
+                p3[i] = ld4(reinterpret_cast<const float *>(P.g3h + colh), t);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NY; ++i) {
+            const int k = lane + 64 * i;
+            if (k < Ph) py[i] = gp(y3s)[(size_t)m * Ph + k];
+        }
+    };
+    if constexpr (kPf) {
+        if (w < M) load_next(w);
+    }
     for (int m = w; m < M; m += W) {
         const int mm = __builtin_amdgcn_readfirstlane(m);
         const size_t colh = ((size_t)n * M + mm) * Hs, fcol = (size_t)mm * Hs;
@@ -878,9 +902,21 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
         };
         if constexpr (kPf) {
 #pragma unroll
-            for (int i = 0; i < NQ; ++i) load2(i, a1v[i], a3v[i], f1v[i], f3v[i]);
+            for (int i = 0; i < NQ; ++i) {
+                a1v[i] = p1[i];
+                a3v[i] = p3[i];
+                const int t = lane + 64 * i;
+                if (t < nq) { f1v[i] = ld4(G1h + 2 * fcol, t); f3v[i] = ld4(G3h + 2 * fcol, t); }
+            }
+#pragma unroll
+            for (int i = 0; i < NY; ++i) {
+                const int k = lane + 64 * i;
+                if (k < Ph) ya[k] = py[i];
+            }
+            if (mm + W < M) load_next(mm + W);
+        } else {
+            for (int k = lane; k < Ph; k += 64) ya[k] = gp(y3s)[(size_t)mm * Ph + k];
         }
-        for (int k = lane; k < Ph; k += 64) ya[k] = gp(y3s)[(size_t)mm * Ph + k];
         wave_lds_sync();
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
