@@ -70,14 +70,27 @@ def cpu_baseline(sample_n: int):
 
 C3_ITER_NUM, C3_CSC_ITER, C3_CDL_ITER = 10, 15, 1
 
+# Freeing device memory starts work in the driver that slows the device for a few seconds
+# afterwards (measured: about 1 s per 50 GB freed; the next batch ran 2-4 s slower after 228 GB
+# were freed, and at its normal speed after a 6 s pause).  The legs pause this long, outside
+# their timed regions, after steps that free tens of GB, so that one leg's frees do not land in
+# the next leg's timing.
+SETTLE_S = 6.0
 
-def batched_leg(n_datasets: int, log, N_range=(200, 20000), cache=True):
+
+def settle(log, what):
+    log("bench: %.0f s pause after %s" % (SETTLE_S, what))
+    time.sleep(SETTLE_S)
+
+
+def batched_leg(n_datasets: int, log, N_range=(200, 20000), cache=True, wave_frac=0.25):
     """Batched workload sample: n_datasets datasets drawn as in the batched configuration
     (N in [200, 20000], L in {100..500}, M in {32, 64}, filter_len in {16, 20, 24}), each
     trained once (10 x (15 CSC + update_best + 1 dict) = 150 ADMM iterations, code
     threshold).  Plain: one dataset at a time through the single-problem entry points.
-    Batched: datasets packed by device footprint into waves; one batch training (a captured
-    HIP graph) per wave.  Both include problem creation (data upload) and teardown."""
+    Batched: datasets packed by device footprint into waves of at most wave_frac of the free
+    device memory; one batch training (a captured HIP graph) per wave.  Both include problem
+    creation (data upload) and teardown."""
     import numpy as np
     import cdlm
     from cdlm import synth
@@ -114,9 +127,10 @@ def batched_leg(n_datasets: int, log, N_range=(200, 20000), cache=True):
         t_each.append(time.perf_counter() - t1)
     t_plain = time.perf_counter() - t0
     log("bench: batched workload, plain loop: %d datasets in %.2f s" % (len(specs), t_plain))
+    settle(log, "the plain loop")
 
     free, _total = ctx.mem_info()
-    budget = int(free * 0.8)
+    budget = int(free * wave_frac)
     sizes = [cdlm.footprint_bytes(sp["L"], sp["N"], sp["M"], sp["filter_len"]) for sp in specs]
     waves = cdlm.pack_by_footprint(sizes, budget)
     th_batch = [None] * len(specs)
@@ -138,7 +152,7 @@ def batched_leg(n_datasets: int, log, N_range=(200, 20000), cache=True):
     return {
         "workload": "%d datasets of the batched configuration (seeds 3..%d, N in [%d, %d]), 150 ADMM iterations each"
                     % (len(specs), 2 + len(specs), N_range[0], N_range[1]),
-        "datasets": len(specs), "waves": len(waves), "wave_budget_bytes": budget,
+        "datasets": len(specs), "waves": len(waves), "wave_budget_bytes": budget, "wave_fraction_of_free": wave_frac,
         "plain_datasets_per_s": len(specs) / t_plain, "batched_datasets_per_s": len(specs) / t_batch,
         "speedup": t_plain / t_batch,
         "plain_seq_iters_per_s": seq_iters / t_plain, "batched_seq_iters_per_s": seq_iters / t_batch,
@@ -254,6 +268,7 @@ def reference_loop_leg(s, fl, D0, t_fused_step, n_batch, csc_sample, log):
     peak = torch.cuda.max_memory_allocated() / 1e9
     del r
     torch.cuda.empty_cache()
+    settle(log, "freeing the reference-structured loop's %.0f GB" % peak)
     t_train = ITER_NUM * (CSC_ITER * t_csc + t_best + CDL_ITER * t_dict)
     log("bench: reference-structured loop at N=%d: CSC %.1f ms, update_best %.1f ms, dict %.1f ms per call "
         "(init %.2f s, peak %.0f GB) -> %.2f s per training"
@@ -280,6 +295,7 @@ def reference_loop_leg(s, fl, D0, t_fused_step, n_batch, csc_sample, log):
             p.close()
         t_b = time.perf_counter() - t0
         log("bench: batched %d datasets of the same shape in %.2f s (%.3f datasets/s)" % (n_batch, t_b, n_batch / t_b))
+        settle(log, "the same-shape batch")
         out["batched"] = {"datasets": n_batch, "seconds": t_b, "datasets_per_s": n_batch / t_b,
                           "vs_plain_loop": (n_batch / t_b) * t_train}
     return out
@@ -412,7 +428,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=N_SEQ, help="sequences (default: the workload's 20000)")
-    ap.add_argument("--cpu-sample", type=int, default=128)
+    ap.add_argument("--cpu-sample", type=int, default=384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-datasets", type=int, default=16, help="datasets in the batched leg (0: skip it)")
     ap.add_argument("--no-topology", dest="topology", action="store_false", help="skip the activation-topology leg")
@@ -421,6 +437,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2.0, help="job arrival rate of the served leg (jobs/s)")
     ap.add_argument("--small-datasets", type=int, default=64,
                     help="datasets with N in [200, 2000] in the small-dataset batched leg (0: skip it)")
+    ap.add_argument("--wave-frac", type=float, default=0.25,
+                    help="batched legs: device memory per wave, as a fraction of the free memory")
     ap.add_argument("--serve-policy", default="fifo", choices=["fifo", "shortest"], help="server admission order")
     ap.add_argument("--serve-jobs", type=int, default=16, help="jobs in the served leg (0: skip it)")
     ap.add_argument("--reference-loop", type=int, default=2, metavar="CSC_ITERS",
@@ -545,10 +563,12 @@ def main():
     if args.ingest:
         out["ingest"] = ingest_leg(bases, s, fl, D0, lambda m: print(m, file=sys.stderr, flush=True))
     if args.batch_datasets > 0:
-        out["batched"] = batched_leg(args.batch_datasets, lambda m: print(m, file=sys.stderr, flush=True))
+        log = lambda m: print(m, file=sys.stderr, flush=True)
+        out["batched"] = batched_leg(args.batch_datasets, log, wave_frac=args.wave_frac)
         if args.small_datasets > 0:
-            out["batched_small"] = batched_leg(args.small_datasets, lambda m: print(m, file=sys.stderr, flush=True),
-                                               N_range=(200, 2000), cache=False)
+            settle(log, "the batched waves")
+            out["batched_small"] = batched_leg(args.small_datasets, log, N_range=(200, 2000), cache=False,
+                                               wave_frac=args.wave_frac)
         if args.serve_jobs > 0:
             out["served"] = served_leg(args.serve_rate, args.serve_jobs,
                                        lambda m: print(m, file=sys.stderr, flush=True), args.serve_policy)

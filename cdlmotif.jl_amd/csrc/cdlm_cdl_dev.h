@@ -682,9 +682,11 @@ __global__ __launch_bounds__(kThreads) void k_decide(const CdlDev *__restrict__ 
 // spectral column (bins 2t, 2t+1) and s = lane + 64 i of a spatial one.
 // LDS: twiddles | h with wrapped tail | accS, accT, accP (kappa) | fa, fb (Lg complex, the
 // block transforms) | per wave: ya, yb (LgMax/4 complex each).
+// Registers: at most 168 (3 waves per SIMD) for padded lengths up to 1020 and filters up to
+// 24, where that needs no spills; 256 otherwise.
 template <int TAPS, int NSL>
-__global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(const CdlDev *__restrict__ tab, const int *__restrict__ starts,
-                                                                  int np, int LgMax) {
+__global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
+    const CdlDev *__restrict__ tab, const int *__restrict__ starts, int np, int LgMax) {
     constexpr bool kPf = NSL <= 4;              // spectral loads in flight across the Y3 transform
     constexpr int NQ = NSL <= 4 ? 4 : 9;        // spectral float4 slots per lane: Hs/2 <= 64 NQ
     extern __shared__ float4 smem4[];
