@@ -13,6 +13,7 @@
 // Sequences: base codes 0..3, one byte each, (N, L) row-major.
 
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <vector>
 
@@ -33,6 +34,32 @@ constexpr int kPwmG = 1, kPwmCH = 2;
 // compared from them).
 constexpr size_t kPwmLdsMax = 64 * 1024;
 constexpr int kMaxPwmLen = 64;
+
+// HIP events around one scan kernel on the context stream; stop() waits for the kernel and
+// stores its device time in ctx->last_scan_ms (cdlm_pwm_last_ms).
+struct ScanTimer {
+    cdlm_ctx *ctx;
+    hipEvent_t a = nullptr, b = nullptr;
+    explicit ScanTimer(cdlm_ctx *c) : ctx(c) {
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            (void)hipGetLastError();
+            if (a) (void)hipEventDestroy(a);
+            a = b = nullptr;
+        }
+        if (a) (void)hipEventRecord(a, ctx->stream);
+    }
+    void stop() {
+        if (!a) return;
+        float ms = 0.f;
+        if (hipEventRecord(b, ctx->stream) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
+            hipEventElapsedTime(&ms, a, b) == hipSuccess)
+            ctx->last_scan_ms = ms;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        a = b = nullptr;
+    }
+    ~ScanTimer() { stop(); }
+};
 
 // ---------------------------------------------------------------------------- hit list on MFMA
 // One-hot im2col x PWM contraction on v_mfma_f32_16x16x4_f32.  OH: the wave's sequence is held
@@ -186,6 +213,149 @@ __global__ __launch_bounds__(256) void k_pwm_hits_mfma(const uint8_t *__restrict
         }
         wave_lds_sync();
     }
+}
+
+// ---------------------------------------------------------------------------- best window on MFMA
+// Per strand-PWM slot and sequence: the first window of largest value, on the hit list's one-hot
+// im2col x PWM contraction (tiles, B operands and sequence in LDS as there).
+//   NUC = false, scan_w_gpu! (scan.jl:176-221): the position-major sum of greedy_search!
+//     (:231-234); value = score > thresh ? score : 0, and windows past the end (l + m > L, l < L)
+//     are 0 as in the reference's dense array.
+//   NUC = true, scan_n (scan.jl:10-21) for overlapping_scan!: the nucleotide-major sum, one pass
+//     per base a with A[i][a'] = [a' == a][base(w0 + i + j) == a], so every MFMA step still adds
+//     a single product (pwm_c[a][j] or +0) in the order a = A, C, G, T, j ascending; value =
+//     score; only windows that fit compete.
+// The first maximum wins (strict >; across lanes the smaller window).  Output per slot s and
+// sequence n: bv[s N + n], bp[s N + n] (0-based window; INT_MAX when no window competes).
+template <int MAXJ, int CH, bool OH, bool NUC>
+__global__ __launch_bounds__(256) void k_pwm_best_mfma(const uint8_t *__restrict__ bases, int L, long N, PwmTiles P,
+                                                       float *__restrict__ bv, int *__restrict__ bp) {
+    extern __shared__ uint8_t seqs_lds[];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int Lpad = pwm_lpad(L, MAXJ, CH);
+    uint8_t *sq = seqs_lds + w * Lpad;
+    float4 *oh = reinterpret_cast<float4 *>(seqs_lds) + (size_t)w * Lpad;
+    const float *ohf = reinterpret_cast<const float *>(oh);
+    const int t = blockIdx.x;
+    const bool ok = t < P.T;
+    const int jsteps = ok ? P.tlen[t] : 0;    // uniform: the tile's longest PWM
+    float Bv[MAXJ];
+#pragma unroll
+    for (int j = 0; j < MAXJ; ++j) Bv[j] = (ok && j < jsteps) ? P.B[P.boff[t] + j * 64 + lane] : 0.f;
+    const int sl = t * 16 + (lane & 15);
+    const int sk = ok ? P.slot_k[sl] : -1, sm = ok ? P.slot_m[sl] : 0;
+    const float sth = ok ? P.slot_th[sl] : 0.f;
+    const int arow = lane & 15, abase = lane >> 4;
+    const long nstride = (long)gridDim.y * 4;
+    for (long n = (long)blockIdx.y * 4 + w; n < N; n += nstride) {
+        const uint8_t *src = bases + n * (long)L;
+        if constexpr (OH) {
+            for (int i = lane; i < Lpad; i += 64) {
+                const int b = i < L ? src[i] : 4;
+                oh[i] = make_float4(b == 0 ? 1.f : 0.f, b == 1 ? 1.f : 0.f, b == 2 ? 1.f : 0.f, b == 3 ? 1.f : 0.f);
+            }
+        } else {
+            for (int i = lane; i < Lpad; i += 64) sq[i] = i < L ? src[i] : (uint8_t)4;
+        }
+        wave_lds_sync();
+        float best = 0.f;
+        int bi = INT_MAX;
+        for (int w0 = 0; w0 < L; w0 += 16 * CH) {
+            floatx4 acc[CH];
+#pragma unroll
+            for (int c = 0; c < CH; ++c) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
+            for (int a = 0; a < (NUC ? 4 : 1); ++a) {
+#define CDLM_BEST_STEPS(JS)                                                                              \
+    case JS:                                                                                             \
+        _Pragma("unroll") for (int j = 0; j < JS; ++j) {                                                 \
+            _Pragma("unroll") for (int c = 0; c < CH; ++c) {                                             \
+                const int p = w0 + 16 * c + arow + j;                                                    \
+                float av = OH ? ohf[4 * p + abase] : ((sq[p] == abase) ? 1.0f : 0.0f);                   \
+                if (NUC) av = (abase == a) ? av : 0.0f;                                                  \
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, Bv[j], acc[c], 0, 0, 0);               \
+            }                                                                                            \
+        }                                                                                                \
+        break;
+#define CDLM_BEST_STEPS8(J0)                                                                             \
+    CDLM_BEST_STEPS(J0 + 1) CDLM_BEST_STEPS(J0 + 2) CDLM_BEST_STEPS(J0 + 3) CDLM_BEST_STEPS(J0 + 4)      \
+    CDLM_BEST_STEPS(J0 + 5) CDLM_BEST_STEPS(J0 + 6) CDLM_BEST_STEPS(J0 + 7) CDLM_BEST_STEPS(J0 + 8)
+                switch (jsteps) {
+                    CDLM_BEST_STEPS8(0) CDLM_BEST_STEPS8(8) CDLM_BEST_STEPS8(16) CDLM_BEST_STEPS8(24)
+                    default:
+                        if constexpr (MAXJ > 32) {
+                            switch (jsteps) {
+                                CDLM_BEST_STEPS8(32) CDLM_BEST_STEPS8(40) CDLM_BEST_STEPS8(48) CDLM_BEST_STEPS8(56)
+                                default: break;
+                            }
+                        }
+                        break;
+                }
+#undef CDLM_BEST_STEPS8
+#undef CDLM_BEST_STEPS
+            }
+            // this lane's windows of its slot, in increasing order: l = w0 + 16 c + 4 abase + r
+#pragma unroll
+            for (int c = 0; c < CH; ++c)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int l = w0 + 16 * c + 4 * abase + r;
+                    const bool fits = l + sm <= L;
+                    if (sk < 0 || l >= L || (NUC && !fits)) continue;
+                    float v = acc[c][r];
+                    if (!NUC) v = (fits && v > sth) ? v : 0.f;
+                    if (bi == INT_MAX || v > best) { best = v; bi = l; }
+                }
+        }
+        // the four lanes of a slot: larger value, then smaller window
+#pragma unroll
+        for (int off = 16; off < 64; off <<= 1) {
+            const float ob = __shfl_xor(best, off);
+            const int oi = __shfl_xor(bi, off);
+            if (oi != INT_MAX && (bi == INT_MAX || ob > best || (ob == best && oi < bi))) { best = ob; bi = oi; }
+        }
+        if (lane < 16 && sk >= 0) {
+            bv[(size_t)sl * N + n] = best;
+            bp[(size_t)sl * N + n] = bi;
+        }
+        wave_lds_sync();
+    }
+}
+
+// scan_w_gpu! positions from the forward slots (slot_of[2k]): 1-based window, 0 for none or
+// for window 1 (scan.jl:197).
+__global__ void k_pwm_argmax_out(const int *__restrict__ slot_of, int K, long N, const int *__restrict__ bp,
+                                 int *__restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)K * N) return;
+    const int k = (int)(i / N);
+    const long n = i - (long)k * N;
+    const int b = bp[(size_t)slot_of[2 * k] * N + n];
+    out[i] = (b == INT_MAX || b == 0) ? 0 : b + 1;
+}
+
+// overlapping_scan! (scan.jl:89-121): the forward strand wins ties (>=); a strand with no
+// window scores -Inf; hit iff the chosen score > thresh.
+__global__ void k_pwm_best_out(const int *__restrict__ slot_of, const float *__restrict__ thresh, int K, long N,
+                               const float *__restrict__ bv, const int *__restrict__ bp, int *__restrict__ opos,
+                               signed char *__restrict__ ocomp, float *__restrict__ oscore) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)K * N) return;
+    const int k = (int)(i / N);
+    const long n = i - (long)k * N;
+    const size_t f = (size_t)slot_of[2 * k] * N + n, c = (size_t)slot_of[2 * k + 1] * N + n;
+    const int pf = bp[f], pc = bp[c];
+    const float vf = pf == INT_MAX ? -INFINITY : bv[f], vc = pc == INT_MAX ? -INFINITY : bv[c];
+    const float th = thresh[k];
+    int pos = 0, comp = 0;
+    float sc = 0.f;
+    if (vf >= vc) {
+        if (pf != INT_MAX && vf > th) { pos = pf + 1; comp = 0; sc = vf; }
+    } else {
+        if (pc != INT_MAX && vc > th) { pos = pc + 1; comp = 1; sc = vc; }
+    }
+    opos[i] = pos;
+    ocomp[i] = (signed char)comp;
+    oscore[i] = sc;
 }
 
 // Lane layout and rounding probe of v_mfma_f32_16x16x4_f32: lane l supplies a[l] and b[l];
@@ -356,6 +526,7 @@ struct HostTiles {
     std::vector<float> B;
     std::vector<int> boff, tlen, sk, sm, sst;
     std::vector<float> sth;
+    std::vector<int> slot_of;   // (K, 2): slot of PWM k on strand st, -1 if not scanned
     int T = 0;
 };
 
@@ -366,6 +537,8 @@ void build_tiles(const float *pwms, const int *lens, const float *thresh, int K,
         for (int st = 0; st < (both ? 2 : 1); ++st) v.push_back({k, st, lens[k]});
     std::stable_sort(v.begin(), v.end(), [](const SP &x, const SP &y) { return x.m > y.m; });
     ht.T = (int)((v.size() + 15) / 16);
+    ht.slot_of.assign(2 * (size_t)K, -1);
+    for (size_t i = 0; i < v.size(); ++i) ht.slot_of[2 * (size_t)v[i].k + v[i].st] = (int)i;
     ht.sk.assign(ht.T * 16, -1); ht.sm.assign(ht.T * 16, 0); ht.sst.assign(ht.T * 16, 0); ht.sth.assign(ht.T * 16, 0.f);
     for (int t = 0; t < ht.T; ++t) {
         int tl = 0;
@@ -465,6 +638,80 @@ cdlm_status launch_hits(cdlm_ctx *ctx, const uint8_t *d_bases, int L, long N, co
     }
     (void)hipStreamSynchronize(st);
     dt.release();
+    return s;
+}
+
+
+// Best window per (PWM, sequence) on MFMA: the argmax of scan_w_gpu! (nuc = false, forward
+// strand) into o_pos, or the best hit of overlapping_scan! (nuc = true, both strands) into
+// o_pos, o_comp, o_score; device pointers, (K, N).  CDLM_ERR_UNSUPPORTED (no error recorded)
+// when four waves' sequences do not fit the LDS: the caller then runs the per-(k, n) kernels.
+cdlm_status launch_best(cdlm_ctx *ctx, const uint8_t *d_bases, int L, long N, const float *pwms, const int *lens,
+                        const float *thresh, const float *d_thresh, int K, int maxlen, bool nuc, int *o_pos,
+                        signed char *o_comp, float *o_score) {
+    const int MAXJ = maxlen <= 32 ? 32 : 64;
+    const int Lpad = pwm_lpad(L, MAXJ, kPwmCH);
+    const bool oh = 4 * (size_t)Lpad * sizeof(float4) <= kPwmLdsMax;
+    const size_t lds = 4 * (size_t)Lpad * (oh ? sizeof(float4) : 1);
+    if (lds > kPwmLdsMax) return CDLM_ERR_UNSUPPORTED;
+    if (N == 0) return CDLM_OK;
+    hipStream_t st = ctx->stream;
+    HostTiles ht;
+    build_tiles(pwms, lens, thresh, K, maxlen, nuc ? 1 : 0, ht);
+    DevTiles dt;
+    int *d_slot = nullptr, *bpos = nullptr;
+    float *bval = nullptr;
+    cdlm_status s = CDLM_OK;
+    if ((s = up(ctx, &dt.B, ht.B)) == CDLM_OK && (s = up(ctx, &dt.boff, ht.boff)) == CDLM_OK &&
+        (s = up(ctx, &dt.tlen, ht.tlen)) == CDLM_OK && (s = up(ctx, &dt.sk, ht.sk)) == CDLM_OK &&
+        (s = up(ctx, &dt.sm, ht.sm)) == CDLM_OK && (s = up(ctx, &dt.sst, ht.sst)) == CDLM_OK &&
+        (s = up(ctx, &dt.sth, ht.sth)) == CDLM_OK && (s = up(ctx, &d_slot, ht.slot_of)) == CDLM_OK) {
+        const size_t nslot = (size_t)ht.T * 16;
+        if (hipMalloc(&bval, sizeof(float) * nslot * N) != hipSuccess || hipMalloc(&bpos, sizeof(int) * nslot * N) != hipSuccess) {
+            (void)hipGetLastError();
+            s = cdlm_fail(ctx, CDLM_ERR_OOM, "no device memory for %zu best-window records", nslot * (size_t)N);
+        }
+    }
+    if (s == CDLM_OK) {
+        PwmTiles P;
+        P.T = ht.T; P.B = dt.B; P.boff = dt.boff; P.tlen = dt.tlen;
+        P.slot_k = dt.sk; P.slot_m = dt.sm; P.slot_st = dt.sst; P.slot_th = dt.sth;
+        P.groups = ht.T;
+        long yb = (N + 3) / 4;
+        const long target = (long)ctx->num_cu * 16 / (P.groups > 0 ? P.groups : 1) + 1;
+        if (yb > target) yb = target;
+        if (yb > 65535) yb = 65535;
+        if (yb < 1) yb = 1;
+        const long KN = (long)K * N;
+        const unsigned ob = (unsigned)((KN + 255) / 256);
+        ScanTimer tm(ctx);
+#define L_BEST(MJ, OHV, NUCV)                                                                                      \
+    hipLaunchKernelGGL((k_pwm_best_mfma<MJ, kPwmCH, OHV, NUCV>), dim3(P.groups, (unsigned)yb), dim3(256), lds, st, \
+                       d_bases, L, N, P, bval, bpos)
+        if (nuc) {
+            if (MAXJ == 32 && oh) L_BEST(32, true, true);
+            else if (MAXJ == 32) L_BEST(32, false, true);
+            else if (oh) L_BEST(64, true, true);
+            else L_BEST(64, false, true);
+            hipLaunchKernelGGL(k_pwm_best_out, dim3(ob), dim3(256), 0, st, d_slot, d_thresh, K, N, bval, bpos, o_pos,
+                               o_comp, o_score);
+        } else {
+            if (MAXJ == 32 && oh) L_BEST(32, true, false);
+            else if (MAXJ == 32) L_BEST(32, false, false);
+            else if (oh) L_BEST(64, true, false);
+            else L_BEST(64, false, false);
+            hipLaunchKernelGGL(k_pwm_argmax_out, dim3(ob), dim3(256), 0, st, d_slot, K, N, bpos, o_pos);
+        }
+#undef L_BEST
+        const hipError_t e = hipGetLastError();
+        tm.stop();
+        if (e != hipSuccess) s = cdlm_fail(ctx, CDLM_ERR_HIP, "k_pwm_best_mfma: %s", hipGetErrorString(e));
+    }
+    (void)hipStreamSynchronize(st);
+    dt.release();
+    if (d_slot) (void)hipFree(d_slot);
+    if (bval) (void)hipFree(bval);
+    if (bpos) (void)hipFree(bpos);
     return s;
 }
 
@@ -569,14 +816,22 @@ extern "C" cdlm_status cdlm_pwm_argmax(cdlm_ctx *ctx, const uint8_t *bases, int3
         s = cdlm_fail(ctx, CDLM_ERR_OOM, "out");
     }
     if (s == CDLM_OK && N > 0) {
-        const long waves = (long)K * N;
-        const long blocks = (waves + 3) / 4;
-        hipLaunchKernelGGL(k_pwm_argmax, dim3((unsigned)blocks), dim3(kPwmThreads), 0, ctx->stream, d.bases, L, N,
-                           d.pwms, d.lens, d.thresh, K, maxlen, o);
-        if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(out_pos, o, sizeof(int) * (size_t)K * N, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-            hipStreamSynchronize(ctx->stream) != hipSuccess)
-            s = cdlm_fail(ctx, CDLM_ERR_HIP, "k_pwm_argmax failed");
+        // on the MFMA tiles when the sequences fit the LDS, else one wave per (k, n)
+        s = launch_best(ctx, d.bases, L, N, pwms, lens, thresh, d.thresh, K, maxlen, false, o, nullptr, nullptr);
+        if (s == CDLM_ERR_UNSUPPORTED) {
+            s = CDLM_OK;
+            const long waves = (long)K * N;
+            const long blocks = (waves + 3) / 4;
+            ScanTimer tm(ctx);
+            hipLaunchKernelGGL(k_pwm_argmax, dim3((unsigned)blocks), dim3(kPwmThreads), 0, ctx->stream, d.bases, L, N,
+                               d.pwms, d.lens, d.thresh, K, maxlen, o);
+            tm.stop();
+            if (hipGetLastError() != hipSuccess) s = cdlm_fail(ctx, CDLM_ERR_HIP, "k_pwm_argmax failed");
+        }
+        if (s == CDLM_OK &&
+            (hipMemcpyAsync(out_pos, o, sizeof(int) * (size_t)K * N, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+             hipStreamSynchronize(ctx->stream) != hipSuccess))
+            s = cdlm_fail(ctx, CDLM_ERR_HIP, "copy of the argmax positions failed");
     }
     release(ctx, d, true);
     if (o) (void)hipFree(o);
@@ -602,15 +857,23 @@ extern "C" cdlm_status cdlm_pwm_best(cdlm_ctx *ctx, const uint8_t *bases, int32_
         s = cdlm_fail(ctx, CDLM_ERR_OOM, "out");
     }
     if (s == CDLM_OK && N > 0) {
-        const long blocks = ((long)KN + 3) / 4;
-        hipLaunchKernelGGL(k_pwm_best, dim3((unsigned)blocks), dim3(kPwmThreads), 0, ctx->stream, d.bases, L, N, d.pwms,
-                           d.lens, d.thresh, K, maxlen, op, oc, os);
+        // on the MFMA tiles when the sequences fit the LDS, else one wave per (k, n)
+        s = launch_best(ctx, d.bases, L, N, pwms, lens, thresh, d.thresh, K, maxlen, true, op, oc, os);
+        if (s == CDLM_ERR_UNSUPPORTED) {
+            s = CDLM_OK;
+            const long blocks = ((long)KN + 3) / 4;
+            ScanTimer tm(ctx);
+            hipLaunchKernelGGL(k_pwm_best, dim3((unsigned)blocks), dim3(kPwmThreads), 0, ctx->stream, d.bases, L, N,
+                               d.pwms, d.lens, d.thresh, K, maxlen, op, oc, os);
+            tm.stop();
+            if (hipGetLastError() != hipSuccess) s = cdlm_fail(ctx, CDLM_ERR_HIP, "k_pwm_best failed");
+        }
         hipStream_t st = ctx->stream;
-        if (hipGetLastError() != hipSuccess || hipMemcpyAsync(out_pos, op, KN * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipMemcpyAsync(out_strand, oc, KN, hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipMemcpyAsync(out_score, os, KN * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipStreamSynchronize(st) != hipSuccess)
-            s = cdlm_fail(ctx, CDLM_ERR_HIP, "k_pwm_best failed");
+        if (s == CDLM_OK && (hipMemcpyAsync(out_pos, op, KN * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                             hipMemcpyAsync(out_strand, oc, KN, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                             hipMemcpyAsync(out_score, os, KN * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                             hipStreamSynchronize(st) != hipSuccess))
+            s = cdlm_fail(ctx, CDLM_ERR_HIP, "copy of the best hits failed");
     }
     release(ctx, d, true);
     if (op) (void)hipFree(op);

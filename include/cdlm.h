@@ -345,7 +345,8 @@ cdlm_status cdlm_pwm_argmax(cdlm_ctx *ctx, const uint8_t *bases, int32_t L, int6
 cdlm_status cdlm_pwm_best(cdlm_ctx *ctx, const uint8_t *bases, int32_t L, int64_t N, const float *pwms,
                           const int32_t *lens, const float *thresh, int32_t K, int32_t maxlen, int32_t *out_pos,
                           int8_t *out_strand, float *out_score);
-/* Device time (HIP events on the context stream) of the last hit-list scan kernel. */
+/* Device time (HIP events on the context stream) of the last scan kernel: hit list
+ * (cdlm_pwm_hits), argmax (cdlm_pwm_argmax) or best hit (cdlm_pwm_best). */
 cdlm_status cdlm_pwm_last_ms(cdlm_ctx *ctx, double *ms);
 /* get_thresh(pwms, pval, Float32) (src/SEARCH/helpers.jl:99): per PWM the Float32 of
  * pvalue2score(pwm, pval) (src/SEARCH/PWM_Touzet.jl:199-217; granularity eps, 0.1 in the

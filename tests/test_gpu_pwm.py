@@ -147,7 +147,7 @@ def test_hits_long_pwms_and_ragged_lengths_vs_oracle(L, lmin, lmax, frac):
     """PWMs longer than 32 columns (the 64-column kernel), sequence lengths that are not a
     multiple of the kernel's window chunk, single-column PWMs; sequences up to 960 bases with
     PWMs of at most 32 columns are held in LDS as one-hot floats, from 961 on (and 1200 with
-    64-column PWMs) as bytes; thresholds frac x best score."""
+    64-column PWMs) as bytes; thresholds frac x best score.  Hit list, argmax and best hit."""
     bases, pwms, lens, _ = _random_case(L + lmax, 90, L, 20, lmin, lmax)
     thresh = np.array([np.float32(frac) * O.best_score(pwms[k, :, :lens[k]]) for k in range(len(lens))], np.float32)
     k, n, p, st, _ = O.pwm_hits(bases, pwms, lens, thresh, both_strands=True)
@@ -157,3 +157,12 @@ def test_hits_long_pwms_and_ragged_lengths_vs_oracle(L, lmin, lmax, frac):
     flush("L=%d lens %d-%d: %d hits" % (L, lmin, lmax, len(exp)))
     assert len(exp) > 0
     np.testing.assert_array_equal(got, exp)
+    # the best-window scans on the same tiles: argmax (position-major) and best hit
+    # (nucleotide-major, both strands)
+    np.testing.assert_array_equal(cdlm.scan_w_gpu(bases, pwms, lens, thresh),
+                                  O.pwm_gpu_argmax(bases, pwms, lens, thresh))
+    pos, comp, sc = cdlm.overlapping_scan(bases, pwms, lens, thresh)
+    opos, ocomp, osc = O.pwm_best(bases, pwms, lens, thresh)
+    np.testing.assert_array_equal(pos, opos)
+    np.testing.assert_array_equal(comp.astype(np.int8), ocomp)
+    np.testing.assert_array_equal(sc.view(np.uint32), osc.view(np.uint32))
