@@ -380,8 +380,9 @@ def ingest_leg(bases, s, fl, D0, log):
 def pwm_leg(n_seq: int, log):
     """PWM scan sample: n_seq sequences of L=500 iid bases (GC 0.42) against 512 log-odds
     PWMs of length 8-24 (the PWM-scan configuration at a bounded N), both strands,
-    threshold 0.8 x best score, full hit list.  Kernel time from HIP events around the
-    MFMA hit kernel; algorithmic flops = sum_k N (L - m_k + 1) x 2 strands x 2 x 4 m_k."""
+    threshold 0.8 x best score, full hit list; then the argmax and best-hit modes on the same
+    workload.  Kernel time from HIP events around the MFMA kernels; algorithmic flops of the
+    hit list = sum_k N (L - m_k + 1) x 2 strands x 2 x 4 m_k."""
     import ctypes as Cc
     import numpy as np
     import cdlm
@@ -403,6 +404,25 @@ def pwm_leg(n_seq: int, log):
     tf = flops / (k_ms * 1e-3) / 1e12
     log("bench: PWM scan %d x %d, %d PWMs: kernel %.2f ms (%.1f TFLOP/s), %d hits, call %.2f s"
         % (n_seq, PWM_L, PWM_K, k_ms, tf, len(h), wall))
+    # the best-window scans on the same workload: argmax of the thresholded position-major
+    # scores, forward strand (scan_w_gpu!), and best hit per strand pair with nucleotide-major
+    # sums (overlapping_scan!, the workload's best-hit mode); MFMA flops: one pass for the
+    # argmax, four (one per base) per strand for the best hit
+    from cdlm import scan as S
+    modes = {}
+    for name, fn, passes, strands in (("argmax", S.scan_w_gpu, 1, 1), ("best_hit", S.overlapping_scan, 4, 2)):
+        mms = []
+        for _ in range(3):
+            fn(bases, pwms, lens, thresh)
+            v = Cc.c_double()
+            _lib.check(_lib.load().cdlm_pwm_last_ms(ctx.h, Cc.byref(v)), ctx.h)
+            mms.append(v.value)
+        mm = float(np.median(mms))
+        fl = float(sum(n_seq * (PWM_L - int(m) + 1) * strands * 2 * 4 * int(m) * passes for m in lens))
+        wn = float(sum(n_seq * (PWM_L - int(m) + 1) * strands for m in lens))
+        modes[name] = {"kernel_ms": mm, "windows_per_s": wn / (mm * 1e-3), "mfma_tflops": fl / (mm * 1e-3) / 1e12}
+        log("bench: PWM %s: kernels %.2f ms (%.2e windows/s, %.1f TFLOP/s on MFMA)"
+            % (name, mm, wn / (mm * 1e-3), fl / (mm * 1e-3) / 1e12))
     # score thresholds of the same PWMs for the reference's p-value (get_thresh, pvalue 2.7e-4
     # as Float32): exact score distributions, one work-group per PWM
     pv = float(np.float32(2.7e-4))
@@ -416,6 +436,7 @@ def pwm_leg(n_seq: int, log):
         % (PWM_K, float(np.median(th_ms)), float(np.median(th_wall)), float(th.mean())))
     return {"workload": "PWM scan: %d sequences x L=%d, %d PWMs of length 8-24, both strands, hit list" % (n_seq, PWM_L, PWM_K),
             "kernel_ms": k_ms, "windows_per_s": windows / (k_ms * 1e-3), "hits": int(len(h)),
+            "best_window_modes": modes,
             "pvalue_thresholds": {"pwms": PWM_K, "pvalue": pv, "kernel_ms": float(np.median(th_ms)),
                                   "call_ms": float(np.median(th_wall))},
             "roofline": {"bound": "mfma", "achieved": tf, "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
