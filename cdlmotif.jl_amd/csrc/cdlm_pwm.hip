@@ -321,6 +321,101 @@ __global__ __launch_bounds__(256) void k_pwm_best_mfma(const uint8_t *__restrict
     }
 }
 
+// ---------------------------------------------------------------------------- best hit on the K=1 MFMA
+// The nucleotide-major best hit (NUC mode above) on v_mfma_f32_32x32x1_2b_f32, whose single
+// K row makes every product useful: each step adds pwm_c[a][j] or +0, so the four one-base passes
+// cost what one position-major pass of the K=4 form costs.  Layout (measured): block b uses A and
+// B lanes 32b..32b+31; result register r of lane l is block r/16, row 8((r%16)/4) + 4(l/32) + r%4,
+// column l%32.  Both blocks take the tile's 32 strand-PWM slots, block b the windows
+// w0 + 32b + row.  LDS: the tile's B values [a][j][32 slots] | per wave the sequence as four
+// one-hot planes of LP floats.  Same outputs as k_pwm_best_mfma (slot s = 32 t + column).
+constexpr int kBest32Lds = 64 * 1024;
+__host__ __device__ constexpr int best32_lpad(int L, int MAXJ) { return ((L + 63) / 64) * 64 + MAXJ + 16; }
+
+template <int MAXJ>
+__global__ __launch_bounds__(256) void k_pwm_best32_mfma(const uint8_t *__restrict__ bases, int L, long N,
+                                                         const float *__restrict__ B32, const int *__restrict__ tlen,
+                                                         const int *__restrict__ slot_k, const int *__restrict__ slot_m,
+                                                         int T, float *__restrict__ bv, int *__restrict__ bp) {
+    typedef float floatx32 __attribute__((ext_vector_type(32)));
+    extern __shared__ float lds32[];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int LP = best32_lpad(L, MAXJ);
+    float *bt = lds32;                                   // 4 * MAXJ * 32
+    float *oh = lds32 + 4 * MAXJ * 32 + (size_t)w * 4 * LP;
+    const int t = blockIdx.x;
+    const int jsteps = t < T ? tlen[t] : 0;
+    for (int i = threadIdx.x; i < 4 * MAXJ * 32; i += 256) {
+        const int a = i / (MAXJ * 32), j = (i / 32) % MAXJ;
+        bt[i] = (t < T && j < jsteps) ? B32[((size_t)t * 4 + a) * MAXJ * 32 + (i % (MAXJ * 32))] : 0.f;
+    }
+    __syncthreads();
+    const int col = lane & 31, half = lane >> 5;
+    const int sl = t * 32 + col;
+    const int sk = t < T ? slot_k[sl] : -1, sm = t < T ? slot_m[sl] : 0;
+    const long nstride = (long)gridDim.y * 4;
+    for (long n = (long)blockIdx.y * 4 + w; n < N; n += nstride) {
+        const uint8_t *src = bases + n * (long)L;
+        for (int i = lane; i < LP; i += 64) {
+            const int b = i < L ? src[i] : 4;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) oh[a * LP + i] = b == a ? 1.f : 0.f;
+        }
+        wave_lds_sync();
+        float best = 0.f;
+        int bi = INT_MAX;
+        for (int w0 = 0; w0 < L; w0 += 64) {
+            floatx32 acc;
+#pragma unroll
+            for (int r = 0; r < 32; ++r) acc[r] = 0.f;
+            for (int a = 0; a < 4; ++a) {
+                const float *pa = oh + a * LP + w0 + lane;     // window w0 + lane (row lane%32 of block lane/32)
+                const float *ba = bt + a * MAXJ * 32 + col;
+#define CDLM_B32_STEPS(JS)                                                                               \
+    case JS:                                                                                             \
+        _Pragma("unroll") for (int j = 0; j < JS; ++j)                                                   \
+            acc = __builtin_amdgcn_mfma_f32_32x32x1f32(pa[j], ba[32 * j], acc, 0, 0, 0);                 \
+        break;
+#define CDLM_B32_STEPS8(J0)                                                                              \
+    CDLM_B32_STEPS(J0 + 1) CDLM_B32_STEPS(J0 + 2) CDLM_B32_STEPS(J0 + 3) CDLM_B32_STEPS(J0 + 4)          \
+    CDLM_B32_STEPS(J0 + 5) CDLM_B32_STEPS(J0 + 6) CDLM_B32_STEPS(J0 + 7) CDLM_B32_STEPS(J0 + 8)
+                switch (jsteps) {
+                    CDLM_B32_STEPS8(0) CDLM_B32_STEPS8(8) CDLM_B32_STEPS8(16) CDLM_B32_STEPS8(24)
+                    default:
+                        if constexpr (MAXJ > 32) {
+                            switch (jsteps) {
+                                CDLM_B32_STEPS8(32) CDLM_B32_STEPS8(40) CDLM_B32_STEPS8(48) CDLM_B32_STEPS8(56)
+                                default: break;
+                            }
+                        }
+                        break;
+                }
+#undef CDLM_B32_STEPS8
+#undef CDLM_B32_STEPS
+            }
+            // this lane's windows in increasing order: block r/16, row 8((r%16)/4) + 4 half + r%4
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                const int l = w0 + 32 * (r >> 4) + 8 * ((r & 15) >> 2) + 4 * half + (r & 3);
+                if (sk < 0 || l + sm > L) continue;
+                const float v = acc[r];
+                if (bi == INT_MAX || v > best) { best = v; bi = l; }
+            }
+        }
+        // the two lanes of a slot: larger value, then smaller window
+        {
+            const float ob = __shfl_xor(best, 32);
+            const int oi = __shfl_xor(bi, 32);
+            if (oi != INT_MAX && (bi == INT_MAX || ob > best || (ob == best && oi < bi))) { best = ob; bi = oi; }
+        }
+        if (half == 0 && sk >= 0) {
+            bv[(size_t)sl * N + n] = best;
+            bp[(size_t)sl * N + n] = bi;
+        }
+        wave_lds_sync();
+    }
+}
+
 // scan_w_gpu! positions from the forward slots (slot_of[2k]): 1-based window, 0 for none or
 // for window 1 (scan.jl:197).
 __global__ void k_pwm_argmax_out(const int *__restrict__ slot_of, int K, long N, const int *__restrict__ bp,
@@ -568,6 +663,39 @@ void build_tiles(const float *pwms, const int *lens, const float *thresh, int K,
     }
 }
 
+// Tiles of 32 strand-PWM slots for k_pwm_best32_mfma: B values [tile][a][j < MAXJ][slot], the
+// same length-sorted slot order as build_tiles.
+struct HostTiles32 {
+    std::vector<float> B;
+    std::vector<int> tlen, sk, sm, slot_of;
+    int T = 0;
+};
+
+void build_tiles32(const float *pwms, const int *lens, int K, int maxlen, int MAXJ, HostTiles32 &ht) {
+    struct SP { int k, st, m; };
+    std::vector<SP> v;
+    for (int k = 0; k < K; ++k)
+        for (int st = 0; st < 2; ++st) v.push_back({k, st, lens[k]});
+    std::stable_sort(v.begin(), v.end(), [](const SP &x, const SP &y) { return x.m > y.m; });
+    ht.T = (int)((v.size() + 31) / 32);
+    ht.slot_of.assign(2 * (size_t)K, -1);
+    ht.sk.assign((size_t)ht.T * 32, -1);
+    ht.sm.assign((size_t)ht.T * 32, 0);
+    ht.B.assign((size_t)ht.T * 4 * MAXJ * 32, 0.f);
+    ht.tlen.assign(ht.T, 0);
+    for (size_t i = 0; i < v.size(); ++i) {
+        const int t = (int)(i / 32), c = (int)(i % 32), k = v[i].k, m = v[i].m;
+        ht.slot_of[2 * (size_t)k + v[i].st] = (int)i;
+        ht.sk[i] = k;
+        ht.sm[i] = m;
+        ht.tlen[t] = std::max(ht.tlen[t], m);
+        const float *pw = pwms + (size_t)k * 4 * maxlen;
+        for (int a = 0; a < 4; ++a)
+            for (int j = 0; j < m; ++j)
+                ht.B[(((size_t)t * 4 + a) * MAXJ + j) * 32 + c] = v[i].st ? pw[(3 - a) * maxlen + (m - 1 - j)] : pw[a * maxlen + j];
+    }
+}
+
 struct DevTiles {
     float *B = nullptr, *sth = nullptr;
     int *boff = nullptr, *tlen = nullptr, *sk = nullptr, *sm = nullptr, *sst = nullptr;
@@ -656,6 +784,49 @@ cdlm_status launch_best(cdlm_ctx *ctx, const uint8_t *d_bases, int L, long N, co
     if (lds > kPwmLdsMax) return CDLM_ERR_UNSUPPORTED;
     if (N == 0) return CDLM_OK;
     hipStream_t st = ctx->stream;
+    // the best hit on the K=1 MFMA when its LDS (tile B values and four one-hot planes per
+    // wave) fits
+    const size_t lds32 = sizeof(float) * (4 * (size_t)MAXJ * 32 + 4 * 4 * (size_t)best32_lpad(L, MAXJ));
+    if (nuc && lds32 <= (size_t)kBest32Lds) {
+        HostTiles32 h32;
+        build_tiles32(pwms, lens, K, maxlen, MAXJ, h32);
+        float *dB = nullptr, *bval = nullptr;
+        int *dtl = nullptr, *dsk = nullptr, *dsm = nullptr, *dso = nullptr, *bpos = nullptr;
+        cdlm_status s = CDLM_OK;
+        if ((s = up(ctx, &dB, h32.B)) == CDLM_OK && (s = up(ctx, &dtl, h32.tlen)) == CDLM_OK &&
+            (s = up(ctx, &dsk, h32.sk)) == CDLM_OK && (s = up(ctx, &dsm, h32.sm)) == CDLM_OK &&
+            (s = up(ctx, &dso, h32.slot_of)) == CDLM_OK) {
+            const size_t nslot = (size_t)h32.T * 32;
+            if (hipMalloc(&bval, sizeof(float) * nslot * N) != hipSuccess || hipMalloc(&bpos, sizeof(int) * nslot * N) != hipSuccess) {
+                (void)hipGetLastError();
+                s = cdlm_fail(ctx, CDLM_ERR_OOM, "no device memory for %zu best-window records", nslot * (size_t)N);
+            }
+        }
+        if (s == CDLM_OK) {
+            long yb = (N + 3) / 4;
+            const long target = (long)ctx->num_cu * 16 / (h32.T > 0 ? h32.T : 1) + 1;
+            if (yb > target) yb = target;
+            if (yb > 65535) yb = 65535;
+            if (yb < 1) yb = 1;
+            const long KN = (long)K * N;
+            ScanTimer tm(ctx);
+            if (MAXJ == 32)
+                hipLaunchKernelGGL((k_pwm_best32_mfma<32>), dim3(h32.T, (unsigned)yb), dim3(256), lds32, st, d_bases, L, N,
+                                   dB, dtl, dsk, dsm, h32.T, bval, bpos);
+            else
+                hipLaunchKernelGGL((k_pwm_best32_mfma<64>), dim3(h32.T, (unsigned)yb), dim3(256), lds32, st, d_bases, L, N,
+                                   dB, dtl, dsk, dsm, h32.T, bval, bpos);
+            hipLaunchKernelGGL(k_pwm_best_out, dim3((unsigned)((KN + 255) / 256)), dim3(256), 0, st, dso, d_thresh, K, N,
+                               bval, bpos, o_pos, o_comp, o_score);
+            const hipError_t e = hipGetLastError();
+            tm.stop();
+            if (e != hipSuccess) s = cdlm_fail(ctx, CDLM_ERR_HIP, "k_pwm_best32_mfma: %s", hipGetErrorString(e));
+        }
+        (void)hipStreamSynchronize(st);
+        for (void *q : {(void *)dB, (void *)dtl, (void *)dsk, (void *)dsm, (void *)dso, (void *)bval, (void *)bpos})
+            if (q) (void)hipFree(q);
+        return s;
+    }
     HostTiles ht;
     build_tiles(pwms, lens, thresh, K, maxlen, nuc ? 1 : 0, ht);
     DevTiles dt;
