@@ -406,11 +406,11 @@ def pwm_leg(n_seq: int, log):
         % (n_seq, PWM_L, PWM_K, k_ms, tf, len(h), wall))
     # the best-window scans on the same workload: argmax of the thresholded position-major
     # scores, forward strand (scan_w_gpu!), and best hit per strand pair with nucleotide-major
-    # sums (overlapping_scan!, the workload's best-hit mode); MFMA flops: one pass for the
-    # argmax, four (one per base) per strand for the best hit
+    # sums (overlapping_scan!, the workload's best-hit mode); algorithmic flops as for the hit
+    # list, one product per base and position: 2 x 4 m_k per window and strand
     from cdlm import scan as S
     modes = {}
-    for name, fn, passes, strands in (("argmax", S.scan_w_gpu, 1, 1), ("best_hit", S.overlapping_scan, 4, 2)):
+    for name, fn, passes, strands in (("argmax", S.scan_w_gpu, 1, 1), ("best_hit", S.overlapping_scan, 1, 2)):
         mms = []
         for _ in range(3):
             fn(bases, pwms, lens, thresh)
