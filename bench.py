@@ -35,11 +35,13 @@ HBM_PEAK_GBS = 8000.0
 
 
 def csc_bytes(N, Lg, M, iters):
-    """Algorithmic HBM bytes of one CSC launch of `iters` iterations: per iteration X and y2
-    read and written (4M) and y3 read and written off the stride-4 support (1.5M; on the
-    support y3 is identically zero), per (l, n), 4 bytes each; once per launch y1, Y1, DX, h
-    and the data read and h, DX, y1, Y1 written (9), which stay on chip between iterations."""
-    return 4.0 * N * Lg * (5.5 * M * iters + 9)
+    """Algorithmic HBM bytes of one CSC launch of `iters` iterations, per (l, n), 4 bytes each:
+    the state enters as Xt and y2 (2M) and y3 off the stride-4 support (0.75M; on the support
+    y3 is identically zero) and leaves the same way; between two iterations of the launch it
+    is carried as the two shrink/projection inputs u2 = X + y2 and u3 = X + y3 (2M written,
+    2M read).  So 5.5M + 4M (iters - 1) in all; once per launch y1, Y1, DX, h and the data
+    read and h, DX, y1, Y1 written (9), which stay on chip between iterations."""
+    return 4.0 * N * Lg * ((5.5 + 4.0 * (iters - 1)) * M + 9)
 
 
 def cpu_baseline(sample_n: int):

@@ -132,6 +132,29 @@ __device__ __forceinline__ void st3(float *p, int s, const float *v) {
     q[2] = v[3];
 }
 
+// Between the iterations of one k_csc launch a slot's state is carried as the two shrink /
+// projection inputs u2 = X + y2 (in the y2 buffer) and u3 = X + y3 (in the Xt buffer; X on
+// the stride-4 support, where y3 = 0), 8 floats per slot instead of Xt, y2, y3 (11).  This
+// recovers, with the same float operations the iteration used, the Xt, y2 and y3 it would
+// have stored: y2 = u2 - shrink(u2), y3 = u3 off the support, Xt = v / c0 with
+// v = (Y2 - y2) + (Y3 - y3), bit for bit (the product is kept out of any contraction).
+__device__ __forceinline__ void csc_unpack(float4 &xt, float4 &y2, float4 &y3, float lam, float inv_c0) {
+    float a[4], b[4], c[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float t = f4(y2, q), u3 = f4(xt, q);
+        const float at = fabsf(t) - lam;
+        const float Y2 = at > 0.f ? copysignf(at, t) : 0.f;
+        const float Y3 = (q == 0) ? u3 : 0.f;
+        b[q] = t - Y2;
+        c[q] = (q == 0) ? 0.f : u3;
+        a[q] = __fmul_rn((Y2 - b[q]) + (Y3 - c[q]), inv_c0);
+    }
+    xt = mk4(a);
+    y2 = mk4(b);
+    y3 = mk4(c);
+}
+
 // X on the 4 samples of slot s: Xt + sum_j g[j] * h[l + j] (circular; hb4 holds h plus a
 // wrapped tail of TAPS+4 samples, as float4).
 template <int TAPS>
@@ -333,7 +356,10 @@ __global__ __launch_bounds__(kThreads) void k_init(const CdlDev *__restrict__ ta
 // `iters` iterations of code_update! (train.jl:5-43) for one sequence.  The iterations of
 // different sequences are independent (the dictionary spectrum is fixed during code_update!),
 // so a work-group runs all of its sequence's iterations back to back and h stays in LDS
-// between them (written to HBM after the last).  WAVES waves per work-group; wave w owns the
+// between them (written to HBM after the last).  Between two iterations of the launch the
+// streamed state is the pair (u2, u3) of csc_unpack, 8 floats per slot, in the Xt and y2
+// buffers: the first iteration reads (Xt, y2, y3) and the last writes them, so every other
+// kernel sees the usual state.  WAVES waves per work-group; wave w owns the
 // columns m = w, w + WAVES, ...; a lane owns the float4 slots s = lane + 64 i (i < NSL) of a
 // column, the same in every iteration.  With WAVES = 4 (three work-groups per CU) column
 // m + WAVES's loads are issued before column m is processed (two register sets), and so
@@ -400,18 +426,24 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
     float4 *zacc4 = zbase4 + w * (Lg / 4);
     float4 nx[kPf ? NSL : 2], n2[kPf ? NSL : 2], n3[kPf ? NSL : 2];
     bool have_next = false;       // without prefetch: the first two slots of column w are loaded
-    auto load_col = [&](int m, float4 *a, float4 *b, float4 *c) {
+    // packed: the slots hold (u3, u2) (every iteration after a launch's first), so y3 is not read
+    auto load_col = [&](int m, float4 *a, float4 *b, float4 *c, bool packed) {
         const size_t col = ((size_t)n * M + m) * Lg, col3 = col3_of(n, m, M, nslot);
 #pragma unroll
         for (int i = 0; i < NSL; ++i) {
             const int s = lane + 64 * i;
-            if (s < nslot) { a[i] = ld4(P.Xt + col, s); b[i] = ld4(P.y2 + col, s); c[i] = ld3(P.y3 + col3, s); }
+            if (s < nslot) {
+                a[i] = ld4(P.Xt + col, s);
+                b[i] = ld4(P.y2 + col, s);
+                c[i] = packed ? make_float4(0.f, 0.f, 0.f, 0.f) : ld3(P.y3 + col3, s);
+            }
         }
     };
-    if (kPf && w < M) load_col(w, nx, n2, n3);
+    if (kPf && w < M) load_col(w, nx, n2, n3, false);
     const int nw = M < WAVES ? M : WAVES;       // waves that own columns (and z partials)
 #pragma nounroll
   for (int it = 0; it < iters; ++it) {
+    const bool rdU = it > 0, wrU = it + 1 < iters;   // (u2, u3) read / written instead of (Xt, y2, y3)
     __syncthreads();
     for (int m = w; m < M; m += WAVES) {
         const int mm = __builtin_amdgcn_readfirstlane(m);
@@ -420,7 +452,7 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
         if constexpr (kPf) {
 #pragma unroll
             for (int i = 0; i < NSL; ++i) { xt[i] = nx[i]; a2[i] = n2[i]; a3[i] = n3[i]; }
-            if (mm + WAVES < M) load_col(mm + WAVES, nx, n2, n3);
+            if (mm + WAVES < M) load_col(mm + WAVES, nx, n2, n3, rdU);
         }
         float g[TAPS];
         load_taps<TAPS>(G1, mm, Lg, g);
@@ -435,7 +467,11 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
 #pragma unroll
                     for (int u = 0; u < kSg; ++u) {
                         const int s = lane + 64 * (i0 + u);
-                        if (s < nslot) { lx[u] = ld4(P.Xt + col, s); l2[u] = ld4(P.y2 + col, s); l3[u] = ld3(P.y3 + col3, s); }
+                        if (s < nslot) {
+                            lx[u] = ld4(P.Xt + col, s);
+                            l2[u] = ld4(P.y2 + col, s);
+                            l3[u] = rdU ? make_float4(0.f, 0.f, 0.f, 0.f) : ld3(P.y3 + col3, s);
+                        }
                     }
                 }
             }
@@ -444,12 +480,13 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
                 const int i = i0 + u;
                 const int s = lane + 64 * i;
                 if (s < nslot) {
-                    const float4 xv = kPf ? xt[kPf ? i : 0] : lx[u];
-                    const float4 b2 = kPf ? a2[kPf ? i : 0] : l2[u];
-                    const float4 b3 = kPf ? a3[kPf ? i : 0] : l3[u];
+                    float4 xv = kPf ? xt[kPf ? i : 0] : lx[u];
+                    float4 b2 = kPf ? a2[kPf ? i : 0] : l2[u];
+                    float4 b3 = kPf ? a3[kPf ? i : 0] : l3[u];
+                    if (rdU) csc_unpack(xv, b2, b3, lam, inv_c0);
                     float X[4];
                     recon4<TAPS>(xv, hb4, s, g, X);
-                    float y2n[4], y3n[4], v[4], xn[4];
+                    float u2[4], y2n[4], y3n[4], v[4], xn[4];
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const float x = X[q];
@@ -457,14 +494,21 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
                         const float at = fabsf(t) - lam;
                         const float Y2 = at > 0.f ? copysignf(at, t) : 0.f;
                         const float Y3 = (q == 0) ? x : 0.f;                  // projectionX! (:73-77), y3 = 0 there
-                        y2n[q] = f4(b2, q) + x - Y2;                          // train.jl:13
+                        u2[q] = t;
+                        y2n[q] = t - Y2;                                      // train.jl:13, (y2 + X) - Y2
                         y3n[q] = (q == 0) ? 0.f : f4(b3, q) + x;              // train.jl:14
                         v[q] = (Y2 - y2n[q]) + (Y3 - y3n[q]);
-                        xn[q] = v[q] * inv_c0;
+                        xn[q] = __fmul_rn(v[q], inv_c0);
                     }
-                    st4(P.y2 + col, s, mk4(y2n));
-                    st3(P.y3 + col3, s, y3n);
-                    st4(P.Xt + col, s, mk4(xn));
+                    if (wrU) {
+                        const float u3[4] = {X[0], y3n[1], y3n[2], y3n[3]};
+                        st4(P.y2 + col, s, mk4(u2));
+                        st4(P.Xt + col, s, mk4(u3));
+                    } else {
+                        st4(P.y2 + col, s, mk4(y2n));
+                        st3(P.y3 + col3, s, y3n);
+                        st4(P.Xt + col, s, mk4(xn));
+                    }
                     vb4[TAPS / 4 + s] = mk4(v);
                 }
             }
@@ -484,13 +528,13 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
     // it with prefetch, its first two slots without)
     if (it + 1 < iters && w < M) {
         if constexpr (kPf) {
-            load_col(w, nx, n2, n3);
+            load_col(w, nx, n2, n3, true);
         } else {
-            const size_t col = ((size_t)n * M + w) * Lg, col3 = col3_of(n, w, M, nslot);
+            const size_t col = ((size_t)n * M + w) * Lg;
 #pragma unroll
             for (int u = 0; u < kSg; ++u) {
                 const int s = lane + 64 * u;
-                if (s < nslot) { nx[u] = ld4(P.Xt + col, s); n2[u] = ld4(P.y2 + col, s); n3[u] = ld3(P.y3 + col3, s); }
+                if (s < nslot) { nx[u] = ld4(P.Xt + col, s); n2[u] = ld4(P.y2 + col, s); n3[u] = make_float4(0.f, 0.f, 0.f, 0.f); }
             }
             have_next = true;
         }
