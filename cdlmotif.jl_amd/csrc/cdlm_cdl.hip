@@ -88,10 +88,11 @@ constexpr int kTraceCap = 4096;
 
 size_t lds_bytes_csc(int Lg, int taps, int waves = kWaves) {
     return sizeof(float) * (2 * (size_t)Lg + (size_t)(Lg + taps + 4) + waves * (size_t)(Lg + taps) + waves * (size_t)Lg +
-                            6 * (size_t)Lg);
+                            4 * (size_t)Lg);
 }
 // CSC waves per work-group: 16 (one sequence per CU, no register prefetch) where the slots
-// fit 4 per lane and the LDS fits, else 4; a tuning of 4 or 16 overrides when it fits.
+// fit 4 per lane and the LDS fits, else 4; a tuning of 4, 8 (two sequences per CU where two
+// work-groups' LDS fits) or 16 overrides when it fits.
 int pick_csc_waves(int Lg, int taps, int nsl, int tuned) {
     const bool ok8 = nsl == 4 && lds_bytes_csc(Lg, taps, 8) <= kLdsLimit;
     const bool ok16 = nsl == 4 && lds_bytes_csc(Lg, taps, 16) <= kLdsLimit;
@@ -932,9 +933,8 @@ ShapeClass shape_class(const cdlm_problem *p) {
         c.B = p->tune.B;
     c.taps = p->gamma;
     c.waves = pick_csc_waves(p->Lg, p->gamma, c.nsl, p->tune.waves);
-    c.occ_csc = c.waves >= 8 ? 1
-                              : std::min<int>(c.nsl == 4 && c.taps <= 20 ? 3 : 2,
-                                              (int)(kLdsLimit / lds_bytes_csc(p->Lg, p->gamma, c.waves)));
+    c.occ_csc = std::min<int>(c.waves == 16 ? 1 : c.waves == 8 ? 2 : (c.nsl == 4 && c.taps <= 20 ? 3 : 2),
+                              (int)(kLdsLimit / lds_bytes_csc(p->Lg, p->gamma, c.waves)));
     c.occ_dict = c.B > 0 ? dict_wg_per_cu(p->Lg, p->gamma, c.B) : 0;
     return c;
 }

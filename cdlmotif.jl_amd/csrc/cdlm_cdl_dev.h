@@ -362,22 +362,22 @@ __global__ __launch_bounds__(kThreads) void k_init(const CdlDev *__restrict__ ta
 // kernel sees the usual state.  WAVES waves per work-group; wave w owns the
 // columns m = w, w + WAVES, ...; a lane owns the float4 slots s = lane + 64 i (i < NSL) of a
 // column, the same in every iteration.  With WAVES = 4 (three work-groups per CU) column
-// m + WAVES's loads are issued before column m is processed (two register sets), and so
-// with WAVES = 8 (one work-group per CU by registers); with WAVES = 16 (one work-group per
-// CU, at most 128 VGPRs) there is no prefetch and the 16 waves hide the latency instead.
-// With one sequence per CU resident, a sequence's state (Xt, y2, y3: 0.6 MB at Lg = 840,
-// M = 64) is largely re-read from the last-level cache in the next iteration rather than
-// from HBM.
+// m + WAVES's loads are issued before column m is processed (two register sets); with
+// WAVES = 8 (two work-groups per CU where the LDS allows, so that one streams while the other
+// runs its per-sequence transforms) and WAVES = 16 (one work-group per CU) there is no
+// prefetch, at most 128 VGPRs, and the waves hide the latency instead.  With one or two
+// sequences per CU resident, a sequence's state (u2, u3: 0.43 MB at Lg = 840, M = 64) is
+// largely re-read from the last-level cache in the next iteration rather than from HBM.
 // The (Lg) vectors y1, Y1, DX and the data window stay in LDS for the whole launch, and the
 // next iteration's first column is loaded before the per-sequence transforms.
 // LDS (float4 units): twiddles (Lg/2) | h with wrapped tail | WAVES x v column with wrapped
-// head | WAVES x z partial | y1, Y1, DX, data window, s_k, c_k (Lg each); the FFT ping-pong buffers
+// head | WAVES x z partial | y1, Y1, DX, data window (Lg each); the FFT ping-pong buffers
 // reuse the v columns after the column loop.
 template <int WAVES, int NSL, int TAPS>
 struct CscShape {
     static constexpr int kThr = 64 * WAVES;
-    static constexpr bool kPrefetch = WAVES <= 8;
-    static constexpr int kWavesPerSimd = WAVES <= 4 ? (NSL <= 4 && TAPS <= 20 ? 3 : 2) : WAVES / 4;
+    static constexpr bool kPrefetch = WAVES <= 4;
+    static constexpr int kWavesPerSimd = WAVES <= 4 ? (NSL <= 4 && TAPS <= 20 ? 3 : 2) : 4;
 };
 
 template <int TAPS, int NSL, int WAVES>
@@ -402,8 +402,6 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
     float *Y1s = y1s + Lg;
     float *DXs = Y1s + Lg;
     float *bs = DXs + Lg;
-    float *sSs = bs + Lg;                           // the spectral constants s_k, c_k
-    float *cSs = sSs + Lg;
     float2 *fa = reinterpret_cast<float2 *>(vbase4);
     float2 *fb = fa + Lg;
     const size_t ro = (size_t)n * Lg;
@@ -414,8 +412,6 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
         DXs[l] = gp(P.DX)[ro + l];
         const bool win = (l >= P.gamma) && (l < P.gamma + P.L4);
         bs[l] = win ? gp(P.s)[(size_t)n * P.L4 + (l - P.gamma)] : 0.f;
-        sSs[l] = gp(P.sS)[l];
-        cSs[l] = gp(P.cS)[l];
     }
     load_h<TAPS>(P, ro, hb, tid, kThr);
     if (n == 0 && tid == 0) P.sc->xmat = 0;   // this launch leaves h != 0
@@ -558,8 +554,8 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
         const float2 Ck = C[k], Cm = C[k == 0 ? 0 : Lg - k];
         const float2 Z = make_float2(0.5f * (Ck.x + Cm.x), 0.5f * (Ck.y - Cm.y));   // F(z)
         const float2 U = make_float2(0.5f * (Ck.y + Cm.y), -0.5f * (Ck.x - Cm.x));  // F(u)
-        const float ss = sSs[k];
-        const float den = c0 * cSs[k];
+        const float ss = gp(P.sS)[k];             // the spectral constants s_k, c_k (read once per
+        const float den = c0 * gp(P.cS)[k];       // iteration, from the L2 cache)
         const float2 hh = make_float2((c0 * U.x - Z.x) / den, (c0 * U.y - Z.y) / den);
         const float2 dx = make_float2(Z.x * inv_c0 + ss * hh.x, Z.y * inv_c0 + ss * hh.y);
         O[k] = make_float2(hh.x - dx.y, hh.y + dx.x);   // h_hat + i DX_hat
