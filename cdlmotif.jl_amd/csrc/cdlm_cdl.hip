@@ -1258,7 +1258,9 @@ extern "C" cdlm_status cdlm_autotune(cdlm_ctx *ctx, int32_t L4, int32_t M, int32
     if (!ctx) return CDLM_ERR_ARG;
     cdlm_status st = tune_args(ctx, L4, M, gamma);
     if (st != CDLM_OK) return st;
-    if (n_seq <= 0) n_seq = 3 * (ctx->num_cu > 0 ? ctx->num_cu : 256);
+    // six per CU: whole rounds for one, two and three resident work-groups per CU, so that no
+    // candidate is timed with a part-empty last round
+    if (n_seq <= 0) n_seq = 6 * (ctx->num_cu > 0 ? ctx->num_cu : 256);
     const int Lg = L4 + 2 * gamma;
     // scratch data: one-hot columns from a fixed LCG
     std::vector<float> s((size_t)L4 * n_seq, 0.f);

@@ -616,17 +616,26 @@ __global__ __launch_bounds__(kThreads) void k_best(const CdlDev *__restrict__ ta
         float g[TAPS];
         load_taps<TAPS>(G1, mm, Lg, g);
         float abf = 0.f;
-        for (int s = lane; s < nslot; s += 64) {
-            float X[4];
-            recon4<TAPS>(ld4(P.Xt + col, s), hb4, s, g, X);
+        for (int sb = lane; sb < nslot; sb += 4 * 64) {   // four slots' loads in flight, then use them in order
+            float4 xv[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                abf += fabsf(X[q]);
-                if (X[q] > 0.f) { cnt += 1.0; sx += X[q]; sxx += (double)X[q] * X[q]; }
+            for (int i = 0; i < 4; ++i)
+                if (sb + 64 * i < nslot) xv[i] = ld4(P.Xt + col, sb + 64 * i);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int s = sb + 64 * i;
+                if (s >= nslot) break;
+                float X[4];
+                recon4<TAPS>(xv[i], hb4, s, g, X);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    abf += fabsf(X[q]);
+                    if (X[q] > 0.f) { cnt += 1.0; sx += X[q]; sxx += (double)X[q] * X[q]; }
+                }
+                if (Xfull) st4(Xfull + col, s, mk4(X));
+                if (mat) st4(P.Xt + col, s, mk4(X));
+                if (s >= s0 && s < s0 + L) gp(xc)[((size_t)n * M + mm) * L + (s - s0)] = X[0];
             }
-            if (Xfull) st4(Xfull + col, s, mk4(X));
-            if (mat) st4(P.Xt + col, s, mk4(X));
-            if (s >= s0 && s < s0 + L) gp(xc)[((size_t)n * M + mm) * L + (s - s0)] = X[0];
         }
         ab += abf;
     }

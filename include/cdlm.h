@@ -255,7 +255,7 @@ typedef struct {
     int32_t csc_slots;
     int32_t csc_waves;
 } cdlm_tuning;
-/* Times every candidate on a scratch problem of n_seq sequences (<= 0: three per CU),
+/* Times every candidate on a scratch problem of n_seq sequences (<= 0: six per CU),
  * keeps the fastest for 15 CSC + 1 dictionary iterations, and returns it in *out (may be NULL). */
 cdlm_status cdlm_autotune(cdlm_ctx *ctx, int32_t L4, int32_t M, int32_t gamma, int32_t n_seq, cdlm_tuning *out);
 /* Sets (t != NULL) or clears (t == NULL) the choices for a shape; get returns those in force. */
