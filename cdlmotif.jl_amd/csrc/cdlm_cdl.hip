@@ -1321,7 +1321,7 @@ extern "C" cdlm_status cdlm_autotune(cdlm_ctx *ctx, int32_t L4, int32_t M, int32
             if (nsl < need || pick_csc_waves(Lg, gamma, nsl, cand_waves[ci]) != cand_waves[ci]) continue;
             Tuning tu;
             tu.order = o; tu.nsl = nsl; tu.waves = cand_waves[ci];
-            if ((st = apply(tu)) == CDLM_OK) st = timed(false, 5, &best_csc[o][ci]);
+            if ((st = apply(tu)) == CDLM_OK) st = timed(false, 20, &best_csc[o][ci]);
         }
         for (int o4 = 0; o4 < kFftOrders && st == CDLM_OK; ++o4)
             for (int bi = 0; bi < kDict && st == CDLM_OK; ++bi) {

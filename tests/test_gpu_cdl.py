@@ -129,6 +129,52 @@ def test_c2_shape_slice_vs_oracle():
     envelope_check("C2-slice", g, o, o64, ("X", "DX", "d", "G1", "G3", "Dbest"))
 
 
+C2_SLICE = os.path.join(os.path.dirname(__file__), "golden", "c2_slice.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(C2_SLICE), reason="c2_slice.npz not generated")
+@pytest.mark.parametrize("reps", [1, 20])
+def test_c2_slice_1000_vs_oracle_fixture(reps):
+    """A 1000-sequence dataset of the configuration-2 shape (L=200, 64 filters of length 20,
+    Lg=840) for the whole schedule, 10 x (20 CSC + best + 1 dict) = 200 ADMM iterations,
+    against the FP32 and FP64 oracle results committed by tests/golden/make_c2_slice.py (the
+    oracle takes ~15 min on the host for this size).
+
+    reps=1 runs the slice as its own problem.  reps=20 runs it tiled 20 times, N=20000 at the
+    full configuration-2 size: every mean over n is then the slice's mean and the objective is
+    exactly 20 times the slice's (checked on the oracle to 1e-15), so the full-size run must
+    track the same fixture, and the copies of a sequence, computed independently by the
+    per-sequence kernels with the same filters, must stay equal bit for bit."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import make_c2_slice as mk
+    ref = np.load(C2_SLICE)
+    s, D0, X0 = mk.inputs()
+    for nm, a in (("s", s), ("D0", D0), ("X0", X0)):
+        assert mk.digest(a) == str(ref["digest_" + nm]), "input %s differs from the fixture's" % nm
+    if reps > 1:
+        s, X0 = np.tile(s, (reps, 1)), np.tile(X0, (reps, 1, 1))
+    tag = "C2-%d" % (mk.N * reps)
+    flush("%s: N=%d" % (tag, s.shape[0]))
+    fl = [mk.FLEN] * mk.M
+    g = cdlm.CDL(s, fl, mk.RHO, mk.SIGMA, mk.LAM, D0=D0, X0=X0)
+    del X0
+    tg = np.asarray(g.optimize(mk.ITER_NUM, mk.CSC_ITER, mk.CDL_ITER)) / reps
+    flush("%s obj trace gpu / %d %s" % (tag, reps, np.array2string(tg, precision=6)))
+    np.testing.assert_allclose(tg, ref["trace_f32"], rtol=TOL)
+    np.testing.assert_allclose(tg, ref["trace_f64"], rtol=TOL)
+    for f in ("G1", "G3", "Dbest", "DX"):
+        a = g.get(f)
+        if f == "DX":
+            for r in range(1, reps):
+                np.testing.assert_array_equal(a[r * mk.N:r * mk.N + mk.N_DX], a[:mk.N_DX], err_msg="copy %d" % r)
+            a = a[:mk.N_DX]
+        b32, b64 = ref[f + "_f32"], ref[f + "_f64"]
+        e_g64, e_3264, e_g32 = rel(a, b64), rel(b32, b64), rel(a, b32)
+        flush("%s %s: gpu-f64 %.2e  f32oracle-f64 %.2e  gpu-f32oracle %.2e" % (tag, f, e_g64, e_3264, e_g32))
+        assert e_g64 <= max(TOL, 1.5 * e_3264), (f, e_g64, e_3264)
+
+
 def test_csc_is_per_sequence_exact():
     """Size-independent property: a CSC iteration touches each sequence alone, so sequences
     5..8 of a 40-sequence problem equal the same 4 sequences run as their own problem, bit

@@ -156,3 +156,31 @@ def test_box_projection():
     v = np.array([[-0.5, 0.0, 0.3, 1.7, 0.4, 2.0]])
     out = O.box(v, P, "f64")
     np.testing.assert_array_equal(out, [[0, 0, 0.3, 1.0, 0, 0]])
+
+
+C2_SLICE = "c2_slice.npz"
+
+
+def test_c2_slice_fixture_is_consistent(golden_dir):
+    """The 1000-sequence, 200-iteration oracle fixture: its inputs rebuild bit for bit from the
+    generators and seeds in make_c2_slice.py (the GPU test rebuilds them the same way), the FP32
+    and FP64 oracle objective traces agree to 1e-4, and the stored filters satisfy the
+    projections (G1, G3 vanish off the support of length 20, G3 in [0, 1])."""
+    path = os.path.join(golden_dir, C2_SLICE)
+    if not os.path.exists(path):
+        pytest.skip("c2_slice.npz not generated")
+    import sys
+    sys.path.insert(0, golden_dir)
+    import make_c2_slice as mk
+    z = np.load(path)
+    s, D0, X0 = mk.inputs()
+    for nm, a in (("s", s), ("D0", D0), ("X0", X0)):
+        assert mk.digest(a) == str(z["digest_" + nm]), nm
+    assert z["trace_f64"].shape == (mk.ITER_NUM,)
+    np.testing.assert_allclose(z["trace_f32"], z["trace_f64"], rtol=1e-4)
+    for p in ("f32", "f64"):
+        G1, G3 = z["G1_" + p], z["G3_" + p]
+        assert G1.shape == (mk.M, 4 * 200 + 2 * mk.FLEN)
+        assert np.all(G1[:, mk.FLEN:] == 0) and np.all(G3[:, mk.FLEN:] == 0)
+        assert G3.min() >= 0 and G3.max() <= 1
+        assert z["DX_" + p].shape == (mk.N_DX, G1.shape[1])
