@@ -188,6 +188,32 @@ def test_csc_is_per_sequence_exact():
         np.testing.assert_array_equal(a.get(f)[5:9], b.get(f), err_msg=f)
 
 
+@pytest.mark.parametrize("waves", [4, 8, 16])
+def test_csc_launch_split_is_exact(waves):
+    """One launch of 12 CSC iterations carries the state between its iterations as
+    (X + y2, X + y3) and rebuilds Xt, y2, y3 from it; 12 launches of one iteration each store
+    (Xt, y2, y3) after every iteration.  Both run the same float operations, so every field
+    must agree bit for bit, for each CSC work-group shape, and so must the next dictionary step."""
+    fl = [20] * 12
+    s, D0, X0, _ = _synth(6, 24, 200, 12, fl)
+    L, M = 200, len(fl)
+    cdlm.tuning_set(L, M, 20, {"fft_order": 0, "fft4_order": 0, "dict_batch": 2, "csc_slots": 4, "csc_waves": waves})
+    try:
+        a = cdlm.CDL(s, fl, 1.0, 100.0, 0.05, D0=D0, X0=X0)
+        b = cdlm.CDL(s, fl, 1.0, 100.0, 0.05, D0=D0, X0=X0)
+    finally:
+        cdlm.tuning_set(L, M, 20, None)
+    a.code_update(12)
+    for _ in range(12):
+        b.code_update(1)
+    for f in ("X", "DX", "y1", "Y1", "y2", "y3"):
+        np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
+    assert a.update_best() == b.update_best()
+    a.dict_update(1); b.dict_update(1)
+    for f in ("d", "G1", "G3", "g1", "g3"):
+        np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
+
+
 def test_projection_invariants():
     """y3 is exactly 0 on the stride-4 support after a CSC iteration (y3 += X - Pi4(X + y3)),
     G1 and G3 vanish off the filter support, G3 lies in [0, 1]."""
