@@ -64,7 +64,14 @@ def cpu_baseline(sample_n: int):
     oc.dict_update(CDL_ITER)
     dt = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": sample_n * CSC_ITER / dt, "unit": "seq*iters/s", "cores": threads, "kind": "port",
+            "host_cpus": os.cpu_count(), "cpu_model": model,
             "sample": "oracle f32 reference-structured CDL on %d sequences of the same shape (L=%d, M=%d, "
                       "filter_len=%d), one outer iteration (%d CSC + update_best + %d dict) in %.1f s"
                       % (sample_n, L_BASES, M_FILT, FLEN, CSC_ITER, CDL_ITER, dt)}
