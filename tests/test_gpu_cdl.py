@@ -8,7 +8,7 @@ Errors are relative norms |a-b|_2 / |b|_2 per array.  Tolerances:
   * arrays after long schedules (100-200 iterations): GPU vs FP64 oracle
     <= max(1e-4, 1.5 x (FP32 oracle vs FP64 oracle)).  The iteration amplifies rounding:
     the reference-structured FP32 computation itself ends ~1e-3 away from FP64 on the
-    configuration-2 shape, so two FP32 implementations with different operation orders
+    single-dataset shape (L=200, M=64, filter_len=20), so two FP32 implementations with different operation orders
     cannot agree to 1e-4 there; the gate requires the GPU to be as accurate as the
     reference's own FP32 arithmetic.  All three errors are printed."""
 import os
@@ -88,7 +88,7 @@ def test_single_steps_match_oracle():
 
 
 def test_c1_like_full_schedule_vs_oracle():
-    """Configuration-1 shape (L=100, M=10, filter_len=16, lambda=0.05, rho=1) with N=60,
+    """The small CPU-runnable workload's shape (L=100, M=10, filter_len=16, lambda=0.05, rho=1) with N=60,
     10 outer x (10 CSC + best + 1 dict) = 100 ADMM iterations."""
     fl = [16] * 10
     s, D0, X0, _ = _synth(1, 60, 100, 10, fl)
@@ -110,7 +110,7 @@ def test_c1_like_full_schedule_vs_oracle():
 
 
 def test_c2_shape_slice_vs_oracle():
-    """Configuration-2 shape per sequence (L=200, M=64, filter_len=20, Lg=840) on a
+    """Single-dataset shape per sequence (L=200, M=64, filter_len=20, Lg=840) on a
     24-sequence slice, 10 outer x (20 CSC + best + 1 dict) = 200 ADMM iterations."""
     fl = [20] * 64
     s, D0, X0, _ = _synth(2, 24, 200, 64, fl)
@@ -135,13 +135,13 @@ C2_SLICE = os.path.join(os.path.dirname(__file__), "golden", "c2_slice.npz")
 @pytest.mark.skipif(not os.path.exists(C2_SLICE), reason="c2_slice.npz not generated")
 @pytest.mark.parametrize("reps", [1, 20])
 def test_c2_slice_1000_vs_oracle_fixture(reps):
-    """A 1000-sequence dataset of the configuration-2 shape (L=200, 64 filters of length 20,
+    """A 1000-sequence dataset of the single-dataset shape (L=200, 64 filters of length 20,
     Lg=840) for the whole schedule, 10 x (20 CSC + best + 1 dict) = 200 ADMM iterations,
     against the FP32 and FP64 oracle results committed by tests/golden/make_c2_slice.py (the
-    oracle takes ~15 min on the host for this size).
+    oracle takes ~45 min on 8 host cores for this size).
 
     reps=1 runs the slice as its own problem.  reps=20 runs it tiled 20 times, N=20000 at the
-    full configuration-2 size: every mean over n is then the slice's mean and the objective is
+    full single-dataset size: every mean over n is then the slice's mean and the objective is
     exactly 20 times the slice's (checked on the oracle to 1e-15), so the full-size run must
     track the same fixture, and the copies of a sequence, computed independently by the
     per-sequence kernels with the same filters, must stay equal bit for bit."""
