@@ -245,9 +245,10 @@ cdlm_status cdlm_server_stats(cdlm_server *srv, cdlm_server_stats_t *out);
  * (2, 4 or 8; 0 = by occupancy; it changes the order of the sum over waves, hence rounding);
  * csc_slots: CSC float4 slots per lane (4 or 8; 0 = the fewest that cover Lg);
  * csc_waves: CSC waves per work-group (4: three sequences per CU with register prefetch;
- * 8 or 16: one sequence per CU, which keeps the resident sequences' state in the last-level
- * cache between the CSC iterations of one launch; 0 = 16 where it fits, else 4; it changes
- * the order of the sum over waves, hence rounding). */
+ * 8: two sequences per CU where their LDS fits, one streaming while the other transforms;
+ * 16: one sequence per CU; with 8 and 16 the resident sequences' state stays in the
+ * last-level cache between the CSC iterations of one launch; 0 = 16 where it fits, else 4;
+ * it changes the order of the sum over waves, hence rounding). */
 typedef struct {
     int32_t fft_order;
     int32_t fft4_order;
