@@ -90,16 +90,17 @@ size_t lds_bytes_csc(int Lg, int taps, int waves = kWaves) {
     return sizeof(float) * (2 * (size_t)Lg + (size_t)(Lg + taps + 4) + waves * (size_t)(Lg + taps) + waves * (size_t)Lg +
                             4 * (size_t)Lg);
 }
-// CSC waves per work-group: 16 (one sequence per CU, no register prefetch) where the slots
-// fit 4 per lane and the LDS fits, else 4; a tuning of 4, 8 (two sequences per CU where two
-// work-groups' LDS fits) or 16 overrides when it fits.
+// CSC waves per work-group where the slots fit 4 per lane: 8 when two such work-groups' LDS
+// fits a CU (two sequences per CU, no register prefetch), else 16 when its LDS fits (one
+// sequence per CU), else 4; a tuning of 4, 8 or 16 overrides when it fits.
 int pick_csc_waves(int Lg, int taps, int nsl, int tuned) {
     const bool ok8 = nsl == 4 && lds_bytes_csc(Lg, taps, 8) <= kLdsLimit;
+    const bool two8 = nsl == 4 && 2 * lds_bytes_csc(Lg, taps, 8) <= kLdsLimit;
     const bool ok16 = nsl == 4 && lds_bytes_csc(Lg, taps, 16) <= kLdsLimit;
     if (tuned == 8) return ok8 ? 8 : 4;
     if (tuned == 16) return ok16 ? 16 : 4;
     if (tuned == 4) return 4;
-    return ok16 ? 16 : 4;
+    return two8 ? 8 : ok16 ? 16 : 4;
 }
 size_t lds_bytes_init(int Lg, int taps) { return sizeof(float) * (kWaves * (size_t)(Lg + taps) + kWaves * (size_t)Lg); }
 size_t lds_bytes_best(int Lg, int taps) { return sizeof(float) * (size_t)(Lg + taps + 4); }

@@ -247,7 +247,8 @@ cdlm_status cdlm_server_stats(cdlm_server *srv, cdlm_server_stats_t *out);
  * csc_waves: CSC waves per work-group (4: three sequences per CU with register prefetch;
  * 8: two sequences per CU where their LDS fits, one streaming while the other transforms;
  * 16: one sequence per CU; with 8 and 16 the resident sequences' state stays in the
- * last-level cache between the CSC iterations of one launch; 0 = 16 where it fits, else 4;
+ * last-level cache between the CSC iterations of one launch; 0 = 8 where two work-groups'
+ * LDS fits a CU, else 16 where one fits, else 4;
  * it changes the order of the sum over waves, hence rounding). */
 typedef struct {
     int32_t fft_order;
