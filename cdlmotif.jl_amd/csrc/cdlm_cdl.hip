@@ -231,7 +231,7 @@ BufPlan buf_plan(int L4, int N, int M, int gamma, bool keep_x0) {
     const size_t Lg = (size_t)L4 + 2 * (size_t)gamma;
     b.s = (size_t)L4 * N * sizeof(float);
     b.E = Lg * M * (size_t)N * sizeof(float);
-    b.Eh = (size_t)half_bins((int)Lg) * M * (size_t)N * sizeof(float2);   // d, g1, g3 as half spectra
+    b.Eh = (size_t)half_bins((int)Lg) * M * (size_t)N * sizeof(float2);   // d, g1 as half spectra
     b.E3 = 3 * (Lg / 4) * M * (size_t)N * sizeof(float);   // y3 off the stride-4 support
     b.Ex0 = keep_x0 ? b.E : 0;
     b.LN = Lg * (size_t)N * sizeof(float);
@@ -242,14 +242,14 @@ BufPlan buf_plan(int L4, int N, int M, int gamma, bool keep_x0) {
     b.y3s = (size_t)M * N * (Lg / 8 + 1) * sizeof(float2);
     b.M = (size_t)M * sizeof(int);
     b.partials = (size_t)N * kPartStride * sizeof(double);
-    b.dpart = (size_t)pick_nch((int)Lg, M, N) * 6 * ((size_t)half_bins((int)Lg) * M) * sizeof(double);
+    b.dpart = (size_t)pick_nch((int)Lg, M, N) * kDpart * ((size_t)half_bins((int)Lg) * M) * sizeof(double);
     b.trace = (size_t)kTraceCap * sizeof(double);
     b.sc = sizeof(CdlScal);
     return b;
 }
 
 size_t plan_total(const BufPlan &b) {
-    return b.s + 2 * b.E + 3 * b.Eh + b.E3 + b.Ex0 + 7 * b.LN + 7 * b.LM + 3 * b.LMh + 2 * b.Lg /* cS, sS */ +
+    return b.s + 2 * b.E + 2 * b.Eh + b.E3 + b.Ex0 + 7 * b.LN + 7 * b.LM + 4 * b.LMh + 2 * b.Lg /* cS, sS */ +
            2 * b.Lg /* tw */ + 2 * b.xs + b.y3s + b.M + b.partials + b.dpart + b.trace + b.sc;
 }
 
@@ -626,12 +626,12 @@ cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const flo
     float2 *tw_dev = nullptr;
 #define A(ptr, bytes) do { st = palloc_t(p.get(), &(ptr), (bytes)); if (st != CDLM_OK) { free_problem(p.get()); return st; } } while (0)
     A(s_dev, b.s);
-    A(d.Xt, b.E); A(d.y2, b.E); A(d.y3, b.E3); A(d.dh, b.Eh); A(d.g1h, b.Eh); A(d.g3h, b.Eh);
+    A(d.Xt, b.E); A(d.y2, b.E); A(d.y3, b.E3); A(d.dh, b.Eh); A(d.g1h, b.Eh);
     if (X0) A(X0_dev, b.Ex0);
     A(d.h, b.LN); A(d.y1, b.LN); A(d.Y1, b.LN); A(d.DX, b.LN); A(d.G2, b.LN); A(d.g2, b.LN); A(d.Td, b.LN);
     A(d.G1b[0], b.LM); A(d.G1b[1], b.LM); A(d.G3, b.LM); A(d.meand, b.LM); A(d.Dbest, b.LM); A(d.gsq, b.LM);
     A(D0_dev, b.LM);
-    A(d.G1hb[0], b.LMh); A(d.G1hb[1], b.LMh); A(d.G3h, b.LMh);
+    A(d.G1hb[0], b.LMh); A(d.G1hb[1], b.LMh); A(d.G3h, b.LMh); A(d.d13h, b.LMh);
     A(d.cS, b.Lg); A(d.sS, b.Lg);
     A(tw_dev, 2 * b.Lg);
     A(d.xs[0], b.xs); A(d.xs[1], b.xs);
@@ -822,12 +822,13 @@ extern "C" cdlm_status cdlm_get(cdlm_problem *p, int32_t field, void *dst, size_
     case CDLM_F_g3: {
         // held as half spectra: back to the spatial domain on the device first
         if (bytes != E * sizeof(float)) return cdlm_fail(p->ctx, CDLM_ERR_ARG, "field %d needs %zu bytes", field, E * sizeof(float));
-        const float2 *h = field == CDLM_F_D ? d.dh : field == CDLM_F_g1 ? d.g1h : d.g3h;
+        const float2 *h = field == CDLM_F_D ? d.dh : d.g1h;
+        const float2 *sub = field == CDLM_F_g3 ? d.d13h : nullptr;   // g3 = g1 - (g1 - g3)
         float *tmp = nullptr;
         hipError_t e = hipMalloc(&tmp, E * sizeof(float));
         if (e != hipSuccess) { (void)hipGetLastError(); return cdlm_fail(p->ctx, CDLM_ERR_OOM, "no memory for field %d", field); }
         hipLaunchKernelGGL(k_half_to_spatial, dim3((unsigned)((size_t)p->M * p->N)), dim3(kThreads), lds_bytes_half(p->Lg), st,
-                           p->set.tab, h, tmp);
+                           p->set.tab, h, sub, tmp);
         cdlm_status s = lcheck(p->ctx, "k_half_to_spatial");
         hipError_t e2 = s == CDLM_OK ? hipMemcpyAsync(dst, tmp, bytes, hipMemcpyDeviceToHost, st) : hipSuccess;
         hipError_t e3 = hipStreamSynchronize(st);

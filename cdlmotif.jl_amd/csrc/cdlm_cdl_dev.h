@@ -19,7 +19,9 @@
 //     dictionary step) X is written back materialised (Xt = X, h = 0) -- by update_best's
 //     pass, which rebuilds all of X anyway, or else by the dictionary step itself.
 //   * the dictionary step holds d, g1 and g3 as half spectra (dh, g1h, g3h, as the reference
-//     keeps dh, g1h, g3h); its solve is
+//     keeps dh, g1h, g3h), g3 through g1 - g3: every sequence's g1 and g3 receive the same
+//     d and differ by the filters only (g1 += d - G1, g3 += d - G3 from g1 = g3 = 0), so
+//     g1 - g3 = sum over the steps of G3 - G1 is one (Hs, M) array; its solve is
 //       dh_m = Qh_m + conj(Y3h_m) kappa,   kappa = (c1 F(G2-g2) - sum_m Y3h_m Qh_m)/(|Y3h|^2 + c1)
 //     with Qh_m = (G1h_m - g1h_m) + (G3h_m - g3h_m) (c1 = sherman_const_cdl; no 1/c1, as
 //     sherman_CDL_combine), and g1h, g3h are updated in the same domain.  Y3 = Pi4(X + y3) is
@@ -36,7 +38,7 @@
 // wrap modulo the problem's own Lg.
 //
 // HBM layout: the reference's column-major (Lg, M, N) with l fastest; one work-group owns
-// one sequence n and streams its contiguous (Lg x M) slab.  dh, g1h, g3h are (Hs, M, N)
+// one sequence n and streams its contiguous (Lg x M) slab.  dh, g1h are (Hs, M, N)
 // complex with Hs = Lg/2 + 1 rounded up to even (bins 0..Lg/2 and a zero pad).  y3 is stored without its
 // stride-4 support samples: there y3 + X - Pi4(X + y3) = (y3 + X) - (X + y3) = 0 exactly
 // (IEEE addition commutes) from the first iteration on, and y3 starts at 0, so the
@@ -51,6 +53,7 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxTaps = 32;
 constexpr int kPartStride = 8;  // doubles per sequence in the objective partials
+constexpr int kDpart = 4;       // doubles per (bin, filter) and sequence chunk in the dictionary means
 
 // Device-side scalar state of one problem.
 struct CdlScal {
@@ -71,7 +74,8 @@ struct CdlDev {
     const float *s;
     const int *fl;
     float *Xt, *y2;                                // (Lg, M, N)
-    float2 *dh, *g1h, *g3h;                        // (Hs, M, N): half spectra of d, g1, g3
+    float2 *dh, *g1h;                              // (Hs, M, N): half spectra of d, g1
+    float2 *d13h;                                  // (Hs, M): g1h - g3h, the same for every sequence
     float *y3;                                     // (3 Lg/4, M, N): y3 off the stride-4 support
     float *h, *y1, *Y1, *DX, *G2, *g2, *Td;       // (Lg, N)
     float *G1b[2], *G3, *meand, *Dbest, *gsq;     // (Lg, M)
@@ -120,6 +124,9 @@ __device__ __forceinline__ float4 ld4(const float *p, int s) { return ((const CD
 __device__ __forceinline__ void st4(float *p, int s, float4 v) { ((CDLM_GLOBAL float4 *)p)[s] = v; }
 __device__ __forceinline__ float f4(const float4 &v, int q) { return q == 0 ? v.x : q == 1 ? v.y : q == 2 ? v.z : v.w; }
 __device__ __forceinline__ float4 mk4(const float *a) { return make_float4(a[0], a[1], a[2], a[3]); }
+__device__ __forceinline__ float4 sub4(const float4 a, const float4 b) {
+    return make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
+}
 // Samples 1..3 of slot s of a y3 column (3 floats per slot).
 __device__ __forceinline__ float4 ld3(const float *p, int s) {
     const CDLM_GLOBAL float *q = gp(p) + 3 * s;
@@ -319,14 +326,14 @@ __global__ __launch_bounds__(kThreads) void k_init(const CdlDev *__restrict__ ta
             st3(P.y3 + col3_of(n, mm, M, nslot), s, reinterpret_cast<const float *>(&zero4));
             vb4[TAPS / 4 + s] = xp;
         }
-        // d = D (dh = F(D), CDL_setup.jl:207-208), g1 = g3 = 0, as half spectra
+        // d = D (dh = F(D), CDL_setup.jl:207-208), g1 = g3 = 0, as half spectra (g1 - g3 = 0)
         {
             const int Hs = half_bins(Lg);
             const size_t colh = ((size_t)n * M + mm) * Hs;
             const float2 *G1h = P.G1hb[P.sc->g1cur & 1] + (size_t)mm * Hs;
             for (int t = lane; t < Hs / 2; t += 64) {
                 st4(reinterpret_cast<float *>(P.g1h + colh), t, zero4);
-                st4(reinterpret_cast<float *>(P.g3h + colh), t, zero4);
+                if (n == 0) st4(reinterpret_cast<float *>(P.d13h + (size_t)mm * Hs), t, zero4);
                 st4(reinterpret_cast<float *>(P.dh + colh), t, ld4(reinterpret_cast<const float *>(G1h), t));
             }
         }
@@ -724,7 +731,8 @@ __global__ __launch_bounds__(kThreads) void k_decide(const CdlDev *__restrict__ 
 //   a block-wide spectral step gives Td = F^-1(sum_m Y3h dh) (:56-57), G2, g2 (update_g2!,
 //     :68-71) and kappa = (c1 F(G2 - g2) - sum_m Y3h Qh) / (|Y3h|^2 + c1).
 //   pass 2, the same columns: dh = Qh + conj(Y3h) kappa (sherman_CDL_combine, :60),
-//     g1h += dh - G1h, g3h += dh - G3h (:70-72 in the transform domain).
+//     g1h += dh - G1h (:70 in the transform domain); g3h = g1h - d13h throughout, and
+//     g3h += dh - G3h (:72) is d13h += G3h - G1h, once per filter in k_dict_finalize.
 // The partial-sum order depends on W, which belongs to the problem's launch choices (a
 // problem runs with the same W alone and in a batch).  Y3h is taken Hermitian from its half
 // spectrum (what pass 1 leaves in y3s).  A lane owns the float4 slots t = lane + 64 i of a
@@ -758,6 +766,7 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
     const float *G1 = g1cur_of(P);
     const float *G1h = reinterpret_cast<const float *>(P.G1hb[P.sc->g1cur & 1]);
     const float *G3h = reinterpret_cast<const float *>(P.G3h);
+    const float *D13h = reinterpret_cast<const float *>(P.d13h);
     const bool mat = !P.sc->xmat;   // set again only by k_cs after this step
     const float4 *hb4 = reinterpret_cast<const float4 *>(hb);
 
@@ -797,7 +806,7 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
             if (t < nq) {
                 D = ld4(reinterpret_cast<const float *>(P.dh + colh), t);
                 const float4 a1 = ld4(reinterpret_cast<const float *>(P.g1h + colh), t);
-                const float4 a3 = ld4(reinterpret_cast<const float *>(P.g3h + colh), t);
+                const float4 a3 = sub4(a1, ld4(D13h + 2 * fcol, t));   // g3h = g1h - (g1h - g3h)
                 const float4 F1 = ld4(G1h + 2 * fcol, t), F3 = ld4(G3h + 2 * fcol, t);
                 Q = make_float4((F1.x - a1.x) + (F3.x - a3.x), (F1.y - a1.y) + (F3.y - a3.y),
                                 (F1.z - a1.z) + (F3.z - a3.z), (F1.w - a1.w) + (F3.w - a3.w));
@@ -910,21 +919,19 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
     }
     __syncthreads();
 
-    // ---- pass 2: dh = Qh + conj(Y3h) kappa; g1h += dh - G1h; g3h += dh - G3h
-    // With prefetch, the next column's g1h, g3h and stored Y3h are loaded before this column's
-    // arithmetic (the filter spectra G1h, G3h come from the cache at the column's start).
+    // ---- pass 2: dh = Qh + conj(Y3h) kappa; g1h += dh - G1h (g3h += dh - G3h lives in
+    // g1h - g3h, which k_dict_finalize advances by G3h - G1h once for all sequences)
+    // With prefetch, the next column's g1h and stored Y3h are loaded before this column's
+    // arithmetic (the filter spectra G1h, G3h, g1h - g3h come from the cache at the column's start).
     constexpr int NY = 2;                       // stored Y3h bins per lane: Ph <= 64 NY (Lg <= 1020)
-    float4 p1[kPf ? NQ : 1], p3[kPf ? NQ : 1];
+    float4 p1[kPf ? NQ : 1];
     float2 py[kPf ? NY : 1];
     auto load_next = [&](int m) {
         const size_t colh = ((size_t)n * M + m) * Hs;
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
             const int t = lane + 64 * i;
-            if (t < nq) {
-                p1[i] = ld4(reinterpret_cast<const float *>(P.g1h + colh), t);
-                p3[i] = ld4(reinterpret_cast<const float *>(P.g3h + colh), t);
-            }
+            if (t < nq) p1[i] = ld4(reinterpret_cast<const float *>(P.g1h + colh), t);
         }
 #pragma unroll
         for (int i = 0; i < NY; ++i) {
@@ -940,13 +947,13 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
         const size_t colh = ((size_t)n * M + mm) * Hs, fcol = (size_t)mm * Hs;
         float *dcol = reinterpret_cast<float *>(P.dh + colh);
         float *a1col = reinterpret_cast<float *>(P.g1h + colh);
-        float *a3col = reinterpret_cast<float *>(P.g3h + colh);
+        // a3v / A3 first hold g1h - g3h, then g3h
         float4 a1v[kPf ? NQ : 1], a3v[kPf ? NQ : 1], f1v[kPf ? NQ : 1], f3v[kPf ? NQ : 1];
         auto load2 = [&](int i, float4 &A1, float4 &A3, float4 &F1, float4 &F3) {
             const int t = lane + 64 * i;
             if (t < nq) {
                 A1 = ld4(a1col, t);
-                A3 = ld4(a3col, t);
+                A3 = ld4(D13h + 2 * fcol, t);
                 F1 = ld4(G1h + 2 * fcol, t);
                 F3 = ld4(G3h + 2 * fcol, t);
             }
@@ -955,9 +962,12 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
 #pragma unroll
             for (int i = 0; i < NQ; ++i) {
                 a1v[i] = p1[i];
-                a3v[i] = p3[i];
                 const int t = lane + 64 * i;
-                if (t < nq) { f1v[i] = ld4(G1h + 2 * fcol, t); f3v[i] = ld4(G3h + 2 * fcol, t); }
+                if (t < nq) {
+                    a3v[i] = ld4(D13h + 2 * fcol, t);
+                    f1v[i] = ld4(G1h + 2 * fcol, t);
+                    f3v[i] = ld4(G3h + 2 * fcol, t);
+                }
             }
 #pragma unroll
             for (int i = 0; i < NY; ++i) {
@@ -975,7 +985,8 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
             if constexpr (kPf) { A1 = a1v[i]; A3 = a3v[i]; F1 = f1v[i]; F3 = f3v[i]; } else { load2(i, A1, A3, F1, F3); }
             const int t = lane + 64 * i;
             if (t < nq) {
-                float dn[4], g1n[4], g3n[4];
+                A3 = sub4(A1, A3);   // g3h = g1h - (g1h - g3h)
+                float dn[4], g1n[4];
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
                     const int k = 2 * t + h;
@@ -990,21 +1001,19 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
                     dn[2 * h + 1] = dk.y;
                     g1n[2 * h] = f4(A1, 2 * h) + dk.x - f4(F1, 2 * h);               // train.jl:70
                     g1n[2 * h + 1] = f4(A1, 2 * h + 1) + dk.y - f4(F1, 2 * h + 1);
-                    g3n[2 * h] = f4(A3, 2 * h) + dk.x - f4(F3, 2 * h);               // train.jl:72
-                    g3n[2 * h + 1] = f4(A3, 2 * h + 1) + dk.y - f4(F3, 2 * h + 1);
                 }
                 st4(dcol, t, mk4(dn));
                 st4(a1col, t, mk4(g1n));
-                st4(a3col, t, mk4(g3n));
             }
         }
         wave_lds_sync();
     }
 }
 
-// Partial sums over a chunk of sequences of (dh + g1h), (dh + g3h) and dh, per (bin, m)
-// (train.jl:76,83, train.jl:95: the means of d + g1, d + g3 and d, taken on the spectra), in
-// double, real and imaginary parts.  Blocks of a problem: (tile, chunk).
+// Partial sums over a chunk of sequences of (dh + g1h) and dh, per (bin, m) (train.jl:76,83,
+// train.jl:95: the means of d + g1, d + g3 and d, taken on the spectra; the mean of d + g3 is
+// that of d + g1 less g1 - g3, which is the same for every sequence), in double, real and
+// imaginary parts.  Blocks of a problem: (tile, chunk).
 __global__ __launch_bounds__(kThreads) void k_dict_reduce(const CdlDev *__restrict__ tab, const int *__restrict__ starts,
                                                           int np) {
     int b;
@@ -1016,24 +1025,20 @@ __global__ __launch_bounds__(kThreads) void k_dict_reduce(const CdlDev *__restri
     if (i >= HM) return;
     const int N = P.N, nch = P.nch;
     const int n0 = (int)((long)c * N / nch), n1 = (int)((long)(c + 1) * N / nch);
-    double a1x = 0.0, a1y = 0.0, a3x = 0.0, a3y = 0.0, adx = 0.0, ady = 0.0;
+    double a1x = 0.0, a1y = 0.0, adx = 0.0, ady = 0.0;
     for (int n = n0; n < n1; ++n) {
         const size_t e = (size_t)n * HM + i;
-        const float2 dv = gp(P.dh)[e], v1 = gp(P.g1h)[e], v3 = gp(P.g3h)[e];
+        const float2 dv = gp(P.dh)[e], v1 = gp(P.g1h)[e];
         a1x += (double)(dv.x + v1.x);
         a1y += (double)(dv.y + v1.y);
-        a3x += (double)(dv.x + v3.x);
-        a3y += (double)(dv.y + v3.y);
         adx += (double)dv.x;
         ady += (double)dv.y;
     }
-    double *o = gp(P.dpart) + (size_t)c * 6 * HM + i;
+    double *o = gp(P.dpart) + (size_t)c * kDpart * HM + i;
     o[0 * (size_t)HM] = a1x;
     o[1 * (size_t)HM] = a1y;
-    o[2 * (size_t)HM] = a3x;
-    o[3 * (size_t)HM] = a3y;
-    o[4 * (size_t)HM] = adx;
-    o[5 * (size_t)HM] = ady;
+    o[2 * (size_t)HM] = adx;
+    o[3 * (size_t)HM] = ady;
 }
 
 // Column m: the mean spectra of (d+g1), (d+g3) and d back to the spatial domain (one packed
@@ -1059,20 +1064,29 @@ __global__ __launch_bounds__(kThreads) void k_dict_finalize(const CdlDev *__rest
     const int nxt = (P.sc->g1cur & 1) ^ 1;
     float *G1new = P.G1b[nxt];
     float2 *G1hnew = P.G1hb[nxt];
-    // mean spectrum j (0: d+g1, 1: d+g3, 2: d) at bin k < H
-    auto mean = [&](int j, int k) {
+    // mean spectrum j (0: d+g1, 1: d) at bin k < H, less sub
+    auto mean = [&](int j, int k, float2 sub) {
         const size_t i = (size_t)m * Hs + k;
         double sx = 0.0, sy = 0.0;
         for (int c = 0; c < P.nch; ++c) {
-            sx += gp(P.dpart)[((size_t)c * 6 + 2 * j) * HM + i];
-            sy += gp(P.dpart)[((size_t)c * 6 + 2 * j + 1) * HM + i];
+            sx += gp(P.dpart)[((size_t)c * kDpart + 2 * j) * HM + i];
+            sy += gp(P.dpart)[((size_t)c * kDpart + 2 * j + 1) * HM + i];
         }
-        return make_float2((float)(sx * invN), (float)(sy * invN));
+        return make_float2((float)(sx * invN - (double)sub.x), (float)(sy * invN - (double)sub.y));
     };
     for (int l = tid; l < Lg; l += kThreads) tw[l] = gp(P.tw)[l];
-    // v1 + i v2 = F^-1(V1 + i V2) for Hermitian V1, V2
+    // v1 + i v2 = F^-1(V1 + i V2) for Hermitian V1, V2: the means of d + g1 and of
+    // d + g3 = (d + g1) - (g1 - g3), with g1, g3 after this step's updates (train.jl:70,72
+    // precede the means at :76,:83): g1 - g3 first advances by (dh - G1h) - (dh - G3h) =
+    // G3h - G1h with this step's filters, read before G3h is replaced below.
+    const float2 *G1hcur = P.G1hb[nxt ^ 1];
+    const float2 zero2 = make_float2(0.f, 0.f);
     for (int k = tid; k < H; k += kThreads) {
-        const float2 a = mean(0, k), b = mean(1, k);
+        const size_t i = (size_t)m * Hs + k;
+        const float2 dl = gp(P.d13h)[i], f1 = gp(G1hcur)[i], f3 = gp(P.G3h)[i];
+        const float2 dn = make_float2(dl.x - (f1.x - f3.x), dl.y - (f1.y - f3.y));
+        gp(P.d13h)[i] = dn;
+        const float2 a = mean(0, k, zero2), b = mean(0, k, dn);
         fa[k] = make_float2(a.x - b.y, a.y + b.x);
         if (k > 0 && Lg - k >= H) fa[Lg - k] = make_float2(a.x + b.y, b.x - a.y);
     }
@@ -1092,7 +1106,7 @@ __global__ __launch_bounds__(kThreads) void k_dict_finalize(const CdlDev *__rest
     }
     __syncthreads();
     for (int k = tid; k < H; k += kThreads) {
-        const float2 a = mean(2, k);
+        const float2 a = mean(1, k, zero2);
         fa[k] = a;
         if (k > 0 && Lg - k >= H) fa[Lg - k] = cconj(a);
     }
@@ -1162,7 +1176,7 @@ __global__ __launch_bounds__(kThreads) void k_g1_spectrum(const CdlDev *__restri
 // One column (n, m) of a half-spectrum array back to the spatial domain (field reads of d,
 // g1, g3): out[(n M + m) Lg + l] = F^-1 of the Hermitian extension.  LDS: twiddles | a | b.
 __global__ __launch_bounds__(kThreads) void k_half_to_spatial(const CdlDev *__restrict__ tab, const float2 *__restrict__ src,
-                                                              float *__restrict__ out) {
+                                                              const float2 *__restrict__ sub, float *__restrict__ out) {
     extern __shared__ float4 smem4[];
     const CdlDev &P = tab[0];
     const int Lg = P.Lg, H = Lg / 2 + 1, Hs = half_bins(Lg), tid = threadIdx.x;
@@ -1172,7 +1186,11 @@ __global__ __launch_bounds__(kThreads) void k_half_to_spatial(const CdlDev *__re
     float2 *fb = fa + Lg;
     for (int l = tid; l < Lg; l += kThreads) tw[l] = gp(P.tw)[l];
     for (int k = tid; k < H; k += kThreads) {
-        const float2 v = gp(src)[c * Hs + k];
+        float2 v = gp(src)[c * Hs + k];
+        if (sub) {   // per filter: column c holds filter c mod M
+            const float2 u = gp(sub)[(c % (size_t)P.M) * Hs + k];
+            v = make_float2(v.x - u.x, v.y - u.y);
+        }
         fa[k] = v;
         if (k > 0 && Lg - k >= H) fa[Lg - k] = cconj(v);
     }
