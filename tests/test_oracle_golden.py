@@ -184,3 +184,20 @@ def test_c2_slice_fixture_is_consistent(golden_dir):
         assert np.all(G1[:, mk.FLEN:] == 0) and np.all(G3[:, mk.FLEN:] == 0)
         assert G3.min() >= 0 and G3.max() <= 1
         assert z["DX_" + p].shape == (mk.N_DX, G1.shape[1])
+
+
+def test_g1_minus_g3_is_the_same_for_every_sequence():
+    """The dictionary duals g1 and g3 (train.jl:70,72) start at 0 and receive the same d, so
+    g1 - g3 depends on the filters only and is equal across sequences; the library holds it
+    once per filter.  Checked on the FP64 oracle, which keeps both per sequence as the reference
+    does, after three outer iterations."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    from make_golden import make_cdl_inputs
+    fl = [8, 12, 8]
+    s, D0, X0, _ = make_cdl_inputs(4, 6, 20, 3, fl)
+    oc = O.OracleCDL(s, D0, X0, fl, 1.0, 100.0, 0.05, "f64")
+    _run(oc, (3, 4, 2))
+    diff = oc.get("g1") - oc.get("g3")          # (N, M, Lg)
+    assert np.abs(diff).max() > 1e-3            # not trivially zero
+    np.testing.assert_allclose(diff, np.broadcast_to(diff[:1], diff.shape), rtol=0, atol=1e-12 * np.abs(diff).max())
