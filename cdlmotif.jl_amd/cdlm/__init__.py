@@ -9,7 +9,8 @@ All compute runs in libcdlm.so on a gfx950 device.
 """
 from ._lib import CdlmError, CdlmOOM, load  # noqa: F401
 from .device import Context, default_context  # noqa: F401
-from .cdl import CDL, Batch, footprint_bytes, pack_by_footprint, autotune, tuning_set, tuning_get, CDL_setup, init_cdl, code_update, dict_update, update_best, objective, cdl_optimize  # noqa: F401
+from .cdl import (CDL, Batch, footprint_bytes, pack_by_footprint, autotune, tuning_set, tuning_get, CDL_setup,  # noqa: F401
+                  DNADataset, init_cdl, code_update, dict_update, update_best, objective, cdl_optimize)
 from .serve import Server  # noqa: F401
 from . import io  # noqa: F401
 from .record import (ActivationRecord, record_training_result, get_spatial_info, topology,  # noqa: F401

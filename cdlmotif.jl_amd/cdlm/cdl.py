@@ -21,7 +21,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from ._lib import check, cdlm_params, load
+from ._lib import ERR_ARG, CdlmError, check, cdlm_params, load
 from .device import Context, default_context
 
 FIELD_IDS = {"X": 0, "DX": 1, "d": 2, "G1": 3, "G2": 4, "G3": 5, "Xbest_slice": 6, "Dbest": 7, "Y1": 8, "y1": 11,
@@ -97,6 +97,9 @@ class CDL:
         check(L.cdlm_problem_dims(h, *[C.byref(x) for x in dims]), self.ctx.h)
         self.Lg, self.gamma, self.L, self.N, self.M = [x.value for x in dims]
         self.lam = lam
+        # the Sherman-Morrison constants are fixed when the problem is created (the device
+        # descriptor carries them); calls that name another value are refused
+        self.sherman_csc, self.sherman_cdl = float(np.float32(sherman_csc)), float(np.float32(sherman_cdl))
 
     # -- lifecycle
     def close(self):
@@ -118,10 +121,18 @@ class CDL:
         return int(load().cdlm_problem_bytes(self.h))
 
     # -- reference methods
-    def code_update(self, num_iter: int):
+    def _check_sherman(self, name: str, value: Optional[float], fixed: float):
+        if value is not None and float(np.float32(value)) != fixed:
+            raise CdlmError(ERR_ARG, "%s=%r differs from the problem's constant %r, which is fixed when the problem is "
+                                     "created (pass it to CDL(..., %s=...))" % (name, value, fixed, name.replace(
+                                         "sherman_const_", "sherman_")))
+
+    def code_update(self, num_iter: int, sherman_const_csc: Optional[float] = None):
+        self._check_sherman("sherman_const_csc", sherman_const_csc, self.sherman_csc)
         check(load().cdlm_code_update(self.h, int(num_iter)), self.ctx.h)
 
-    def dict_update(self, num_iter: int):
+    def dict_update(self, num_iter: int, sherman_const_cdl: Optional[float] = None):
+        self._check_sherman("sherman_const_cdl", sherman_const_cdl, self.sherman_cdl)
         check(load().cdlm_dict_update(self.h, int(num_iter)), self.ctx.h)
 
     def update_best(self):
@@ -299,20 +310,55 @@ def pack_by_footprint(sizes: Sequence[int], budget: int):
 
 
 # ---- reference-named free functions ---------------------------------------------------
-def init_cdl(setup: CDL_setup, data: np.ndarray, D0=None, X0=None, seed: int = 0, ctx: Optional[Context] = None) -> CDL:
-    """init_cdl(cdl_setup, dna_dataset) (src/SETUP/CDL_setup.jl:273-283)."""
+@dataclass
+class DNADataset:
+    """The fields of the reference's datasets (FASTA_DNA / Sim_DNA, src/SETUP/LOAD_setup.jl:22-98)
+    that the CDL path reads: data_matrix, the (4L, N) one-hot matrix (numpy (N, 4L) float32),
+    N and L.  raw_data keeps the sequences when built from strings."""
+    data_matrix: np.ndarray
+    N: int = 0
+    L: int = 0
+    raw_data: Optional[Sequence[str]] = None
+
+    def __post_init__(self):
+        self.data_matrix = _f32c(self.data_matrix)
+        self.N = int(self.data_matrix.shape[0])
+        self.L = int(self.data_matrix.shape[1] // 4)
+
+    @classmethod
+    def from_strings(cls, dna_strings: Sequence[str]) -> "DNADataset":
+        """data_2_dummy of the reads (src/LOAD/to_dummy.jl:29-38)."""
+        from .io import data_2_dummy
+        return cls(data_2_dummy(dna_strings), raw_data=list(dna_strings))
+
+    @classmethod
+    def from_fasta(cls, path: str, max_entries: Optional[int] = None) -> "DNADataset":
+        """FASTA_DNA's reads: read_fasta(path; max_entries) then data_2_dummy (LOAD_setup.jl:72-98; the
+        shuffled background it also builds comes from an external program and is not made here)."""
+        from .io import MAX_NUM_READ_FASTA, read_fasta
+        return cls.from_strings(read_fasta(path, MAX_NUM_READ_FASTA if max_entries is None else max_entries))
+
+
+def init_cdl(setup: CDL_setup, dna_dataset, D0=None, X0=None, seed: int = 0, ctx: Optional[Context] = None) -> CDL:
+    """init_cdl(cdl_setup, dna_dataset) (src/SETUP/CDL_setup.jl:273-283): filters of length
+    4*filter_size, rho, sigma, lambda from the setup and the dataset's data_matrix.  dna_dataset
+    is anything with a data_matrix attribute (DNADataset, or a reference-shaped object) or the
+    (N, 4L) matrix itself."""
+    data = getattr(dna_dataset, "data_matrix", dna_dataset)
     return CDL(data, setup.filter_lens(), setup.rho, setup.sigma, setup.lam, D0=D0, X0=X0, seed=seed, ctx=ctx)
 
 
-def code_update(c: CDL, num_iter: int, sherman_const_csc: float = 2.0):
-    """code_update!(c, num_iter, ComplexF32) (src/CDL/train.jl:1-44); the Sherman-Morrison
-    constant is a problem parameter fixed at creation."""
-    c.code_update(num_iter)
+def code_update(c: CDL, num_iter: int, type_complex=None, sherman_const_csc: float = 2.0):
+    """code_update!(c, num_iter, ComplexF32; sherman_const_csc=2f0) (src/CDL/train.jl:1-44).  The
+    Sherman-Morrison constant is a problem parameter fixed at creation: a call naming another
+    value raises CdlmError instead of running with the problem's."""
+    c.code_update(num_iter, sherman_const_csc=sherman_const_csc)
 
 
-def dict_update(c: CDL, num_iter: int, sherman_const_cdl: float = 2.0):
-    """dict_update!(c, num_iter, ComplexF32) (src/CDL/train.jl:46-89)."""
-    c.dict_update(num_iter)
+def dict_update(c: CDL, num_iter: int, type_complex=None, sherman_const_cdl: float = 2.0):
+    """dict_update!(c, num_iter, ComplexF32; sherman_const_cdl=2f0) (src/CDL/train.jl:46-89); the
+    constant is checked as in code_update."""
+    c.dict_update(num_iter, sherman_const_cdl=sherman_const_cdl)
 
 
 def update_best(c: CDL):

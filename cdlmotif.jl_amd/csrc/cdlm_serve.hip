@@ -385,8 +385,12 @@ extern "C" cdlm_status cdlm_server_destroy(cdlm_server *srv) {
 
 extern "C" const char *cdlm_server_last_error(cdlm_server *srv) {
     if (!srv) return "null server";
+    // The server thread may replace last_error at any time, so the caller gets a copy owned
+    // by its own thread, valid until that thread calls this function again.
+    static thread_local std::string copy;
     std::lock_guard<std::mutex> lk(srv->mu);
-    return srv->last_error.c_str();
+    copy = srv->last_error;
+    return copy.c_str();
 }
 
 extern "C" cdlm_status cdlm_server_submit(cdlm_server *srv, const cdlm_job_spec *spec, uint64_t *id) {

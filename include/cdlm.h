@@ -221,6 +221,8 @@ typedef struct {
 cdlm_status cdlm_server_create(int32_t device, const cdlm_server_opts *opts, cdlm_server **out);
 /* Stops the thread; queued and resident jobs end as CDLM_JOB_FAILED. */
 cdlm_status cdlm_server_destroy(cdlm_server *srv);
+/* Message of the last failed server call or job; the string is a copy owned by the calling
+ * thread, valid until that thread calls cdlm_server_last_error again. */
 const char *cdlm_server_last_error(cdlm_server *srv);
 /* Queues a job; CDLM_ERR_OOM when its footprint exceeds the whole budget (it could never
  * be admitted), CDLM_ERR_UNSUPPORTED outside the kernels' shape range. */

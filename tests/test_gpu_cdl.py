@@ -173,6 +173,20 @@ def test_c2_slice_1000_vs_oracle_fixture(reps):
         e_g64, e_3264, e_g32 = rel(a, b64), rel(b32, b64), rel(a, b32)
         flush("%s %s: gpu-f64 %.2e  f32oracle-f64 %.2e  gpu-f32oracle %.2e" % (tag, f, e_g64, e_3264, e_g32))
         assert e_g64 <= max(TOL, 1.5 * e_3264), (f, e_g64, e_3264)
+    if "thresh_f64" not in ref.files:
+        return
+    # after thresholding (cdl_optimize, wrap.jl:12; record_training_result!, store.jl:15-19): the
+    # code threshold within 1e-4 of FP64, the active sets equal FP64's except at codes within
+    # 1e-3 of the threshold (counted and bounded), identical uinds, the X_best slice within 1e-4
+    # of FP64.  The tiled run's copies are equal, its positive count is reps times the slice's and
+    # its corrected std differs from the slice's by a factor 1 - O(1/count).
+    import thresh_summary as TS
+    th, cnt = g.code_thresh(1.0)
+    xs = g.get("Xbest_slice")
+    for r in range(1, reps):
+        np.testing.assert_array_equal(xs[r * mk.N:(r + 1) * mk.N], xs[:mk.N], err_msg="X_best copy %d" % r)
+    assert cnt % reps == 0
+    TS.check(tag, xs[:mk.N], th, cnt // reps, ref, log=flush)
 
 
 def test_csc_is_per_sequence_exact():

@@ -166,3 +166,47 @@ def test_hits_long_pwms_and_ragged_lengths_vs_oracle(L, lmin, lmax, frac):
     np.testing.assert_array_equal(pos, opos)
     np.testing.assert_array_equal(comp.astype(np.int8), ocomp)
     np.testing.assert_array_equal(sc.view(np.uint32), osc.view(np.uint32))
+
+
+def _oracle_by_pwm_chunks(bases, pwms, lens, thresh, workers=16):
+    """The oracle's three scans over chunks of PWMs in parallel threads (the C calls release
+    the interpreter lock); results reassembled in PWM order."""
+    from concurrent.futures import ThreadPoolExecutor
+    K = len(lens)
+    bounds = np.linspace(0, K, workers + 1).astype(int)
+    chunks = [(a, b) for a, b in zip(bounds[:-1], bounds[1:]) if b > a]
+
+    def one(ab):
+        a, b = ab
+        sl = (bases, pwms[a:b], lens[a:b], thresh[a:b])
+        k, n, p, st, _ = O.pwm_hits(*sl, both_strands=True)
+        hits = np.stack([k + a, n, p, st], 1).astype(np.int64)
+        return hits, O.pwm_gpu_argmax(*sl), O.pwm_best(*sl)
+
+    with ThreadPoolExecutor(len(chunks)) as ex:
+        parts = list(ex.map(one, chunks))
+    hits = np.concatenate([p[0] for p in parts])
+    hits = hits[np.lexsort((hits[:, 3], hits[:, 2], hits[:, 1], hits[:, 0]))]
+    argmax = np.concatenate([p[1] for p in parts])
+    best = tuple(np.concatenate([p[2][i] for p in parts]) for i in range(3))
+    return hits, argmax, best
+
+
+def test_pwm_scan_workload_all_512_pwms_vs_oracle():
+    """The PWM-scan workload as configured (512 PWMs of length 8-24 with Dirichlet(0.5) columns,
+    background 0.25, thresholds 0.8 x best score; sequences of L=500 with GC fraction 0.42, both
+    strands) on 1000 sequences: hit list, argmax and best hit bit for bit against the oracle, so
+    every one of the 64 tiles of 16 strand-PWMs (and 32 tiles of 32 for the best hit) is covered."""
+    from cdlm import synth
+    bases, pwms, lens, thresh = synth.pwm_workload(5, 1000, 500, 512)
+    flush("PWM workload: 1000 x 500, 512 PWMs, oracle in parallel chunks")
+    hits, argmax, (opos, ocomp, osc) = _oracle_by_pwm_chunks(bases, pwms, lens, thresh)
+    got = cdlm.scan_hits(bases, pwms, lens, thresh, both_strands=True)
+    flush("  %d hits (oracle %d), %d PWMs with a hit" % (len(got), len(hits), len(np.unique(hits[:, 0]))))
+    assert len(hits) > 1000
+    np.testing.assert_array_equal(got, hits)
+    np.testing.assert_array_equal(cdlm.scan_w_gpu(bases, pwms, lens, thresh), argmax)
+    pos, comp, sc = cdlm.overlapping_scan(bases, pwms, lens, thresh)
+    np.testing.assert_array_equal(pos, opos)
+    np.testing.assert_array_equal(comp.astype(np.int8), ocomp)
+    np.testing.assert_array_equal(sc.view(np.uint32), osc.view(np.uint32))
