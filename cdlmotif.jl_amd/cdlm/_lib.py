@@ -146,6 +146,8 @@ def load():
         "cdlm_pwm_pvalue_last_ms": ([vp, P(f64)], C.c_int),
         "cdlm_onehot_to_bases": ([vp, vp, i32, i64, vp], C.c_int),
         "cdlm_pwm_hits_device": ([vp, vp, i32, i64, vp, vp, vp, i32, i32, i32, vp, vp, vp, vp, i64, P(i64)], C.c_int),
+        "cdlm_pwm_hits_device_noalloc": ([vp, vp, i32, i64, vp, vp, vp, i32, i32, i32, vp, vp, vp, vp, i64, P(i64)],
+                                         C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

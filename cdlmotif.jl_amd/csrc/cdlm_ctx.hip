@@ -39,6 +39,11 @@ extern "C" cdlm_status cdlm_create(cdlm_ctx **out, int32_t device) {
         delete c;
         return CDLM_ERR_HIP;
     }
+    if (cdlm_pwm_scratch_init(c) != CDLM_OK) {   // this device's copy of the fixed scan scratch
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return CDLM_ERR_HIP;
+    }
     *out = c;
     return CDLM_OK;
 }

@@ -18,7 +18,12 @@ struct cdlm_ctx {
     double last_scan_ms = 0.0;   // device time of the last PWM hit-scan kernel
     double last_topology_ms = 0.0;   // device time of the last activation-topology kernel
     double last_pvalue_ms = 0.0;     // device time of the last p-value threshold kernels
+    void *pwm_scratch = nullptr;     // this device's address of the fixed PWM-scan scratch array
 };
+
+// Resolves the device address of the PWM scan's fixed scratch array for ctx's device (current)
+// into ctx->pwm_scratch; once per device, from cdlm_create.
+cdlm_status cdlm_pwm_scratch_init(cdlm_ctx *ctx);
 
 // Records an error message on the context and returns the status.
 cdlm_status cdlm_fail(cdlm_ctx *ctx, cdlm_status st, const char *fmt, ...);

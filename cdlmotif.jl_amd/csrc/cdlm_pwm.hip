@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "cdlm_internal.h"
@@ -712,6 +713,52 @@ cdlm_status up(cdlm_ctx *ctx, T **dst, const std::vector<T> &src) {
     return CDLM_OK;
 }
 
+// Launches k_pwm_hits_mfma over strand-PWM tiles already on the device (P, T tiles of up to
+// maxlen columns) with the hit counter already zeroed.  timed: HIP events around the launch
+// give ctx->last_scan_ms (waits for the kernel); otherwise the launch is only enqueued.
+cdlm_status run_hits_kernel(cdlm_ctx *ctx, const uint8_t *d_bases, int L, long N, int maxlen, PwmTiles P, int *hk,
+                            long long *hn, int *hp, signed char *hs, long long cap, unsigned long long *d_count,
+                            bool timed) {
+    hipStream_t st = ctx->stream;
+    const int G = kPwmG;
+    const int MAXJ = maxlen <= 32 ? 32 : 64;
+    P.groups = (P.T + G - 1) / G;
+    const int Lpad = pwm_lpad(L, MAXJ, kPwmCH);
+    const bool oh = 4 * (size_t)Lpad * sizeof(float4) <= kPwmLdsMax;
+    const size_t lds = 4 * (size_t)Lpad * (oh ? sizeof(float4) : 1);
+    if (lds > kPwmLdsMax) return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "sequence length %d too long for the scan", L);
+    long yb = (N + 3) / 4;
+    const long target = (long)ctx->num_cu * 16 / (P.groups > 0 ? P.groups : 1) + 1;
+    if (yb > target) yb = target;
+    if (yb > 65535) yb = 65535;
+    if (yb < 1) yb = 1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timed && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) {
+        (void)hipGetLastError();
+        if (e0) (void)hipEventDestroy(e0);
+        e0 = e1 = nullptr;
+    }
+    if (e0) (void)hipEventRecord(e0, st);
+#define L_HITS(MJ, OHV)                                                                                         \
+    hipLaunchKernelGGL((k_pwm_hits_mfma<kPwmG, MJ, kPwmCH, OHV>), dim3(P.groups, (unsigned)yb), dim3(256), lds, st, \
+                       d_bases, L, N, P, hk, hn, hp, hs, cap, d_count)
+    if (maxlen <= 32 && oh) L_HITS(32, true);
+    else if (maxlen <= 32) L_HITS(32, false);
+    else if (oh) L_HITS(64, true);
+    else L_HITS(64, false);
+#undef L_HITS
+    hipError_t e = hipGetLastError();
+    cdlm_status s = e == hipSuccess ? CDLM_OK : cdlm_fail(ctx, CDLM_ERR_HIP, "k_pwm_hits_mfma: %s", hipGetErrorString(e));
+    if (e1) {
+        (void)hipEventRecord(e1, st);
+        float ms = 0.f;
+        if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) ctx->last_scan_ms = ms;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return s;
+}
+
 cdlm_status launch_hits(cdlm_ctx *ctx, const uint8_t *d_bases, int L, long N, const float *pwms, const int *lens,
                         const float *thresh, int K, int maxlen, int both, int *hk, long long *hn, int *hp,
                         signed char *hs, long long cap, unsigned long long *d_count) {
@@ -729,44 +776,49 @@ cdlm_status launch_hits(cdlm_ctx *ctx, const uint8_t *d_bases, int L, long N, co
         PwmTiles P;
         P.T = ht.T; P.B = dt.B; P.boff = dt.boff; P.tlen = dt.tlen;
         P.slot_k = dt.sk; P.slot_m = dt.sm; P.slot_st = dt.sst; P.slot_th = dt.sth;
-        const int G = kPwmG;
-        const int MAXJ = maxlen <= 32 ? 32 : 64;
-        P.groups = (ht.T + G - 1) / G;
-        const int Lpad = pwm_lpad(L, MAXJ, kPwmCH);
-        const bool oh = 4 * (size_t)Lpad * sizeof(float4) <= kPwmLdsMax;
-        const size_t lds = 4 * (size_t)Lpad * (oh ? sizeof(float4) : 1);
-        long yb = (N + 3) / 4;
-        const long target = (long)ctx->num_cu * 16 / (P.groups > 0 ? P.groups : 1) + 1;
-        if (yb > target) yb = target;
-        if (yb > 65535) yb = 65535;
-        if (yb < 1) yb = 1;
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
-            (void)hipGetLastError();
-            e0 = e1 = nullptr;
-        }
-        if (e0) (void)hipEventRecord(e0, st);
-#define L_HITS(MJ, OHV)                                                                                         \
-    hipLaunchKernelGGL((k_pwm_hits_mfma<kPwmG, MJ, kPwmCH, OHV>), dim3(P.groups, (unsigned)yb), dim3(256), lds, st, \
-                       d_bases, L, N, P, hk, hn, hp, hs, cap, d_count)
-        if (lds > kPwmLdsMax) s = cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "sequence length %d too long for the scan", L);
-        else if (maxlen <= 32 && oh) L_HITS(32, true);
-        else if (maxlen <= 32) L_HITS(32, false);
-        else if (oh) L_HITS(64, true);
-        else L_HITS(64, false);
-#undef L_HITS
-        hipError_t e = hipGetLastError();
-        if (s == CDLM_OK && e != hipSuccess) s = cdlm_fail(ctx, CDLM_ERR_HIP, "k_pwm_hits_mfma: %s", hipGetErrorString(e));
-        if (e1) (void)hipEventRecord(e1, st);
-        (void)hipStreamSynchronize(st);
-        float ms = 0.f;
-        if (e0 && e1 && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) ctx->last_scan_ms = ms;
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
+        s = run_hits_kernel(ctx, d_bases, L, N, maxlen, P, hk, hn, hp, hs, cap, d_count, true);
     }
     (void)hipStreamSynchronize(st);
     dt.release();
     return s;
+}
+
+// ---------------------------------------------------------------------------- fixed scratch
+// The allocation-free hit list (cdlm_pwm_hits_device_noalloc) keeps its strand-PWM tiles and
+// hit counter in this file-scope array instead of buffers allocated per call.  Each GPU holds
+// its own copy until the process ends; host code never passes the array's name to a kernel
+// (on the host it is only a stand-in for the symbol), but the device address that
+// hipGetSymbolAddress returns for the GPU, resolved once per GPU when a context is created.
+}  // namespace
+static __device__ __align__(256) unsigned char scratch[CDLM_PWM_SCRATCH_BYTES];
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxScratchDevices] = {};
+
+__global__ void k_zero_count(unsigned long long *c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0ull;
+}
+
+// Byte offsets of the tile arrays inside the scratch array (each 256-byte aligned); total
+// bytes in *total.
+struct ScratchLayout {
+    size_t count, B, boff, tlen, sk, sm, sst, sth, total;
+};
+ScratchLayout scratch_layout(size_t nB, size_t T) {
+    auto up256 = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    ScratchLayout o;
+    size_t at = 0;
+    o.count = at; at += up256(sizeof(unsigned long long));
+    o.B = at; at += up256(nB * sizeof(float));
+    o.boff = at; at += up256(T * sizeof(int));
+    o.tlen = at; at += up256(T * sizeof(int));
+    o.sk = at; at += up256(T * 16 * sizeof(int));
+    o.sm = at; at += up256(T * 16 * sizeof(int));
+    o.sst = at; at += up256(T * 16 * sizeof(int));
+    o.sth = at; at += up256(T * 16 * sizeof(float));
+    o.total = at;
+    return o;
 }
 
 
@@ -950,10 +1002,8 @@ extern "C" cdlm_status cdlm_pwm_hits_device(cdlm_ctx *ctx, const uint8_t *d_base
     cdlm_status s = check_pwm_args(ctx, d_bases, L, N, pwms, lens, thresh, K, maxlen);
     if (s != CDLM_OK) return s;
     if (!n_hits) return cdlm_fail(ctx, CDLM_ERR_ARG, "null n_hits");
-    DevPwm d;
-    s = upload(ctx, d, nullptr, L, N, pwms, lens, thresh, K, maxlen);
     unsigned long long *cnt = nullptr;
-    if (s == CDLM_OK && hipMalloc(&cnt, 8) != hipSuccess) {
+    if (hipMalloc(&cnt, 8) != hipSuccess) {
         (void)hipGetLastError();
         s = cdlm_fail(ctx, CDLM_ERR_OOM, "counter");
     }
@@ -964,9 +1014,84 @@ extern "C" cdlm_status cdlm_pwm_hits_device(cdlm_ctx *ctx, const uint8_t *d_base
     if (s == CDLM_OK && (hipMemcpyAsync(&total, cnt, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
                          hipStreamSynchronize(ctx->stream) != hipSuccess))
         s = cdlm_fail(ctx, CDLM_ERR_HIP, "count readback failed");
-    release(ctx, d, false);
     if (cnt) (void)hipFree(cnt);
     if (s != CDLM_OK) return s;
+    *n_hits = (int64_t)total;
+    if ((int64_t)total > cap) return cdlm_fail(ctx, CDLM_ERR_OOM, "%llu hits exceed cap %lld", total, (long long)cap);
+    return CDLM_OK;
+}
+
+// Resolves this GPU's address of the fixed scratch array once per device (with the context's
+// device current, from cdlm_create) and keeps it on the context.
+cdlm_status cdlm_pwm_scratch_init(cdlm_ctx *ctx) {
+    if (ctx->device < 0 || ctx->device >= kMaxScratchDevices)
+        return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "device %d beyond the scratch table", ctx->device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[ctx->device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            return cdlm_fail(ctx, CDLM_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        }
+        g_scratch_ptr[ctx->device] = ptr;
+    }
+    ctx->pwm_scratch = g_scratch_ptr[ctx->device];
+    return CDLM_OK;
+}
+
+extern "C" cdlm_status cdlm_pwm_hits_device_noalloc(cdlm_ctx *ctx, const uint8_t *d_bases, int32_t L, int64_t N,
+                                                    const float *pwms, const int32_t *lens, const float *thresh,
+                                                    int32_t K, int32_t maxlen, int32_t both_strands, int32_t *d_hit_k,
+                                                    int64_t *d_hit_n, int32_t *d_hit_pos, int8_t *d_hit_strand,
+                                                    int64_t cap, int64_t *n_hits) {
+    if (!ctx) return CDLM_ERR_ARG;
+    cdlm_status s = check_pwm_args(ctx, d_bases, L, N, pwms, lens, thresh, K, maxlen);
+    if (s != CDLM_OK) return s;
+    if (!n_hits) return cdlm_fail(ctx, CDLM_ERR_ARG, "null n_hits");
+    if (!ctx->pwm_scratch) return cdlm_fail(ctx, CDLM_ERR_STATE, "scratch array not resolved for device %d", ctx->device);
+    HostTiles ht;
+    build_tiles(pwms, lens, thresh, K, maxlen, both_strands, ht);
+    const ScratchLayout lay = scratch_layout(ht.B.size(), (size_t)ht.T);
+    if (lay.total > (size_t)CDLM_PWM_SCRATCH_BYTES)
+        return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "%d PWMs of up to %d columns need %zu scratch bytes, more than the %d "
+                         "of the allocation-free scan", K, maxlen, lay.total, (int)CDLM_PWM_SCRATCH_BYTES);
+    hipStream_t st = ctx->stream;
+    unsigned char *base = static_cast<unsigned char *>(ctx->pwm_scratch);
+    unsigned long long *cnt = reinterpret_cast<unsigned long long *>(base + lay.count);
+    hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, st, cnt);
+    CDLM_HIP(ctx, hipGetLastError());
+    unsigned long long total = 0;
+    if (N > 0) {
+        auto put = [&](size_t off, const void *src, size_t bytes) {
+            return bytes ? hipMemcpyAsync(base + off, src, bytes, hipMemcpyHostToDevice, st) : hipSuccess;
+        };
+        hipError_t e = put(lay.B, ht.B.data(), ht.B.size() * sizeof(float));
+        if (e == hipSuccess) e = put(lay.boff, ht.boff.data(), ht.boff.size() * sizeof(int));
+        if (e == hipSuccess) e = put(lay.tlen, ht.tlen.data(), ht.tlen.size() * sizeof(int));
+        if (e == hipSuccess) e = put(lay.sk, ht.sk.data(), ht.sk.size() * sizeof(int));
+        if (e == hipSuccess) e = put(lay.sm, ht.sm.data(), ht.sm.size() * sizeof(int));
+        if (e == hipSuccess) e = put(lay.sst, ht.sst.data(), ht.sst.size() * sizeof(int));
+        if (e == hipSuccess) e = put(lay.sth, ht.sth.data(), ht.sth.size() * sizeof(float));
+        if (e != hipSuccess) return cdlm_fail(ctx, CDLM_ERR_HIP, "tile upload: %s", hipGetErrorString(e));
+        PwmTiles P;
+        P.T = ht.T;
+        P.B = reinterpret_cast<const float *>(base + lay.B);
+        P.boff = reinterpret_cast<const int *>(base + lay.boff);
+        P.tlen = reinterpret_cast<const int *>(base + lay.tlen);
+        P.slot_k = reinterpret_cast<const int *>(base + lay.sk);
+        P.slot_m = reinterpret_cast<const int *>(base + lay.sm);
+        P.slot_st = reinterpret_cast<const int *>(base + lay.sst);
+        P.slot_th = reinterpret_cast<const float *>(base + lay.sth);
+        s = run_hits_kernel(ctx, d_bases, L, N, maxlen, P, d_hit_k, (long long *)d_hit_n, d_hit_pos,
+                            (signed char *)d_hit_strand, cap, cnt, false);
+        if (s != CDLM_OK) {
+            (void)hipStreamSynchronize(st);   // the host tiles must outlive any enqueued copy
+            return s;
+        }
+    }
+    CDLM_HIP(ctx, hipMemcpyAsync(&total, cnt, sizeof(total), hipMemcpyDeviceToHost, st));
+    CDLM_HIP(ctx, hipStreamSynchronize(st));
     *n_hits = (int64_t)total;
     if ((int64_t)total > cap) return cdlm_fail(ctx, CDLM_ERR_OOM, "%llu hits exceed cap %lld", total, (long long)cap);
     return CDLM_OK;

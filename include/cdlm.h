@@ -383,6 +383,22 @@ cdlm_status cdlm_pwm_hits_device(cdlm_ctx *ctx, const uint8_t *d_bases, int32_t 
                                  int32_t both_strands, int32_t *d_hit_k, int64_t *d_hit_n, int32_t *d_hit_pos,
                                  int8_t *d_hit_strand, int64_t cap, int64_t *n_hits);
 
+/* Allocation-free variant of cdlm_pwm_hits_device.  It has no counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results as cdlm_pwm_hits_device;
+ * the strand-PWM tiles (B operands) and the hit counter live in a fixed library array of
+ * CDLM_PWM_SCRATCH_BYTES per GPU (held from first use until the process ends) instead of buffers
+ * allocated per call, and the call allocates and frees no device memory.  Limit: the tiles of
+ * both strands, about 2*K*maxlen*16 bytes plus 4 KiB, must fit the array (K = 512 PWMs of up to
+ * 64 columns fit); larger inputs get CDLM_ERR_UNSUPPORTED.  The counter is cleared by a kernel
+ * at the start of each call.  All calls on one GPU share its array: calls that overlap on one
+ * GPU (from other threads or contexts) race and must be serialised by the caller. */
+#define CDLM_PWM_SCRATCH_BYTES (4 * 1024 * 1024)
+cdlm_status cdlm_pwm_hits_device_noalloc(cdlm_ctx *ctx, const uint8_t *d_bases, int32_t L, int64_t N,
+                                         const float *pwms, const int32_t *lens, const float *thresh, int32_t K,
+                                         int32_t maxlen, int32_t both_strands, int32_t *d_hit_k, int64_t *d_hit_n,
+                                         int32_t *d_hit_pos, int8_t *d_hit_strand, int64_t cap, int64_t *n_hits);
+
 #ifdef __cplusplus
 }
 #endif
