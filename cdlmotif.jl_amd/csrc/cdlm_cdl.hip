@@ -25,10 +25,6 @@
 
 namespace {
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-};
 
 // HIP events around launches (per kernel kind: 0 CSC, 1 dict, 2 best, 3 dict reduce).
 struct Timer {
@@ -104,12 +100,12 @@ int pick_csc_waves(int Lg, int taps, int nsl, int tuned) {
 }
 size_t lds_bytes_init(int Lg, int taps) { return sizeof(float) * (kWaves * (size_t)(Lg + taps) + kWaves * (size_t)Lg); }
 size_t lds_bytes_best(int Lg, int taps) { return sizeof(float) * (size_t)(Lg + taps + 4); }
-// k_dict with W waves: twiddles, h with its tail, three spectrum accumulators, two length-Lg
-// complex buffers, and per wave two of length Lg/4.
+// k_dict with W waves: twiddles, h with its tail, three spectrum accumulators and the stored
+// kappa, two length-Lg complex buffers, and per wave two of length Lg/4 and the stored Y3h.
 size_t lds_bytes_dict(int Lg, int taps, int W) {
     const size_t Hp = (size_t)(((Lg / 2 + 1) + 3) & ~3);
-    return sizeof(float) * (2 * (size_t)Lg + (size_t)((Lg + taps + 4 + 3) & ~3) + 5 * Hp + 4 * (size_t)Lg +
-                            (size_t)W * (size_t)Lg);
+    return sizeof(float) * (2 * (size_t)Lg + (size_t)((Lg + taps + 4 + 3) & ~3) + 7 * Hp + 4 * (size_t)Lg +
+                            (size_t)W * ((size_t)Lg + 2 * (size_t)dict_phx(Lg)));
 }
 size_t lds_bytes_half(int Lg) { return sizeof(float2) * 3 * (size_t)Lg; }
 size_t lds_bytes_fin(int Lg) { return sizeof(float) * (((size_t)Lg + 3) / 4 * 4 + 6 * (size_t)Lg); }
@@ -190,7 +186,12 @@ struct cdlm_problem {
     cdlm_ctx *ctx = nullptr;
     int L4 = 0, N = 0, M = 0, Lg = 0, gamma = 0, L = 0;
     std::vector<int> fl;
-    std::vector<DevBuf> bufs;
+    // all device buffers are carved from one arena (256-byte aligned, in the order of the
+    // buffer plan); an arena handed in by the caller (the server's recycled memory) is not
+    // freed with the problem
+    void *arena = nullptr;
+    size_t arena_bytes = 0, arena_used = 0;
+    bool own_arena = true;
     size_t total_bytes = 0;
     CdlDev dv;           // descriptor with the device pointers
     LaunchSet set;       // this problem alone
@@ -223,7 +224,7 @@ namespace {
 // Buffer plan of one problem: sizes in order of allocation.  The footprint query and the
 // allocator both use it, so cdlm_problem_bytes == cdlm_footprint_bytes.
 struct BufPlan {
-    size_t s, E, Eh, E3, Ex0, LN, LM, LMh, Lg, xs, y3s, M, partials, dpart, trace, sc;
+    size_t s, E, Eh, E3, Ex0, LN, LM, LMh, Lg, xs, y3s, kap, M, partials, dpart, trace, sc;
 };
 
 BufPlan buf_plan(int L4, int N, int M, int gamma, bool keep_x0) {
@@ -231,7 +232,7 @@ BufPlan buf_plan(int L4, int N, int M, int gamma, bool keep_x0) {
     const size_t Lg = (size_t)L4 + 2 * (size_t)gamma;
     b.s = (size_t)L4 * N * sizeof(float);
     b.E = Lg * M * (size_t)N * sizeof(float);
-    b.Eh = (size_t)half_bins((int)Lg) * M * (size_t)N * sizeof(float2);   // d, g1 as half spectra
+    b.Eh = (size_t)half_bins((int)Lg) * M * (size_t)N * sizeof(float2);   // g1 as half spectra
     b.E3 = 3 * (Lg / 4) * M * (size_t)N * sizeof(float);   // y3 off the stride-4 support
     b.Ex0 = keep_x0 ? b.E : 0;
     b.LN = Lg * (size_t)N * sizeof(float);
@@ -240,6 +241,7 @@ BufPlan buf_plan(int L4, int N, int M, int gamma, bool keep_x0) {
     b.Lg = Lg * sizeof(float);
     b.xs = (size_t)(L4 / 4) * M * (size_t)N * sizeof(float);
     b.y3s = (size_t)M * N * (Lg / 8 + 1) * sizeof(float2);
+    b.kap = (size_t)half_bins((int)Lg) * N * sizeof(float2);
     b.M = (size_t)M * sizeof(int);
     b.partials = (size_t)N * kPartStride * sizeof(double);
     b.dpart = (size_t)pick_nch((int)Lg, M, N) * kDpart * ((size_t)half_bins((int)Lg) * M) * sizeof(double);
@@ -248,21 +250,23 @@ BufPlan buf_plan(int L4, int N, int M, int gamma, bool keep_x0) {
     return b;
 }
 
+constexpr size_t kBufAlign = 256;
+size_t al(size_t x) { return (x + kBufAlign - 1) & ~(kBufAlign - 1); }
+
+// Arena bytes of a plan: every buffer rounded up to kBufAlign, in allocation order.
 size_t plan_total(const BufPlan &b) {
-    return b.s + 2 * b.E + 2 * b.Eh + b.E3 + b.Ex0 + 7 * b.LN + 7 * b.LM + 4 * b.LMh + 2 * b.Lg /* cS, sS */ +
-           2 * b.Lg /* tw */ + 2 * b.xs + b.y3s + b.M + b.partials + b.dpart + b.trace + b.sc;
+    return al(b.s) + 2 * al(b.E) + al(b.Eh) + al(b.E3) + (b.Ex0 ? al(b.Ex0) : 0) + 7 * al(b.LN) + 7 * al(b.LM) +
+           6 * al(b.LMh) + 2 * al(b.Lg) /* cS, sS */ + al(2 * b.Lg) /* tw */ + 2 * al(b.xs) + al(b.y3s) + al(b.kap) +
+           al(b.M) + al(b.partials) + al(b.dpart) + al(b.trace) + al(b.sc);
 }
 
 cdlm_status palloc(cdlm_problem *p, void **out, size_t bytes) {
-    void *q = nullptr;
-    hipError_t e = hipMalloc(&q, bytes ? bytes : 4);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        return cdlm_fail(p->ctx, CDLM_ERR_OOM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
-    }
-    p->bufs.push_back({q, bytes});
-    p->total_bytes += bytes;
-    *out = q;
+    const size_t need = al(bytes ? bytes : 4);
+    if (p->arena_used + need > p->arena_bytes)
+        return cdlm_fail(p->ctx, CDLM_ERR_STATE, "buffer plan exceeds its arena (%zu + %zu > %zu bytes)", p->arena_used,
+                         need, p->arena_bytes);
+    *out = static_cast<char *>(p->arena) + p->arena_used;
+    p->arena_used += need;
     return CDLM_OK;
 }
 
@@ -277,8 +281,9 @@ cdlm_status palloc_t(cdlm_problem *p, T **out, size_t bytes) {
 void free_problem(cdlm_problem *p) {
     p->set.release();
     p->timer.clear();
-    for (auto &b : p->bufs) (void)hipFree(b.p);
-    p->bufs.clear();
+    if (p->own_arena && p->arena) (void)hipFree(p->arena);
+    p->arena = nullptr;
+    p->arena_bytes = p->arena_used = 0;
 }
 
 // Builds the launch set of np problems (all on ctx): maxima, LDS sizes, device table.
@@ -343,7 +348,7 @@ cdlm_status set_build(cdlm_ctx *ctx, cdlm_problem *const *ps, int np, LaunchSet 
 #undef SETA
     if ((st = ensure_lds(ctx, (const void *)k_dict_finalize, T.lds_fin)) != CDLM_OK) return st;
     if ((st = ensure_lds(ctx, (const void *)k_g1_spectrum, T.lds_g1)) != CDLM_OK) return st;
-    if ((st = ensure_lds(ctx, (const void *)k_half_to_spatial, lds_bytes_half(Lg))) != CDLM_OK) return st;
+    if ((st = ensure_lds(ctx, (const void *)k_dict_to_spatial, lds_bytes_half(Lg))) != CDLM_OK) return st;
 
     std::vector<CdlDev> tab(np);
     std::vector<int> pre(5 * (size_t)np);
@@ -557,8 +562,16 @@ __global__ void k_bases_to_onehot(const uint8_t *__restrict__ codes, int L, long
 }
 
 cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const float *s, const uint8_t *codes, int32_t L4,
-                                int32_t N, const float *D0, const float *X0, uint64_t seed, cdlm_problem **out);
+                                int32_t N, const float *D0, const float *X0, uint64_t seed, cdlm_problem **out,
+                                void *arena = nullptr, size_t arena_bytes = 0);
 }  // namespace
+
+cdlm_status cdlm_problem_create_in(cdlm_ctx *ctx, const cdlm_params *prm, const float *s, int32_t L4, int32_t N,
+                                   const float *D0, uint64_t seed, void *arena, size_t arena_bytes, cdlm_problem **out) {
+    if (!ctx) return CDLM_ERR_ARG;
+    if (!s || !arena) return cdlm_fail(ctx, CDLM_ERR_ARG, "null argument");
+    return problem_create_impl(ctx, prm, s, nullptr, L4, N, D0, nullptr, seed, out, arena, arena_bytes);
+}
 
 extern "C" cdlm_status cdlm_problem_create(cdlm_ctx *ctx, const cdlm_params *prm, const float *s, int32_t L4, int32_t N,
                                            const float *D0, const float *X0, uint64_t seed, cdlm_problem **out) {
@@ -578,7 +591,8 @@ extern "C" cdlm_status cdlm_problem_create_from_bases(cdlm_ctx *ctx, const cdlm_
 
 namespace {
 cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const float *s, const uint8_t *codes, int32_t L4,
-                                int32_t N, const float *D0, const float *X0, uint64_t seed, cdlm_problem **out) {
+                                int32_t N, const float *D0, const float *X0, uint64_t seed, cdlm_problem **out,
+                                void *arena, size_t arena_bytes) {
     if (!prm || !out || !prm->filter_len) return cdlm_fail(ctx, CDLM_ERR_ARG, "null argument");
     *out = nullptr;
     if (L4 <= 0 || L4 % 4 != 0) return cdlm_fail(ctx, CDLM_ERR_ARG, "first dimension of dat must be a positive multiple of 4 (got %d)", L4);
@@ -621,21 +635,40 @@ cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const flo
 
     const BufPlan b = buf_plan(L4, N, M, gamma, X0 != nullptr);
     cdlm_status st = CDLM_OK;
+    const size_t total = plan_total(b);
+    if (arena) {
+        if (arena_bytes < total)
+            return cdlm_fail(ctx, CDLM_ERR_ARG, "arena of %zu bytes is smaller than the problem's %zu", arena_bytes, total);
+        p->arena = arena;
+        p->arena_bytes = arena_bytes;
+        p->own_arena = false;
+    } else {
+        hipError_t e = hipMalloc(&p->arena, total);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            p->arena = nullptr;
+            return cdlm_fail(ctx, CDLM_ERR_OOM, "hipMalloc(%zu bytes) failed: %s", total, hipGetErrorString(e));
+        }
+        p->arena_bytes = total;
+        p->own_arena = true;
+    }
+    p->total_bytes = total;
     float *s_dev = nullptr, *X0_dev = nullptr, *D0_dev = nullptr;
     int *fl_dev = nullptr;
     float2 *tw_dev = nullptr;
 #define A(ptr, bytes) do { st = palloc_t(p.get(), &(ptr), (bytes)); if (st != CDLM_OK) { free_problem(p.get()); return st; } } while (0)
     A(s_dev, b.s);
-    A(d.Xt, b.E); A(d.y2, b.E); A(d.y3, b.E3); A(d.dh, b.Eh); A(d.g1h, b.Eh);
+    A(d.Xt, b.E); A(d.y2, b.E); A(d.y3, b.E3); A(d.g1h, b.Eh);
     if (X0) A(X0_dev, b.Ex0);
     A(d.h, b.LN); A(d.y1, b.LN); A(d.Y1, b.LN); A(d.DX, b.LN); A(d.G2, b.LN); A(d.g2, b.LN); A(d.Td, b.LN);
     A(d.G1b[0], b.LM); A(d.G1b[1], b.LM); A(d.G3, b.LM); A(d.meand, b.LM); A(d.Dbest, b.LM); A(d.gsq, b.LM);
     A(D0_dev, b.LM);
-    A(d.G1hb[0], b.LMh); A(d.G1hb[1], b.LMh); A(d.G3h, b.LMh); A(d.d13h, b.LMh);
+    A(d.G1hb[0], b.LMh); A(d.G1hb[1], b.LMh); A(d.G3h, b.LMh); A(d.d13h, b.LMh); A(d.G1hp, b.LMh); A(d.Fp, b.LMh);
     A(d.cS, b.Lg); A(d.sS, b.Lg);
     A(tw_dev, 2 * b.Lg);
     A(d.xs[0], b.xs); A(d.xs[1], b.xs);
     A(d.y3s, b.y3s);
+    A(d.kap, b.kap);
     A(fl_dev, b.M);
     A(d.partials, b.partials);
     A(d.dpart, b.dpart);
@@ -822,14 +855,13 @@ extern "C" cdlm_status cdlm_get(cdlm_problem *p, int32_t field, void *dst, size_
     case CDLM_F_g3: {
         // held as half spectra: back to the spatial domain on the device first
         if (bytes != E * sizeof(float)) return cdlm_fail(p->ctx, CDLM_ERR_ARG, "field %d needs %zu bytes", field, E * sizeof(float));
-        const float2 *h = field == CDLM_F_D ? d.dh : d.g1h;
-        const float2 *sub = field == CDLM_F_g3 ? d.d13h : nullptr;   // g3 = g1 - (g1 - g3)
+        const int what = field == CDLM_F_D ? 0 : field == CDLM_F_g1 ? 1 : 2;
         float *tmp = nullptr;
         hipError_t e = hipMalloc(&tmp, E * sizeof(float));
         if (e != hipSuccess) { (void)hipGetLastError(); return cdlm_fail(p->ctx, CDLM_ERR_OOM, "no memory for field %d", field); }
-        hipLaunchKernelGGL(k_half_to_spatial, dim3((unsigned)((size_t)p->M * p->N)), dim3(kThreads), lds_bytes_half(p->Lg), st,
-                           p->set.tab, h, sub, tmp);
-        cdlm_status s = lcheck(p->ctx, "k_half_to_spatial");
+        hipLaunchKernelGGL(k_dict_to_spatial, dim3((unsigned)((size_t)p->M * p->N)), dim3(kThreads), lds_bytes_half(p->Lg),
+                           st, p->set.tab, what, tmp);
+        cdlm_status s = lcheck(p->ctx, "k_dict_to_spatial");
         hipError_t e2 = s == CDLM_OK ? hipMemcpyAsync(dst, tmp, bytes, hipMemcpyDeviceToHost, st) : hipSuccess;
         hipError_t e3 = hipStreamSynchronize(st);
         (void)hipFree(tmp);

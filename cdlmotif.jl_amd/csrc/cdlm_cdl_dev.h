@@ -18,16 +18,18 @@
 //     next kernel that reads X.  g is always the current G1: before G1 changes (the
 //     dictionary step) X is written back materialised (Xt = X, h = 0) -- by update_best's
 //     pass, which rebuilds all of X anyway, or else by the dictionary step itself.
-//   * the dictionary step holds d, g1 and g3 as half spectra (dh, g1h, g3h, as the reference
-//     keeps dh, g1h, g3h), g3 through g1 - g3: every sequence's g1 and g3 receive the same
-//     d and differ by the filters only (g1 += d - G1, g3 += d - G3 from g1 = g3 = 0), so
-//     g1 - g3 = sum over the steps of G3 - G1 is one (Hs, M) array; its solve is
+//   * the dictionary step works on half spectra (dh, g1h, g3h, as the reference keeps dh, g1h,
+//     g3h), g3 through g1 - g3: every sequence's g1 and g3 receive the same d and differ by the
+//     filters only (g1 += d - G1, g3 += d - G3 from g1 = g3 = 0), so g1 - g3 = sum over the
+//     steps of G3 - G1 is one (Hs, M) array; its solve is
 //       dh_m = Qh_m + conj(Y3h_m) kappa,   kappa = (c1 F(G2-g2) - sum_m Y3h_m Qh_m)/(|Y3h|^2 + c1)
 //     with Qh_m = (G1h_m - g1h_m) + (G3h_m - g3h_m) (c1 = sherman_const_cdl; no 1/c1, as
-//     sherman_CDL_combine), and g1h, g3h are updated in the same domain.  Y3 = Pi4(X + y3) is
-//     nonzero on every 4th sample only, so its spectrum is a length Lg/4 DFT repeated 4 times:
-//     the only per-column transform of the step.  The means over n of d + g1, d + g3 and d
-//     are taken on the spectra and transformed back once per filter.
+//     sherman_CDL_combine).  Y3 = Pi4(X + y3) is nonzero on every 4th sample only, so its
+//     spectrum is a length Lg/4 DFT repeated 4 times: the only per-column transform of the
+//     step.  dh and the new g1h are functions of the old g1h, Y3h and kappa, so those are what
+//     the step stores (with the step's filter spectra); the next step rebuilds dh and g1h per
+//     bin as it streams the column, and the means over n of d + g1, d + g3 and d come from
+//     sums over n of the old g1h and of conj(Y3h) kappa, transformed back once per filter.
 // All quantities are exact rewrites of the reference iteration; only rounding differs.
 //
 // Launch model.  Every kernel reads its problem from a device table of descriptors
@@ -74,15 +76,17 @@ struct CdlDev {
     const float *s;
     const int *fl;
     float *Xt, *y2;                                // (Lg, M, N)
-    float2 *dh, *g1h;                              // (Hs, M, N): half spectra of d, g1
+    float2 *g1h;                                   // (Hs, M, N): g1h of the step before the last one
+    float2 *kap;                                   // (Hs, N): kappa of the last dictionary step
     float2 *d13h;                                  // (Hs, M): g1h - g3h, the same for every sequence
+    float2 *G1hp, *Fp;                             // (Hs, M): G1h and G3h + d13h of the last dictionary step
     float *y3;                                     // (3 Lg/4, M, N): y3 off the stride-4 support
     float *h, *y1, *Y1, *DX, *G2, *g2, *Td;       // (Lg, N)
     float *G1b[2], *G3, *meand, *Dbest, *gsq;     // (Lg, M)
     float2 *G1hb[2], *G3h;                         // (Hs, M): half spectra of G1b[i], G3
     float *cS, *sS;                                // (Lg)
     float *xs[2];                                  // (L, M, N) X_best slices
-    float2 *y3s;                                   // (Lg/8+1, M, N) dictionary scratch
+    float2 *y3s;                                   // (Lg/8+1, M, N): half spectra of the last step's Y3h
     const float2 *tw;                              // (Lg) twiddles
     double *partials, *dpart, *trace;
     CdlScal *sc;
@@ -95,6 +99,9 @@ struct CdlDev {
 // Half-spectrum bins kept per column: Lg/2 + 1 rounded up to even, so that a column is a
 // whole number of float4 (two bins each) and columns stay 16-byte aligned.
 __host__ __device__ __forceinline__ int half_bins(int Lg) { return ((Lg / 2 + 1) + 1) & ~1; }
+// Complex entries of the dictionary kernel's per-wave copy of a stored Y3h half spectrum
+// (Lg/8 + 1 bins, rounded up to even).
+__host__ __device__ __forceinline__ int dict_phx(int Lg) { return ((Lg / 8 + 1) + 1) & ~1; }
 
 // Block -> (problem, local unit): the last p with starts[p] <= b.
 __device__ __forceinline__ int find_problem(const int *__restrict__ starts, int np, int b, int &local) {
@@ -326,16 +333,21 @@ __global__ __launch_bounds__(kThreads) void k_init(const CdlDev *__restrict__ ta
             st3(P.y3 + col3_of(n, mm, M, nslot), s, reinterpret_cast<const float *>(&zero4));
             vb4[TAPS / 4 + s] = xp;
         }
-        // d = D (dh = F(D), CDL_setup.jl:207-208), g1 = g3 = 0, as half spectra (g1 - g3 = 0)
+        // d = D (dh = F(D), CDL_setup.jl:207-208), g1 = g3 = 0: the dictionary state A = 0,
+        // Y3h = 0 and kappa = 0 with the filter spectra G1hp = F(D) and Fp = 0 (g1 - g3 = 0)
         {
-            const int Hs = half_bins(Lg);
+            const int Hs = half_bins(Lg), Ph = (Lg >> 2) / 2 + 1;
             const size_t colh = ((size_t)n * M + mm) * Hs;
             const float2 *G1h = P.G1hb[P.sc->g1cur & 1] + (size_t)mm * Hs;
             for (int t = lane; t < Hs / 2; t += 64) {
                 st4(reinterpret_cast<float *>(P.g1h + colh), t, zero4);
-                if (n == 0) st4(reinterpret_cast<float *>(P.d13h + (size_t)mm * Hs), t, zero4);
-                st4(reinterpret_cast<float *>(P.dh + colh), t, ld4(reinterpret_cast<const float *>(G1h), t));
+                if (n == 0) {
+                    st4(reinterpret_cast<float *>(P.d13h + (size_t)mm * Hs), t, zero4);
+                    st4(reinterpret_cast<float *>(P.Fp + (size_t)mm * Hs), t, zero4);
+                    st4(reinterpret_cast<float *>(P.G1hp + (size_t)mm * Hs), t, ld4(reinterpret_cast<const float *>(G1h), t));
+                }
             }
+            for (int k = lane; k < Ph; k += 64) gp(P.y3s)[((size_t)n * M + mm) * Ph + k] = make_float2(0.f, 0.f);
         }
         wave_lds_sync();
         v_head<TAPS>(vb4, nslot, lane);
@@ -357,6 +369,8 @@ __global__ __launch_bounds__(kThreads) void k_init(const CdlDev *__restrict__ ta
         gp(P.g2)[ro + l] = 0.f;
         gp(P.Td)[ro + l] = 0.f;
     }
+    const int Hs = half_bins(Lg);
+    for (int k = tid; k < Hs; k += kThreads) gp(P.kap)[(size_t)n * Hs + k] = make_float2(0.f, 0.f);
 }
 
 // ---------------------------------------------------------------------------- CSC
@@ -719,33 +733,43 @@ __global__ __launch_bounds__(kThreads) void k_decide(const CdlDev *__restrict__ 
 
 // ---------------------------------------------------------------------------- dict
 // One iteration of dict_update! (train.jl:50-72) for one sequence; the means over n
-// (:76,:83) follow in k_dict_reduce / k_dict_finalize.  d, g1 and g3 are held as half
-// spectra (dh, g1h, g3h: bins 0..Lg/2), which the reference keeps too (dh, g1h, g3h), so the
-// only per-column transform left is that of Y3 = Pi4(X + y3) = Pi4(X) (y3 = 0 on the
-// support): one sample per slot, so Y3h is a length-Lg/4 DFT repeated 4 times.
-//   pass 1, columns m = w, w+W, ... of wave w (W = blockDim/64): Y3h by a wave-level DFT;
-//     with Qh = (G1h - g1h) + (G3h - g3h) (cdl_kernels.jl:8), |Y3h|^2, Y3h dh and Y3h Qh add
-//     into the wave's registers in ascending m (:52-59); the wave partials are then summed in
-//     wave order.  Unless X is held materialised already it is written back materialised
-//     (Xt = X, h = 0), since G1 changes after this step.
-//   a block-wide spectral step gives Td = F^-1(sum_m Y3h dh) (:56-57), G2, g2 (update_g2!,
-//     :68-71) and kappa = (c1 F(G2 - g2) - sum_m Y3h Qh) / (|Y3h|^2 + c1).
-//   pass 2, the same columns: dh = Qh + conj(Y3h) kappa (sherman_CDL_combine, :60),
-//     g1h += dh - G1h (:70 in the transform domain); g3h = g1h - d13h throughout, and
-//     g3h += dh - G3h (:72) is d13h += G3h - G1h, once per filter in k_dict_finalize.
-// The partial-sum order depends on W, which belongs to the problem's launch choices (a
-// problem runs with the same W alone and in a batch).  Y3h is taken Hermitian from its half
-// spectrum (what pass 1 leaves in y3s).  A lane owns the float4 slots t = lane + 64 i of a
-// spectral column (bins 2t, 2t+1) and s = lane + 64 i of a spatial one.
-// LDS: twiddles | h with wrapped tail | accS, accT, accP (kappa) | fa, fb (Lg complex, the
-// block transforms) | per wave: ya, yb (LgMax/4 complex each).
+// (:76,:83) follow in k_dict_reduce / k_dict_finalize.
+// State.  d, g1 and g3 are held in the transform domain (half spectra, bins 0..Lg/2, as the
+// reference keeps dh, g1h, g3h), and after step t they are
+//   dh_t  = (G1h_t - A) + (F_t - A) + conj(Y3h_t) kappa_t     (sherman_CDL_combine, :60)
+//   g1h_t = (A + dh_t) - G1h_t                                  (:70)
+//   g3h_t = g1h_t - d13h_t                                      (g1 - g3 is one array per filter)
+// with A = g1h_{t-1}, step t's Y3h and kappa, its filter spectra G1h_t and
+// F_t = G3h_t + d13h_{t-1}.  So the state kept is A (the g1h buffer), the half spectra of Y3h
+// (y3s), kappa per sequence (kap) and the two filter spectra (G1hp, Fp): dh and g1h_t are never
+// written.  The next step rebuilds them per bin while it streams the column anyway, and the
+// means over n follow from sum_n A and sum_n conj(Y3h) kappa (k_dict_reduce), so a step reads
+// and writes each spectral column once.  Before the first step A = Y3h = kappa = Fp = 0 and
+// G1hp = F(D): dh = F(D), g1h = 0.
+// Step t, columns m = w, w+W, ... of wave w (W = blockDim/64):
+//   * Y3 = Pi4(X + y3) = Pi4(X) (y3 = 0 on the support) has one sample per slot, so Y3h is a
+//     length-Lg/4 DFT repeated 4 times: one wave-level transform per column;
+//   * per bin: dh_{t-1} and g1h_{t-1} from (A, Y3h_{t-1}, kappa_{t-1}); Qh = (G1h - g1h_{t-1})
+//     + (G3h - g3h_{t-1}) (cdl_kernels.jl:8); |Y3h|^2, Y3h dh_{t-1} and Y3h Qh add into the
+//     wave's registers in ascending m (:52-59); g1h_{t-1} replaces A, Y3h_t replaces Y3h_{t-1};
+//   * the wave partials are summed in wave order; a block-wide spectral step gives
+//     Td = F^-1(sum_m Y3h dh_{t-1}) (:56-57, quirk: the previous d), G2, g2 (update_g2!,
+//     :68-71) and kappa_t = (c1 F(G2 - g2) - sum_m Y3h Qh) / (|Y3h|^2 + c1), stored.
+// Unless X is held materialised already it is written back materialised (Xt = X, h = 0),
+// since G1 changes after this step.  The partial-sum order depends on W, which belongs to the
+// problem's launch choices (a problem runs with the same W alone and in a batch).  A lane owns
+// the float4 slots t = lane + 64 i of a spectral column (bins 2t, 2t+1) and s = lane + 64 i
+// of a spatial one.
+// LDS: twiddles | h with wrapped tail | accS, accT, accP | kappa_{t-1} | fa, fb (Lg complex,
+// the block transforms) | per wave: ya, yb (LgMax/4 complex each), yo (Y3h_{t-1}).
 // Registers: at most 168 (3 waves per SIMD) for padded lengths up to 1020 and filters up to
-// 24, where that needs no spills; 256 otherwise.
+// 24; 256 otherwise.
 template <int TAPS, int NSL>
 __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
     const CdlDev *__restrict__ tab, const int *__restrict__ starts, int np, int LgMax) {
-    constexpr bool kPf = NSL <= 4;              // spectral loads in flight across the Y3 transform
+    constexpr bool kPf = NSL <= 4;              // the column's A in flight across the Y3 transform
     constexpr int NQ = NSL <= 4 ? 4 : 9;        // spectral float4 slots per lane: Hs/2 <= 64 NQ
+    constexpr int NY = NSL <= 4 ? 2 : 5;        // stored Y3h bins per lane: Lg/8 + 1 <= 64 NY
     extern __shared__ float4 smem4[];
     int n;
     const CdlDev &P = tab[find_problem(starts, np, blockIdx.x, n)];
@@ -753,24 +777,29 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
     const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int THR = blockDim.x, W = THR >> 6;
     const int Lg = P.Lg, M = P.M, Pq = Lg >> 2, Ph = Pq / 2 + 1, H = Lg / 2 + 1, Hs = half_bins(Lg), nq = Hs >> 1;
-    const int HsX = ((LgMax / 2 + 1) + 3) & ~3, PqX = LgMax >> 2;
+    const int HsX = ((LgMax / 2 + 1) + 3) & ~3, PqX = LgMax >> 2, PhX = dict_phx(LgMax);
     float2 *tw = reinterpret_cast<float2 *>(sm);
     float *hb = sm + 2 * LgMax;
     float *accS = hb + ((LgMax + TAPS + 4 + 3) & ~3);
     float2 *accT = reinterpret_cast<float2 *>(accS + HsX);
     float2 *accP = accT + HsX;
-    float2 *fa = accP + HsX, *fb = fa + LgMax;
-    float2 *ya = fb + LgMax + w * 2 * PqX, *yb = ya + PqX;
+    float2 *kold = accP + HsX;
+    float2 *fa = kold + HsX, *fb = fa + LgMax;
+    float2 *ya = fb + LgMax + w * (2 * PqX + PhX), *yb = ya + PqX, *yo = yb + PqX;
     const size_t ro = (size_t)n * Lg;
     float2 *y3s = P.y3s + (size_t)n * M * Ph;
+    float2 *kap = P.kap + (size_t)n * Hs;
     const float *G1 = g1cur_of(P);
     const float *G1h = reinterpret_cast<const float *>(P.G1hb[P.sc->g1cur & 1]);
     const float *G3h = reinterpret_cast<const float *>(P.G3h);
     const float *D13h = reinterpret_cast<const float *>(P.d13h);
+    const float *G1hp = reinterpret_cast<const float *>(P.G1hp);
+    const float *Fp = reinterpret_cast<const float *>(P.Fp);
     const bool mat = !P.sc->xmat;   // set again only by k_cs after this step
     const float4 *hb4 = reinterpret_cast<const float4 *>(hb);
 
     for (int l = tid; l < Lg; l += THR) tw[l] = gp(P.tw)[l];
+    for (int k = tid; k < H; k += THR) kold[k] = gp(kap)[k];
     if (mat) load_h<TAPS>(P, ro, hb, tid, THR);
     __syncthreads();
 
@@ -781,7 +810,6 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
         return kq < Ph ? Y[kq] : cconj(Y[Pq - kq]);
     };
 
-    // ---- pass 1
     float aS[2 * NQ];
     float2 aT[2 * NQ], aP[2 * NQ];
 #pragma unroll
@@ -799,22 +827,21 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
         const int mm = __builtin_amdgcn_readfirstlane(m);
         const size_t col = ((size_t)n * M + mm) * Lg;
         const size_t colh = ((size_t)n * M + mm) * Hs, fcol = (size_t)mm * Hs;
-        // this column's spectra (dh, Qh), in flight during the Y3 transform
-        float4 dq[kPf ? NQ : 1], qq[kPf ? NQ : 1];
-        auto loadq = [&](int i, float4 &D, float4 &Q) {
-            const int t = lane + 64 * i;
-            if (t < nq) {
-                D = ld4(reinterpret_cast<const float *>(P.dh + colh), t);
-                const float4 a1 = ld4(reinterpret_cast<const float *>(P.g1h + colh), t);
-                const float4 a3 = sub4(a1, ld4(D13h + 2 * fcol, t));   // g3h = g1h - (g1h - g3h)
-                const float4 F1 = ld4(G1h + 2 * fcol, t), F3 = ld4(G3h + 2 * fcol, t);
-                Q = make_float4((F1.x - a1.x) + (F3.x - a3.x), (F1.y - a1.y) + (F3.y - a3.y),
-                                (F1.z - a1.z) + (F3.z - a3.z), (F1.w - a1.w) + (F3.w - a3.w));
-            }
-        };
+        float *Acol = reinterpret_cast<float *>(P.g1h + colh);
+        // this column's stored Y3h_{t-1} (and with prefetch its A), in flight during the Y3 transform
+        float4 av[kPf ? NQ : 1];
+        float2 yv[NY];
         if constexpr (kPf) {
 #pragma unroll
-            for (int i = 0; i < NQ; ++i) loadq(i, dq[i], qq[i]);
+            for (int i = 0; i < NQ; ++i) {
+                const int t = lane + 64 * i;
+                if (t < nq) av[i] = ld4(Acol, t);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NY; ++i) {
+            const int k = lane + 64 * i;
+            if (k < Ph) yv[i] = gp(y3s)[(size_t)mm * Ph + k];
         }
         float g[TAPS];
         if (mat) load_taps<TAPS>(G1, mm, Lg, g);
@@ -833,28 +860,46 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
             }
         }
         if (mm + W < M) loadx(col + (size_t)W * Lg);
+#pragma unroll
+        for (int i = 0; i < NY; ++i) {
+            const int k = lane + 64 * i;
+            if (k < Ph) yo[k] = yv[i];
+        }
         wave_lds_sync();
         const float2 *Y = team_fft<false>(ya, yb, P.plan4, tw, lane, 64, WaveSync{}, 1);
-        for (int k = lane; k < Ph; k += 64) gp(y3s)[(size_t)mm * Ph + k] = Y[k];
+        for (int k = lane; k < Ph; k += 64) gp(y3s)[(size_t)mm * Ph + k] = Y[k];   // the old values are in yo
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
-            float4 D, Q;
-            if constexpr (kPf) { D = dq[i]; Q = qq[i]; } else { loadq(i, D, Q); }
             const int t = lane + 64 * i;
             if (t < nq) {
+                const float4 A = kPf ? av[kPf ? i : 0] : ld4(Acol, t);
+                const float4 P1 = ld4(G1hp + 2 * fcol, t), PF = ld4(Fp + 2 * fcol, t);
+                const float4 F1 = ld4(G1h + 2 * fcol, t), F3 = ld4(G3h + 2 * fcol, t), DL = ld4(D13h + 2 * fcol, t);
+                float an[4];
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
-                    const int k = 2 * t + h;
+                    const int k = 2 * t + h, j = 2 * i + h;
+                    float2 Ac = make_float2(0.f, 0.f);
                     if (k < H) {
+                        const float2 a = make_float2(f4(A, 2 * h), f4(A, 2 * h + 1));
+                        const float2 p1 = make_float2(f4(P1, 2 * h), f4(P1, 2 * h + 1));
+                        const float2 pf = make_float2(f4(PF, 2 * h), f4(PF, 2 * h + 1));
+                        const float2 co = cmul(cconj(y3h(yo, k)), kold[k]);
+                        const float2 dho = make_float2(((p1.x - a.x) + (pf.x - a.x)) + co.x,
+                                                       ((p1.y - a.y) + (pf.y - a.y)) + co.y);   // dh_{t-1}
+                        Ac = make_float2((a.x + dho.x) - p1.x, (a.y + dho.y) - p1.y);            // g1h_{t-1}
+                        const float2 a3 = make_float2(Ac.x - f4(DL, 2 * h), Ac.y - f4(DL, 2 * h + 1));   // g3h_{t-1}
+                        const float2 Q = make_float2((f4(F1, 2 * h) - Ac.x) + (f4(F3, 2 * h) - a3.x),
+                                                     (f4(F1, 2 * h + 1) - Ac.y) + (f4(F3, 2 * h + 1) - a3.y));
                         const float2 Yk = y3h(Y, k);
-                        const float2 Dk = h ? make_float2(D.z, D.w) : make_float2(D.x, D.y);
-                        const float2 Qk = h ? make_float2(Q.z, Q.w) : make_float2(Q.x, Q.y);
-                        const int j = 2 * i + h;
                         aS[j] += Yk.x * Yk.x + Yk.y * Yk.y;
-                        aT[j] = cadd(aT[j], cmul(Yk, Dk));
-                        aP[j] = cadd(aP[j], cmul(Yk, Qk));
+                        aT[j] = cadd(aT[j], cmul(Yk, dho));
+                        aP[j] = cadd(aP[j], cmul(Yk, Q));
                     }
+                    an[2 * h] = Ac.x;
+                    an[2 * h + 1] = Ac.y;
                 }
+                st4(Acol, t, mk4(an));
             }
         }
         wave_lds_sync();
@@ -900,10 +945,14 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
         __syncthreads();
         const float2 *R = team_fft<false>(fa, fb, P.plan, tw, tid, THR, BlockSync{}, 1);
         const float c1 = P.sh_cdl;
-        for (int k = tid; k < H; k += THR) {
-            const float2 Gk = R[k];
-            const float den = accS[k] + c1;
-            accP[k] = make_float2((c1 * Gk.x - accP[k].x) / den, (c1 * Gk.y - accP[k].y) / den);
+        for (int k = tid; k < Hs; k += THR) {
+            float2 kp = make_float2(0.f, 0.f);
+            if (k < H) {
+                const float2 Gk = R[k];
+                const float den = accS[k] + c1;
+                kp = make_float2((c1 * Gk.x - accP[k].x) / den, (c1 * Gk.y - accP[k].y) / den);
+            }
+            gp(kap)[k] = kp;
         }
     }
     for (int l = tid; l < Lg; l += THR) {
@@ -917,128 +966,47 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
         gp(P.g2)[ro + l] = g2o + Td - G2n;
         gp(P.Td)[ro + l] = Td;
     }
-    __syncthreads();
-
-    // ---- pass 2: dh = Qh + conj(Y3h) kappa; g1h += dh - G1h (g3h += dh - G3h lives in
-    // g1h - g3h, which k_dict_finalize advances by G3h - G1h once for all sequences)
-    // With prefetch, the next column's g1h and stored Y3h are loaded before this column's
-    // arithmetic (the filter spectra G1h, G3h, g1h - g3h come from the cache at the column's start).
-    constexpr int NY = 2;                       // stored Y3h bins per lane: Ph <= 64 NY (Lg <= 1020)
-    float4 p1[kPf ? NQ : 1];
-    float2 py[kPf ? NY : 1];
-    auto load_next = [&](int m) {
-        const size_t colh = ((size_t)n * M + m) * Hs;
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) {
-            const int t = lane + 64 * i;
-            if (t < nq) p1[i] = ld4(reinterpret_cast<const float *>(P.g1h + colh), t);
-        }
-#pragma unroll
-        for (int i = 0; i < NY; ++i) {
-            const int k = lane + 64 * i;
-            if (k < Ph) py[i] = gp(y3s)[(size_t)m * Ph + k];
-        }
-    };
-    if constexpr (kPf) {
-        if (w < M) load_next(w);
-    }
-    for (int m = w; m < M; m += W) {
-        const int mm = __builtin_amdgcn_readfirstlane(m);
-        const size_t colh = ((size_t)n * M + mm) * Hs, fcol = (size_t)mm * Hs;
-        float *dcol = reinterpret_cast<float *>(P.dh + colh);
-        float *a1col = reinterpret_cast<float *>(P.g1h + colh);
-        // a3v / A3 first hold g1h - g3h, then g3h
-        float4 a1v[kPf ? NQ : 1], a3v[kPf ? NQ : 1], f1v[kPf ? NQ : 1], f3v[kPf ? NQ : 1];
-        auto load2 = [&](int i, float4 &A1, float4 &A3, float4 &F1, float4 &F3) {
-            const int t = lane + 64 * i;
-            if (t < nq) {
-                A1 = ld4(a1col, t);
-                A3 = ld4(D13h + 2 * fcol, t);
-                F1 = ld4(G1h + 2 * fcol, t);
-                F3 = ld4(G3h + 2 * fcol, t);
-            }
-        };
-        if constexpr (kPf) {
-#pragma unroll
-            for (int i = 0; i < NQ; ++i) {
-                a1v[i] = p1[i];
-                const int t = lane + 64 * i;
-                if (t < nq) {
-                    a3v[i] = ld4(D13h + 2 * fcol, t);
-                    f1v[i] = ld4(G1h + 2 * fcol, t);
-                    f3v[i] = ld4(G3h + 2 * fcol, t);
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < NY; ++i) {
-                const int k = lane + 64 * i;
-                if (k < Ph) ya[k] = py[i];
-            }
-            if (mm + W < M) load_next(mm + W);
-        } else {
-            for (int k = lane; k < Ph; k += 64) ya[k] = gp(y3s)[(size_t)mm * Ph + k];
-        }
-        wave_lds_sync();
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) {
-            float4 A1, A3, F1, F3;
-            if constexpr (kPf) { A1 = a1v[i]; A3 = a3v[i]; F1 = f1v[i]; F3 = f3v[i]; } else { load2(i, A1, A3, F1, F3); }
-            const int t = lane + 64 * i;
-            if (t < nq) {
-                A3 = sub4(A1, A3);   // g3h = g1h - (g1h - g3h)
-                float dn[4], g1n[4];
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int k = 2 * t + h;
-                    const float q0 = (f4(F1, 2 * h) - f4(A1, 2 * h)) + (f4(F3, 2 * h) - f4(A3, 2 * h));
-                    const float q1 = (f4(F1, 2 * h + 1) - f4(A1, 2 * h + 1)) + (f4(F3, 2 * h + 1) - f4(A3, 2 * h + 1));
-                    float2 dk = make_float2(0.f, 0.f);
-                    if (k < H) {
-                        const float2 c = cmul(cconj(y3h(ya, k)), accP[k]);
-                        dk = make_float2(q0 + c.x, q1 + c.y);
-                    }
-                    dn[2 * h] = dk.x;
-                    dn[2 * h + 1] = dk.y;
-                    g1n[2 * h] = f4(A1, 2 * h) + dk.x - f4(F1, 2 * h);               // train.jl:70
-                    g1n[2 * h + 1] = f4(A1, 2 * h + 1) + dk.y - f4(F1, 2 * h + 1);
-                }
-                st4(dcol, t, mk4(dn));
-                st4(a1col, t, mk4(g1n));
-            }
-        }
-        wave_lds_sync();
-    }
 }
 
-// Partial sums over a chunk of sequences of (dh + g1h) and dh, per (bin, m) (train.jl:76,83,
-// train.jl:95: the means of d + g1, d + g3 and d, taken on the spectra; the mean of d + g3 is
-// that of d + g1 less g1 - g3, which is the same for every sequence), in double, real and
-// imaginary parts.  Blocks of a problem: (tile, chunk).
+// Partial sums over a chunk of sequences, per (bin, m), of A = g1h_{t-1} and of
+// c = conj(Y3h_t) kappa_t (the same float product the next step forms), in double, real and
+// imaginary parts: the means over n of d, d + g1 and d + g3 (train.jl:76,83,95) follow from
+// them (k_dict_finalize).  Blocks of a problem: (tile, chunk).
 __global__ __launch_bounds__(kThreads) void k_dict_reduce(const CdlDev *__restrict__ tab, const int *__restrict__ starts,
                                                           int np) {
     int b;
     const CdlDev &P = tab[find_problem(starts, np, blockIdx.x, b)];
-    const int HM = half_bins(P.Lg) * P.M;
+    const int Lg = P.Lg, Hs = half_bins(Lg), H = Lg / 2 + 1, M = P.M, HM = Hs * M;
+    const int Pq = Lg >> 2, Ph = Pq / 2 + 1;
     const int tiles = (HM + kThreads - 1) / kThreads;
     const int c = b / tiles, tile = b - c * tiles;
     const int i = tile * kThreads + threadIdx.x;
     if (i >= HM) return;
+    const int m = i / Hs, k = i - m * Hs;
+    int kq = k;
+    while (kq >= Pq) kq -= Pq;
+    const bool cj = kq >= Ph;
+    const int yi = cj ? Pq - kq : kq;
     const int N = P.N, nch = P.nch;
     const int n0 = (int)((long)c * N / nch), n1 = (int)((long)(c + 1) * N / nch);
-    double a1x = 0.0, a1y = 0.0, adx = 0.0, ady = 0.0;
-    for (int n = n0; n < n1; ++n) {
-        const size_t e = (size_t)n * HM + i;
-        const float2 dv = gp(P.dh)[e], v1 = gp(P.g1h)[e];
-        a1x += (double)(dv.x + v1.x);
-        a1y += (double)(dv.y + v1.y);
-        adx += (double)dv.x;
-        ady += (double)dv.y;
+    double ax = 0.0, ay = 0.0, cx = 0.0, cy = 0.0;
+    if (k < H) {
+        for (int n = n0; n < n1; ++n) {
+            const float2 a = gp(P.g1h)[(size_t)n * HM + i];
+            float2 y = gp(P.y3s)[((size_t)n * M + m) * Ph + yi];
+            if (cj) y.y = -y.y;
+            const float2 cc = cmul(cconj(y), gp(P.kap)[(size_t)n * Hs + k]);
+            ax += (double)a.x;
+            ay += (double)a.y;
+            cx += (double)cc.x;
+            cy += (double)cc.y;
+        }
     }
     double *o = gp(P.dpart) + (size_t)c * kDpart * HM + i;
-    o[0 * (size_t)HM] = a1x;
-    o[1 * (size_t)HM] = a1y;
-    o[2 * (size_t)HM] = adx;
-    o[3 * (size_t)HM] = ady;
+    o[0 * (size_t)HM] = ax;
+    o[1 * (size_t)HM] = ay;
+    o[2 * (size_t)HM] = cx;
+    o[3 * (size_t)HM] = cy;
 }
 
 // Column m: the mean spectra of (d+g1), (d+g3) and d back to the spatial domain (one packed
@@ -1046,7 +1014,8 @@ __global__ __launch_bounds__(kThreads) void k_dict_reduce(const CdlDev *__restri
 // G3 = projection2box!(v2) (cdl_kernels.jl:37-51); G1new = prox_simplex_zeros!(v1) (:53-66,
 // FP64 combination, quirk group 4*floor(l/4)) into the non-current G1 buffer; meand =
 // mean(d) (D_best candidate, train.jl:95); gsq = |F(G1new)|^2 and the half spectra of G1new
-// and G3.
+// and G3.  This step's filter spectra are kept as G1hp and Fp = G3h + d13h (the dictionary
+// state refers to them) before they are replaced.
 __global__ __launch_bounds__(kThreads) void k_dict_finalize(const CdlDev *__restrict__ tab, const int *__restrict__ starts,
                                                             int np, int LgMax) {
     extern __shared__ float4 smem4[];
@@ -1064,15 +1033,21 @@ __global__ __launch_bounds__(kThreads) void k_dict_finalize(const CdlDev *__rest
     const int nxt = (P.sc->g1cur & 1) ^ 1;
     float *G1new = P.G1b[nxt];
     float2 *G1hnew = P.G1hb[nxt];
-    // mean spectrum j (0: d+g1, 1: d) at bin k < H, less sub
-    auto mean = [&](int j, int k, float2 sub) {
+    // means over n at bin k < H of d (j = 1) or d + g1 (j = 0), with this step's G1h = f1 and
+    // F = G3h + d13h (old): from S_A = mean(A), S_c = mean(conj(Y3h) kappa),
+    //   mean(dh) = G1h + F - 2 S_A + S_c,   mean(g1h_new) = F - S_A + S_c
+    auto mean = [&](int j, int k, float2 f1, float2 F, float2 sub) {
         const size_t i = (size_t)m * Hs + k;
-        double sx = 0.0, sy = 0.0;
-        for (int c = 0; c < P.nch; ++c) {
-            sx += gp(P.dpart)[((size_t)c * kDpart + 2 * j) * HM + i];
-            sy += gp(P.dpart)[((size_t)c * kDpart + 2 * j + 1) * HM + i];
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int c = 0; c < P.nch; ++c)
+            for (int q = 0; q < 4; ++q) s[q] += gp(P.dpart)[((size_t)c * kDpart + q) * HM + i];
+        const double ax = s[0] * invN, ay = s[1] * invN, cx = s[2] * invN, cy = s[3] * invN;
+        double dx = (double)f1.x + (double)F.x - 2.0 * ax + cx, dy = (double)f1.y + (double)F.y - 2.0 * ay + cy;
+        if (j == 0) {
+            dx += (double)F.x - ax + cx;
+            dy += (double)F.y - ay + cy;
         }
-        return make_float2((float)(sx * invN - (double)sub.x), (float)(sy * invN - (double)sub.y));
+        return make_float2((float)(dx - (double)sub.x), (float)(dy - (double)sub.y));
     };
     for (int l = tid; l < Lg; l += kThreads) tw[l] = gp(P.tw)[l];
     // v1 + i v2 = F^-1(V1 + i V2) for Hermitian V1, V2: the means of d + g1 and of
@@ -1081,12 +1056,20 @@ __global__ __launch_bounds__(kThreads) void k_dict_finalize(const CdlDev *__rest
     // G3h - G1h with this step's filters, read before G3h is replaced below.
     const float2 *G1hcur = P.G1hb[nxt ^ 1];
     const float2 zero2 = make_float2(0.f, 0.f);
-    for (int k = tid; k < H; k += kThreads) {
+    for (int k = tid; k < Hs; k += kThreads) {
         const size_t i = (size_t)m * Hs + k;
+        if (k >= H) {
+            gp(P.G1hp)[i] = zero2;
+            gp(P.Fp)[i] = zero2;
+            continue;
+        }
         const float2 dl = gp(P.d13h)[i], f1 = gp(G1hcur)[i], f3 = gp(P.G3h)[i];
+        const float2 F = make_float2(f3.x + dl.x, f3.y + dl.y);
+        gp(P.G1hp)[i] = f1;
+        gp(P.Fp)[i] = F;
         const float2 dn = make_float2(dl.x - (f1.x - f3.x), dl.y - (f1.y - f3.y));
         gp(P.d13h)[i] = dn;
-        const float2 a = mean(0, k, zero2), b = mean(0, k, dn);
+        const float2 a = mean(0, k, f1, F, zero2), b = mean(0, k, f1, F, dn);
         fa[k] = make_float2(a.x - b.y, a.y + b.x);
         if (k > 0 && Lg - k >= H) fa[Lg - k] = make_float2(a.x + b.y, b.x - a.y);
     }
@@ -1106,7 +1089,8 @@ __global__ __launch_bounds__(kThreads) void k_dict_finalize(const CdlDev *__rest
     }
     __syncthreads();
     for (int k = tid; k < H; k += kThreads) {
-        const float2 a = mean(1, k, zero2);
+        const size_t i = (size_t)m * Hs + k;
+        const float2 a = mean(1, k, gp(P.G1hp)[i], gp(P.Fp)[i], zero2);
         fa[k] = a;
         if (k > 0 && Lg - k >= H) fa[Lg - k] = cconj(a);
     }
@@ -1173,23 +1157,36 @@ __global__ __launch_bounds__(kThreads) void k_g1_spectrum(const CdlDev *__restri
     }
 }
 
-// One column (n, m) of a half-spectrum array back to the spatial domain (field reads of d,
-// g1, g3): out[(n M + m) Lg + l] = F^-1 of the Hermitian extension.  LDS: twiddles | a | b.
-__global__ __launch_bounds__(kThreads) void k_half_to_spatial(const CdlDev *__restrict__ tab, const float2 *__restrict__ src,
-                                                              const float2 *__restrict__ sub, float *__restrict__ out) {
+// One column (n, m) of d (what = 0), g1 (1) or g3 (2) in the spatial domain (field reads):
+// their half spectra from the dictionary state (dh_t = (G1hp - A) + (Fp - A) + conj(Y3h) kappa,
+// g1h_t = (A + dh_t) - G1hp, g3h_t = g1h_t - d13h, as k_dict rebuilds them), then the inverse
+// transform of the Hermitian extension.  out[(n M + m) Lg + l].  LDS: twiddles | a | b.
+__global__ __launch_bounds__(kThreads) void k_dict_to_spatial(const CdlDev *__restrict__ tab, int what,
+                                                              float *__restrict__ out) {
     extern __shared__ float4 smem4[];
     const CdlDev &P = tab[0];
     const int Lg = P.Lg, H = Lg / 2 + 1, Hs = half_bins(Lg), tid = threadIdx.x;
-    const size_t c = blockIdx.x;
+    const int Pq = Lg >> 2, Ph = Pq / 2 + 1;
+    const size_t c = blockIdx.x, n = c / (size_t)P.M, m = c % (size_t)P.M;
     float2 *tw = reinterpret_cast<float2 *>(smem4);
     float2 *fa = tw + Lg;
     float2 *fb = fa + Lg;
     for (int l = tid; l < Lg; l += kThreads) tw[l] = gp(P.tw)[l];
     for (int k = tid; k < H; k += kThreads) {
-        float2 v = gp(src)[c * Hs + k];
-        if (sub) {   // per filter: column c holds filter c mod M
-            const float2 u = gp(sub)[(c % (size_t)P.M) * Hs + k];
-            v = make_float2(v.x - u.x, v.y - u.y);
+        int kq = k;
+        while (kq >= Pq) kq -= Pq;
+        float2 y = gp(P.y3s)[c * Ph + (kq < Ph ? kq : Pq - kq)];
+        if (kq >= Ph) y.y = -y.y;
+        const float2 co = cmul(cconj(y), gp(P.kap)[n * Hs + k]);
+        const float2 a = gp(P.g1h)[c * Hs + k], p1 = gp(P.G1hp)[m * Hs + k], pf = gp(P.Fp)[m * Hs + k];
+        const float2 dh = make_float2(((p1.x - a.x) + (pf.x - a.x)) + co.x, ((p1.y - a.y) + (pf.y - a.y)) + co.y);
+        float2 v = dh;
+        if (what > 0) {
+            v = make_float2((a.x + dh.x) - p1.x, (a.y + dh.y) - p1.y);
+            if (what == 2) {
+                const float2 u = gp(P.d13h)[m * Hs + k];
+                v = make_float2(v.x - u.x, v.y - u.y);
+            }
         }
         fa[k] = v;
         if (k > 0 && Lg - k >= H) fa[Lg - k] = cconj(v);

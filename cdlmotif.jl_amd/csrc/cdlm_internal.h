@@ -28,6 +28,12 @@ cdlm_status cdlm_pwm_scratch_init(cdlm_ctx *ctx);
 // Records an error message on the context and returns the status.
 cdlm_status cdlm_fail(cdlm_ctx *ctx, cdlm_status st, const char *fmt, ...);
 
+// cdlm_problem_create from the one-hot matrix, with every device buffer carved from a caller-owned
+// arena of at least cdlm_footprint_bytes(L4, N, M, gamma, 0) bytes, which the problem does not
+// free (the server recycles finished jobs' memory this way instead of freeing it).
+cdlm_status cdlm_problem_create_in(cdlm_ctx *ctx, const cdlm_params *prm, const float *s, int32_t L4, int32_t N,
+                                   const float *D0, uint64_t seed, void *arena, size_t arena_bytes, cdlm_problem **out);
+
 // Device pointer of a problem's current X_best slice (L, M, N) and its dimensions; reads the
 // slice index from the device (synchronises the context stream).
 cdlm_status cdlm_problem_xbest_device(cdlm_problem *p, const float **slice, int *L, int *M, int *N);

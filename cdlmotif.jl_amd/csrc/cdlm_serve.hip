@@ -19,8 +19,12 @@
 //     as one batch (one launch per kernel per shape class, replayed from a HIP graph while
 //     the membership is unchanged);
 //   * a job that completed its iter_num outer iterations copies its results (objective
-//     trace, code threshold, optionally the X_best slice) to host memory and frees its
-//     device memory before the next admission.
+//     trace, code threshold, optionally the X_best slice) to host memory; its device memory
+//     (one arena per job) goes to a pool that later admissions take from (the smallest arena
+//     that holds the job), so device memory is freed only when a new job fits the budget
+//     but no pooled arena holds it.  Freeing device memory synchronises the device and
+//     leaves driver work that slows the next seconds of kernels, so recycling it keeps
+//     both out of the ticks.
 // A job's arithmetic does not depend on which other jobs share its ticks, so its results
 // are the ones it gets alone.
 
@@ -40,7 +44,7 @@
 #include <thread>
 #include <vector>
 
-#include "../../include/cdlm.h"
+#include "cdlm_internal.h"
 
 namespace {
 
@@ -62,6 +66,8 @@ struct Job {
     int32_t state = CDLM_JOB_QUEUED;
     int bypassed = 0;      // later arrivals admitted before this job
     cdlm_problem *p = nullptr;
+    void *arena = nullptr; // device memory of the job's problem (recycled or new)
+    size_t arena_bytes = 0;
     int outer_done = 0;
     Clock::time_point t_submit, t_admit, t_done;
     // results
@@ -86,7 +92,9 @@ struct cdlm_server {
     std::map<uint64_t, std::unique_ptr<Job>> jobs;
     std::deque<uint64_t> queue;
     std::vector<Job *> active;        // server thread only (under mu when read by clients)
-    size_t in_use = 0;
+    size_t in_use = 0;                // bytes of the resident jobs' arenas
+    std::vector<std::pair<void *, size_t>> pool;   // arenas of finished jobs, kept for reuse
+    size_t pool_bytes = 0;
     uint64_t next_id = 1;
     bool stop = false;
     std::thread th;
@@ -118,10 +126,12 @@ void fail_job(cdlm_server *srv, Job *j, const std::string &why) {
     srv->cv_done.notify_all();
 }
 
-// Admission (server thread, under mu): returns the jobs to create.
-std::vector<Job *> pick_admissions(cdlm_server *srv) {
+// Admission (server thread, under mu): returns the jobs to create, each with a recycled arena
+// (the smallest pooled one that holds it) or none (a new one of its footprint is allocated);
+// pooled arenas that must be freed to make room for new ones are moved to *to_free.  The
+// device memory held, resident arenas plus the pool, never exceeds the budget.
+std::vector<Job *> pick_admissions(cdlm_server *srv, std::vector<void *> *to_free) {
     std::vector<Job *> out;
-    size_t free_b = srv->budget > srv->in_use ? srv->budget - srv->in_use : 0;
     int slots = srv->opts.max_active > 0 ? srv->opts.max_active - (int)srv->active.size() : 1 << 30;
     long seqs = 0;
     for (Job *j : srv->active) seqs += j->N;
@@ -138,9 +148,34 @@ std::vector<Job *> pick_admissions(cdlm_server *srv) {
     bool blocked = false;   // a starving job does not fit: nothing passes it
     for (Job *j : cand) {
         if (slots <= 0 || (srv->fill > 0 && seqs >= srv->fill) || blocked) break;
-        if (j->footprint <= free_b) {
-            free_b -= j->footprint;
-            srv->in_use += j->footprint;
+        int best = -1;
+        for (size_t i = 0; i < srv->pool.size(); ++i)
+            if (srv->pool[i].second >= j->footprint && (best < 0 || srv->pool[i].second < srv->pool[(size_t)best].second))
+                best = (int)i;
+        bool admit = false;
+        if (best >= 0) {
+            j->arena = srv->pool[(size_t)best].first;
+            j->arena_bytes = srv->pool[(size_t)best].second;
+            srv->pool.erase(srv->pool.begin() + best);
+            srv->pool_bytes -= j->arena_bytes;
+            admit = true;
+        } else if (srv->in_use + j->footprint <= srv->budget) {
+            // free pooled arenas, largest first, until a new one of the footprint fits
+            while (srv->in_use + srv->pool_bytes + j->footprint > srv->budget && !srv->pool.empty()) {
+                auto big = std::max_element(srv->pool.begin(), srv->pool.end(),
+                                            [](const std::pair<void *, size_t> &a, const std::pair<void *, size_t> &b) {
+                                                return a.second < b.second;
+                                            });
+                to_free->push_back(big->first);
+                srv->pool_bytes -= big->second;
+                srv->pool.erase(big);
+            }
+            j->arena = nullptr;
+            j->arena_bytes = j->footprint;
+            admit = true;
+        }
+        if (admit) {
+            srv->in_use += j->arena_bytes;
             seqs += j->N;
             --slots;
             j->state = CDLM_JOB_RUNNING;
@@ -159,6 +194,24 @@ std::vector<Job *> pick_admissions(cdlm_server *srv) {
     return out;
 }
 
+// Gives a job's arena back: to the pool (reuse = true) or to the device.
+void return_arena(cdlm_server *srv, Job *j, bool reuse) {
+    void *a = j->arena;
+    const size_t b = j->arena_bytes;
+    j->arena = nullptr;
+    j->arena_bytes = 0;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->in_use -= b;
+        if (a && reuse) {
+            srv->pool.emplace_back(a, b);
+            srv->pool_bytes += b;
+            return;
+        }
+    }
+    if (a) (void)hipFree(a);
+}
+
 void finish_job(cdlm_server *srv, Job *j) {
     double th = 0.0;
     int64_t cnt = 0;
@@ -171,10 +224,10 @@ void finish_job(cdlm_server *srv, Job *j) {
         st = cdlm_get(j->p, CDLM_F_XBEST_SLICE, codes.data(), codes.size() * sizeof(float));
     }
     const std::string err = st == CDLM_OK ? std::string() : ctx_error(srv);
-    cdlm_problem_destroy(j->p);
+    cdlm_problem_destroy(j->p);   // the arena is the server's: it goes back to the pool
     j->p = nullptr;
+    return_arena(srv, j, st == CDLM_OK);
     std::lock_guard<std::mutex> lk(srv->mu);
-    srv->in_use -= j->footprint;
     j->t_done = Clock::now();
     if (st == CDLM_OK) {
         j->thresh = th;
@@ -238,23 +291,33 @@ bool tick(cdlm_server *srv) {
 }
 
 void server_loop(cdlm_server *srv) {
+    (void)hipSetDevice(srv->ctx->device);   // arenas are allocated from this thread
     for (;;) {
         std::vector<Job *> adm;
+        std::vector<void *> to_free;
         {
             std::unique_lock<std::mutex> lk(srv->mu);
             srv->cv_work.wait(lk, [&] { return srv->stop || !srv->active.empty() || !srv->queue.empty(); });
             if (srv->stop) break;
-            adm = pick_admissions(srv);
+            adm = pick_admissions(srv, &to_free);
         }
+        for (void *a : to_free) (void)hipFree(a);
         for (Job *j : adm) {
-            cdlm_problem *p = nullptr;
-            cdlm_status st = cdlm_problem_create(srv->ctx, &j->prm, j->s.data(), j->L4, j->N,
-                                                 j->D0.empty() ? nullptr : j->D0.data(), nullptr, j->seed, &p);
-            if (st != CDLM_OK) {
-                {
-                    std::lock_guard<std::mutex> lk(srv->mu);
-                    srv->in_use -= j->footprint;
+            if (!j->arena) {
+                if (hipMalloc(&j->arena, j->arena_bytes) != hipSuccess) {
+                    (void)hipGetLastError();
+                    j->arena = nullptr;
+                    return_arena(srv, j, false);
+                    fail_job(srv, j, "no device memory for the job's arena");
+                    continue;
                 }
+            }
+            cdlm_problem *p = nullptr;
+            cdlm_status st = cdlm_problem_create_in(srv->ctx, &j->prm, j->s.data(), j->L4, j->N,
+                                                    j->D0.empty() ? nullptr : j->D0.data(), j->seed, j->arena,
+                                                    j->arena_bytes, &p);
+            if (st != CDLM_OK) {
+                return_arena(srv, j, true);
                 fail_job(srv, j, ctx_error(srv));
                 continue;
             }
@@ -295,10 +358,7 @@ void server_loop(cdlm_server *srv) {
             for (Job *j : act) {
                 if (j->p) cdlm_problem_destroy(j->p);
                 j->p = nullptr;
-                {
-                    std::lock_guard<std::mutex> lk(srv->mu);
-                    srv->in_use -= j->footprint;
-                }
+                return_arena(srv, j, false);
                 fail_job(srv, j, err);
             }
         }
@@ -313,8 +373,16 @@ void server_loop(cdlm_server *srv) {
     for (Job *j : act) {
         if (j->p) cdlm_problem_destroy(j->p);
         j->p = nullptr;
+        return_arena(srv, j, false);
         fail_job(srv, j, "server stopped");
     }
+    std::vector<std::pair<void *, size_t>> pool;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        pool.swap(srv->pool);
+        srv->pool_bytes = 0;
+    }
+    for (auto &a : pool) (void)hipFree(a.first);
     std::lock_guard<std::mutex> lk(srv->mu);
     for (uint64_t id : srv->queue) {
         Job *j = srv->jobs[id].get();
@@ -515,7 +583,7 @@ extern "C" cdlm_status cdlm_server_stats(cdlm_server *srv, cdlm_server_stats_t *
     out->ticks = srv->ticks;
     out->p50_s = quantile(srv->latencies, 0.50);
     out->p99_s = quantile(srv->latencies, 0.99);
-    out->bytes_in_use = srv->in_use;
+    out->bytes_in_use = srv->in_use + srv->pool_bytes;   // resident jobs' arenas and the pool
     out->budget_bytes = srv->budget;
     return CDLM_OK;
 }

@@ -176,8 +176,9 @@ cdlm_status cdlm_batch_timing_read(cdlm_batch *b, int32_t kernel, double *total_
  * device (fill_seqs sequences); every tick advances all resident jobs by
  * one outer iteration (csc_iter CSC iterations, update_best, cdl_iter dictionary
  * iterations) in one batch; a finished job's results move to host memory and its device
- * memory is freed for the next admission.  A job's results do not depend on the jobs it
- * shares the device with.  The calls below are thread-safe. */
+ * memory (one arena per job) is kept for later admissions, freed only when a new arena must be
+ * allocated within the budget.  A job's results do not depend on the jobs it shares the device
+ * with.  The calls below are thread-safe. */
 typedef struct cdlm_server cdlm_server;
 typedef struct {
     uint64_t mem_budget;   /* device bytes for resident jobs (0: 80% of the free memory at start) */

@@ -32,7 +32,9 @@ def test_footprint_is_host_arithmetic():
     L = _lib.load()
     b = L.cdlm_footprint_bytes(800, 20000, 64, 20, 0)
     E = 840 * 64 * 20000
-    assert 5 * E * 4 < b < 6 * E * 4   # Xt, y2, y3 (0.75), d and g1 half spectra (~1 each), x0 not kept
+    # Xt, y2, y3 (0.75), the g1 half spectra (~1), the stored Y3 half spectra (~0.25), two X_best
+    # slices (2 L / Lg = 0.48); x0 not kept
+    assert 4 * E * 4 < b < 5 * E * 4
     assert L.cdlm_version().decode().startswith("cdlm")
 
 
