@@ -88,6 +88,9 @@ def load():
         "cdlm_mem_info": ([vp, P(C.c_size_t), P(C.c_size_t)], C.c_int),
         "cdlm_problem_create": ([vp, P(cdlm_params), vp, i32, i32, vp, vp, C.c_uint64, P(vp)], C.c_int),
         "cdlm_problem_create_from_bases": ([vp, P(cdlm_params), vp, i32, i32, vp, vp, C.c_uint64, P(vp)], C.c_int),
+        "cdlm_problem_create_streamed": ([vp, P(cdlm_params), vp, i32, i32, vp, vp, C.c_uint64, C.c_uint64, P(vp)],
+                                         C.c_int),
+        "cdlm_problem_chunks": ([vp, P(i32)], C.c_int),
         "cdlm_problem_destroy": ([vp], C.c_int),
         "cdlm_problem_reset": ([vp], C.c_int),
         "cdlm_problem_dims": ([vp, P(i32), P(i32), P(i32), P(i32), P(i32)], C.c_int),

@@ -60,9 +60,12 @@ class CDL:
     def __init__(self, data: Optional[np.ndarray], filter_len: Sequence[int], rho: float, sigma: float, lam: float,
                  D0: Optional[np.ndarray] = None, X0: Optional[np.ndarray] = None, seed: int = 0,
                  sherman_csc: float = 2.0, sherman_cdl: float = 2.0, ctx: Optional[Context] = None,
-                 bases: Optional[np.ndarray] = None):
+                 bases: Optional[np.ndarray] = None, stream_budget: Optional[int] = None):
         """data: (N, 4L) one-hot float32; or bases: (N, L) uint8 codes 0..3 (one-hot built on the
-        device, cdlm_problem_create_from_bases)."""
+        device, cdlm_problem_create_from_bases).  stream_budget: device bytes the problem may use
+        (0: 80% of the free device memory); a problem whose footprint exceeds it keeps its state
+        in pinned host memory and moves its sequences through the device in chunks
+        (cdlm_problem_create_streamed; data only), with results equal to the resident problem's."""
         self.ctx = ctx or default_context()
         L = load()
         if bases is not None:
@@ -85,7 +88,12 @@ class CDL:
         h = C.c_void_p()
         d0p = None if d0 is None else d0.ctypes.data
         x0p = None if x0 is None else x0.ctypes.data
-        if bases is not None:
+        if stream_budget is not None:
+            if bases is not None:
+                raise ValueError("stream_budget takes one-hot data, not bases")
+            st = L.cdlm_problem_create_streamed(self.ctx.h, C.byref(prm), data.ctypes.data, L4, N, d0p, x0p,
+                                                seed & 0xFFFFFFFFFFFFFFFF, int(stream_budget), C.byref(h))
+        elif bases is not None:
             st = L.cdlm_problem_create_from_bases(self.ctx.h, C.byref(prm), bases.ctypes.data, L4 // 4, N, d0p, x0p,
                                                   seed & 0xFFFFFFFFFFFFFFFF, C.byref(h))
         else:
@@ -119,6 +127,13 @@ class CDL:
     @property
     def device_bytes(self) -> int:
         return int(load().cdlm_problem_bytes(self.h))
+
+    @property
+    def chunks(self) -> int:
+        """Chunks the sequences move through the device in (1: resident)."""
+        c = C.c_int32()
+        check(load().cdlm_problem_chunks(self.h, C.byref(c)), self.ctx.h)
+        return c.value
 
     # -- reference methods
     def _check_sherman(self, name: str, value: Optional[float], fixed: float):

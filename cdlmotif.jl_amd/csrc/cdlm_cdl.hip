@@ -130,14 +130,18 @@ int pick_dict_batch(int Lg, int taps) {
     return best;
 }
 
-// Chunks of sequences in the dictionary mean: about 2048 blocks per problem.
-int pick_nch(int Lg, int M, int N) {
-    const size_t LM = (size_t)Lg * M;
-    const size_t blocks = (LM + kThreads - 1) / kThreads;
-    int nch = (int)((2048 + blocks - 1) / blocks);
-    if (nch < 1) nch = 1;
-    if (nch > N) nch = N;
-    return nch;
+// Chunks of sequences in the dictionary mean: about 4096 blocks per problem and at least 32
+// chunks, a multiple of the 8 XCDs (k_dict_reduce keeps every tile of a chunk on one XCD, so the
+// chunk's stored Y3 half spectra and kappa, which all its tiles read, come from HBM once into that
+// XCD's L2); chunks may be empty when N is small.  A streamed problem moves through the device in
+// a divisor of this many chunks, so that its means are summed exactly as when it is resident.
+constexpr int kXcds = 8;
+int pick_nch(int Lg, int M) {
+    const size_t HM = (size_t)half_bins(Lg) * M;
+    const size_t tiles = (HM + kThreads - 1) / kThreads;
+    int nch = (int)((4096 + tiles - 1) / tiles);
+    nch = (nch + kXcds - 1) / kXcds * kXcds;
+    return nch < 32 ? 32 : nch;
 }
 
 // Dynamic LDS above the default needs hipFuncSetAttribute.  The attribute is per kernel
@@ -198,6 +202,19 @@ struct cdlm_problem {
     Timer timer;
     Tuning tune;         // launch choices in force for this problem
     int batches = 0;     // batches that reference this problem
+    // Streamed: the state of all N sequences lives in pinned host memory and the sequences move
+    // through a device window of `win` sequences in nchunk chunks (0: a resident problem).
+    struct HostArr {
+        char *dev = nullptr;    // the window's buffer
+        char *host = nullptr;   // N * per_seq bytes
+        size_t per_seq = 0;
+    };
+    int nchunk = 0, win = 0;
+    std::vector<int> cn0;              // chunk k: sequences [cn0[k], cn0[k+1])
+    std::vector<LaunchSet> csets;      // chunk k's launch set: its N, n_off and mean chunks
+    std::vector<HostArr> hstate;       // moved in and out around every chunk pass
+    HostArr hs, hx0, hxs[2];           // data and initial codes (in only), X_best slices (out)
+    std::vector<void *> pinned;
 };
 
 struct cdlm_batch {
@@ -227,7 +244,10 @@ struct BufPlan {
     size_t s, E, Eh, E3, Ex0, LN, LM, LMh, Lg, xs, y3s, kap, M, partials, dpart, trace, sc;
 };
 
-BufPlan buf_plan(int L4, int N, int M, int gamma, bool keep_x0) {
+// N sequences resident; the per-sequence objective partials are held for Ntot (a streamed
+// problem's window holds a chunk of Ntot sequences; resident: Ntot = N).
+BufPlan buf_plan(int L4, int N, int M, int gamma, bool keep_x0, int Ntot = 0) {
+    if (Ntot <= 0) Ntot = N;
     BufPlan b;
     const size_t Lg = (size_t)L4 + 2 * (size_t)gamma;
     b.s = (size_t)L4 * N * sizeof(float);
@@ -243,8 +263,8 @@ BufPlan buf_plan(int L4, int N, int M, int gamma, bool keep_x0) {
     b.y3s = (size_t)M * N * (Lg / 8 + 1) * sizeof(float2);
     b.kap = (size_t)half_bins((int)Lg) * N * sizeof(float2);
     b.M = (size_t)M * sizeof(int);
-    b.partials = (size_t)N * kPartStride * sizeof(double);
-    b.dpart = (size_t)pick_nch((int)Lg, M, N) * kDpart * ((size_t)half_bins((int)Lg) * M) * sizeof(double);
+    b.partials = (size_t)Ntot * kPartStride * sizeof(double);
+    b.dpart = (size_t)pick_nch((int)Lg, M) * kDpart * ((size_t)half_bins((int)Lg) * M) * sizeof(double);
     b.trace = (size_t)kTraceCap * sizeof(double);
     b.sc = sizeof(CdlScal);
     return b;
@@ -280,6 +300,10 @@ cdlm_status palloc_t(cdlm_problem *p, T **out, size_t bytes) {
 
 void free_problem(cdlm_problem *p) {
     p->set.release();
+    for (auto &cs : p->csets) cs.release();
+    p->csets.clear();
+    for (void *h : p->pinned) (void)hipHostFree(h);
+    p->pinned.clear();
     p->timer.clear();
     if (p->own_arena && p->arena) (void)hipFree(p->arena);
     p->arena = nullptr;
@@ -362,10 +386,10 @@ cdlm_status set_build(cdlm_ctx *ctx, cdlm_problem *const *ps, int np, LaunchSet 
         pre[3 * np + i] = (int)a_red;
         pre[4 * np + i] = slots ? slots[i] : i;
         const long HM = (long)half_bins(p->Lg) * p->M;
-        a_seq += p->N;
+        a_seq += p->dv.N;
         a_filt += p->M;
         a_cs += (p->Lg + kThreads - 1) / kThreads;
-        a_red += ((HM + kThreads - 1) / kThreads) * p->dv.nch;
+        a_red += ((HM + kThreads - 1) / kThreads) * p->dv.red_cn;
     }
     if (a_seq > INT32_MAX || a_red > INT32_MAX) return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "batch too large for one grid");
     T.blk_seq = (int)a_seq;
@@ -410,17 +434,23 @@ cdlm_status run_g1_cs(cdlm_ctx *ctx, const LaunchSet &S, int flip) {
 
 // Objective partials, their reduction and the update_best decision.  decide = 0 only
 // records the objective; slot < 0 writes no trace entry.
-cdlm_status run_best(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm, int force, int decide, int slot, float *Xfull) {
+cdlm_status run_best_seq(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm, float *Xfull) {
     hipStream_t st = set_stream(ctx, S);
-    {
-        TimedLaunch tl(tm, st, 2);
+    TimedLaunch tl(tm, st, 2);
 #define L_BEST(T) hipLaunchKernelGGL(k_best<T>, dim3(S.blk_seq), dim3(kThreads), S.lds_best, st, S.tab, S.st_seq, S.np, Xfull)
-        CDLM_TAPS_SWITCH(S.taps, L_BEST);
+    CDLM_TAPS_SWITCH(S.taps, L_BEST);
 #undef L_BEST
-    }
+    return lcheck(ctx, "k_best");
+}
+cdlm_status run_best_global(cdlm_ctx *ctx, const LaunchSet &S, int force, int decide, int slot) {
+    hipStream_t st = set_stream(ctx, S);
     hipLaunchKernelGGL(k_reduce_partials, dim3(S.np), dim3(kThreads), 0, st, S.tab);
     hipLaunchKernelGGL(k_decide, dim3(S.np), dim3(kThreads), 0, st, S.tab, force, decide, slot);
-    return lcheck(ctx, "k_best/k_decide");
+    return lcheck(ctx, "k_decide");
+}
+cdlm_status run_best(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm, int force, int decide, int slot, float *Xfull) {
+    cdlm_status s = run_best_seq(ctx, S, tm, Xfull);
+    return s != CDLM_OK ? s : run_best_global(ctx, S, force, decide, slot);
 }
 
 // iters CSC iterations in one launch (the timer counts iterations).
@@ -448,7 +478,8 @@ cdlm_status run_csc(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm, int iters) {
     return lcheck(ctx, "k_csc");
 }
 
-cdlm_status run_dict(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm) {
+// The per-sequence part of a dictionary iteration (k_dict and the partial sums of the means).
+cdlm_status run_dict_seq(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm) {
     hipStream_t st = set_stream(ctx, S);
     {
         TimedLaunch tl(tm, st, 1);
@@ -466,23 +497,38 @@ cdlm_status run_dict(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm) {
         TimedLaunch tl(tm, st, 3);
         hipLaunchKernelGGL(k_dict_reduce, dim3(S.blk_red), dim3(kThreads), 0, st, S.tab, S.st_red, S.np);
     }
+    return lcheck(ctx, "k_dict");
+}
+// The per-filter part: the means back to the filters, the new filter spectra.
+cdlm_status run_dict_filt(cdlm_ctx *ctx, const LaunchSet &S) {
+    hipStream_t st = set_stream(ctx, S);
     hipLaunchKernelGGL(k_dict_finalize, dim3(S.blk_filt), dim3(kThreads), S.lds_fin, st, S.tab, S.st_filt, S.np, S.LgMax);
-    cdlm_status s = lcheck(ctx, "k_dict");
+    cdlm_status s = lcheck(ctx, "k_dict_finalize");
     if (s != CDLM_OK) return s;
     return run_g1_cs(ctx, S, 1);
+}
+cdlm_status run_dict(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm) {
+    cdlm_status s = run_dict_seq(ctx, S, tm);
+    return s != CDLM_OK ? s : run_dict_filt(ctx, S);
 }
 
 // The constructor's device work (CDL_setup.jl:183-236): filters, |G1h|^2, codes, DX, and
 // the forced first update_best.
-cdlm_status run_reset(cdlm_ctx *ctx, const LaunchSet &S) {
+cdlm_status run_reset_filt(cdlm_ctx *ctx, const LaunchSet &S) {
     hipStream_t st = set_stream(ctx, S);
     hipLaunchKernelGGL(k_reset, dim3(S.blk_filt), dim3(kThreads), 0, st, S.tab, S.st_filt, S.np);
-    cdlm_status s = run_g1_cs(ctx, S, 0);
-    if (s != CDLM_OK) return s;
+    return run_g1_cs(ctx, S, 0);
+}
+cdlm_status run_init_seq(cdlm_ctx *ctx, const LaunchSet &S) {
+    hipStream_t st = set_stream(ctx, S);
 #define L_INIT(T) hipLaunchKernelGGL(k_init<T>, dim3(S.blk_seq), dim3(kThreads), S.lds_init, st, S.tab, S.st_seq, S.np)
     CDLM_TAPS_SWITCH(S.taps, L_INIT);
 #undef L_INIT
-    s = lcheck(ctx, "k_init");
+    return lcheck(ctx, "k_init");
+}
+cdlm_status run_reset(cdlm_ctx *ctx, const LaunchSet &S) {
+    cdlm_status s = run_reset_filt(ctx, S);
+    if (s == CDLM_OK) s = run_init_seq(ctx, S);
     if (s != CDLM_OK) return s;
     return run_best(ctx, S, nullptr, 1, 1, -1, nullptr);
 }
@@ -549,6 +595,14 @@ extern "C" size_t cdlm_footprint_bytes(int32_t L4, int32_t N, int32_t M, int32_t
     return plan_total(buf_plan(L4, N, M, gamma, keep_x0 != 0));
 }
 
+size_t cdlm_stream_min_bytes(int32_t L4, int32_t N, int32_t M, int32_t gamma) {
+    if (L4 <= 0 || N <= 0 || M <= 0 || gamma < 0) return 0;
+    const int nch = pick_nch(L4 + 2 * gamma, M);
+    int mx = 1;
+    for (int c = 0; c < nch; ++c) mx = std::max(mx, (int)((long)(c + 1) * N / nch) - (int)((long)c * N / nch));
+    return plan_total(buf_plan(L4, mx, M, gamma, false, N));
+}
+
 namespace {
 // One-hot data (4L, N) column-major from base codes (N, L) row-major: s[4l + a, n] = 1 iff
 // code(n, l) == a (data_2_dummy, to_dummy.jl:2-38); a code > 3 gives an all-zero group.
@@ -563,8 +617,27 @@ __global__ void k_bases_to_onehot(const uint8_t *__restrict__ codes, int L, long
 
 cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const float *s, const uint8_t *codes, int32_t L4,
                                 int32_t N, const float *D0, const float *X0, uint64_t seed, cdlm_problem **out,
-                                void *arena = nullptr, size_t arena_bytes = 0);
+                                void *arena = nullptr, size_t arena_bytes = 0, uint64_t stream_budget = 0);
 }  // namespace
+
+extern "C" cdlm_status cdlm_problem_create_streamed(cdlm_ctx *ctx, const cdlm_params *prm, const float *s, int32_t L4,
+                                                    int32_t N, const float *D0, const float *X0, uint64_t seed,
+                                                    uint64_t device_budget, cdlm_problem **out) {
+    if (!ctx) return CDLM_ERR_ARG;
+    if (!s) return cdlm_fail(ctx, CDLM_ERR_ARG, "null argument");
+    if (device_budget == 0) {
+        size_t f = 0, t = 0;
+        CDLM_HIP(ctx, hipMemGetInfo(&f, &t));
+        device_budget = (uint64_t)(0.8 * (double)f);
+    }
+    return problem_create_impl(ctx, prm, s, nullptr, L4, N, D0, X0, seed, out, nullptr, 0, device_budget);
+}
+
+extern "C" cdlm_status cdlm_problem_chunks(const cdlm_problem *p, int32_t *chunks) {
+    if (!p || !chunks) return CDLM_ERR_ARG;
+    *chunks = p->nchunk ? p->nchunk : 1;
+    return CDLM_OK;
+}
 
 cdlm_status cdlm_problem_create_in(cdlm_ctx *ctx, const cdlm_params *prm, const float *s, int32_t L4, int32_t N,
                                    const float *D0, uint64_t seed, void *arena, size_t arena_bytes, cdlm_problem **out) {
@@ -590,9 +663,123 @@ extern "C" cdlm_status cdlm_problem_create_from_bases(cdlm_ctx *ctx, const cdlm_
 }
 
 namespace {
+// ---------------------------------------------------------------------------- streamed problems
+// A problem larger than its device budget keeps every per-sequence array in pinned host memory;
+// each pass over the sequences moves chunk k into the device window (the state read), runs the
+// per-sequence kernels on it with chunk k's launch set, and moves the state written back.  The
+// per-filter state, the per-sequence objective partials (all N) and the partial sums of the
+// dictionary means (all chunks) stay on the device, so the per-filter kernels and the fixed-order
+// reductions see exactly what they see for a resident problem: a streamed training gives the
+// resident one's results bit for bit.  Chunks are unions of the dictionary-mean chunks.
+
+// Moves chunk k's state (and the data, and with x0 the initial codes) into the window.
+cdlm_status stream_in(cdlm_problem *p, int k, bool x0) {
+    hipStream_t st = p->ctx->stream;
+    const size_t a = (size_t)p->cn0[k], n = (size_t)(p->cn0[k + 1] - p->cn0[k]);
+    auto in = [&](const cdlm_problem::HostArr &h) {
+        return hipMemcpyAsync(h.dev, h.host + a * h.per_seq, n * h.per_seq, hipMemcpyHostToDevice, st);
+    };
+    hipError_t e = in(p->hs);
+    for (size_t i = 0; i < p->hstate.size() && e == hipSuccess; ++i) e = in(p->hstate[i]);
+    if (e == hipSuccess && x0 && p->hx0.host) e = in(p->hx0);
+    if (e != hipSuccess) return cdlm_fail(p->ctx, CDLM_ERR_HIP, "chunk upload: %s", hipGetErrorString(e));
+    return CDLM_OK;
+}
+
+// Moves chunk k's state back; cand >= 0: also the candidate X_best slice buffer cand.
+cdlm_status stream_out(cdlm_problem *p, int k, int cand) {
+    hipStream_t st = p->ctx->stream;
+    const size_t a = (size_t)p->cn0[k], n = (size_t)(p->cn0[k + 1] - p->cn0[k]);
+    auto out = [&](const cdlm_problem::HostArr &h) {
+        return hipMemcpyAsync(h.host + a * h.per_seq, h.dev, n * h.per_seq, hipMemcpyDeviceToHost, st);
+    };
+    hipError_t e = hipSuccess;
+    for (size_t i = 0; i < p->hstate.size() && e == hipSuccess; ++i) e = out(p->hstate[i]);
+    if (e == hipSuccess && cand >= 0) e = out(p->hxs[cand & 1]);
+    if (e != hipSuccess) return cdlm_fail(p->ctx, CDLM_ERR_HIP, "chunk download: %s", hipGetErrorString(e));
+    return CDLM_OK;
+}
+
+// One pass over the chunks: in, fn(chunk set), out.
+template <class F>
+cdlm_status stream_pass(cdlm_problem *p, bool x0, int cand, F fn) {
+    for (int k = 0; k < p->nchunk; ++k) {
+        cdlm_status st = stream_in(p, k, x0);
+        if (st == CDLM_OK) st = fn(p->csets[(size_t)k], k);
+        if (st == CDLM_OK) st = stream_out(p, k, cand);
+        if (st != CDLM_OK) return st;
+    }
+    return sync_ctx(p->ctx);
+}
+
+cdlm_status mark_materialised(cdlm_ctx *ctx, const LaunchSet &S) {
+    hipLaunchKernelGGL(k_mark_materialised, dim3(1), dim3(64), 0, set_stream(ctx, S), S.tab, S.np);
+    return lcheck(ctx, "k_mark_materialised");
+}
+
+// Index of the X_best slice buffer k_best writes its candidate to.
+cdlm_status cand_of(cdlm_problem *p, int *cand) {
+    CdlScal sc;
+    cdlm_status st = read_scal(p, &sc);
+    if (st == CDLM_OK) *cand = (sc.best_idx & 1) ^ 1;
+    return st;
+}
+
+// update_best / objective of a streamed problem (decide = 0: the objective only).
+cdlm_status streamed_best(cdlm_problem *p, int force, int decide, int slot, Timer *tm) {
+    int cand = 0;
+    cdlm_status st = cand_of(p, &cand);
+    if (st == CDLM_OK)
+        st = stream_pass(p, false, cand, [&](const LaunchSet &S, int) { return run_best_seq(p->ctx, S, tm, nullptr); });
+    if (st == CDLM_OK) st = run_best_global(p->ctx, p->set, force, decide, slot);
+    return st;
+}
+
+cdlm_status streamed_reset(cdlm_problem *p) {
+    cdlm_status st = run_reset_filt(p->ctx, p->set);
+    if (st == CDLM_OK)
+        st = stream_pass(p, true, -1, [&](const LaunchSet &S, int) { return run_init_seq(p->ctx, S); });
+    if (st == CDLM_OK) st = streamed_best(p, 1, 1, -1, nullptr);
+    if (st == CDLM_OK) st = sync_ctx(p->ctx);
+    return st;
+}
+
+cdlm_status streamed_dict(cdlm_problem *p, Timer *tm) {
+    cdlm_status st = stream_pass(p, false, -1, [&](const LaunchSet &S, int) { return run_dict_seq(p->ctx, S, tm); });
+    return st != CDLM_OK ? st : run_dict_filt(p->ctx, p->set);
+}
+
+// The schedule of cdl_optimize with one pass over the chunks per outer iteration: CSC
+// iterations, k_best (X materialised, the candidate slice) and the first dictionary iteration's
+// per-sequence part run on a chunk while it is in the window; the objective decision and the
+// per-filter part of the dictionary step follow over all chunks, as for a resident problem.
+cdlm_status streamed_schedule(cdlm_problem *p, Timer *tm, int iter_num, int csc_iter, int cdl_iter) {
+    for (int it = 0; it < iter_num; ++it) {
+        int cand = 0;
+        cdlm_status st = cand_of(p, &cand);
+        if (st == CDLM_OK)
+            st = stream_pass(p, false, cand, [&](const LaunchSet &S, int) {
+                cdlm_status r = run_csc(p->ctx, S, tm, csc_iter);
+                if (r == CDLM_OK) r = run_best_seq(p->ctx, S, tm, nullptr);
+                if (r == CDLM_OK && cdl_iter > 0) {
+                    r = mark_materialised(p->ctx, S);   // as k_reduce_partials records between the two
+                    if (r == CDLM_OK) r = run_dict_seq(p->ctx, S, tm);
+                }
+                return r;
+            });
+        if (st == CDLM_OK) st = run_best_global(p->ctx, p->set, 0, 1, it);
+        if (st == CDLM_OK && cdl_iter > 0) st = run_dict_filt(p->ctx, p->set);
+        for (int i = 1; i < cdl_iter && st == CDLM_OK; ++i) st = streamed_dict(p, tm);
+        if (st != CDLM_OK) return st;
+    }
+    return sync_ctx(p->ctx);
+}
+}  // namespace
+
+namespace {
 cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const float *s, const uint8_t *codes, int32_t L4,
                                 int32_t N, const float *D0, const float *X0, uint64_t seed, cdlm_problem **out,
-                                void *arena, size_t arena_bytes) {
+                                void *arena, size_t arena_bytes, uint64_t stream_budget) {
     if (!prm || !out || !prm->filter_len) return cdlm_fail(ctx, CDLM_ERR_ARG, "null argument");
     *out = nullptr;
     if (L4 <= 0 || L4 % 4 != 0) return cdlm_fail(ctx, CDLM_ERR_ARG, "first dimension of dat must be a positive multiple of 4 (got %d)", L4);
@@ -616,7 +803,8 @@ cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const flo
     CdlDev &d = p->dv;
     memset(&d, 0, sizeof(d));
     d.L4 = L4; d.N = N; d.M = M; d.Lg = Lg; d.gamma = gamma; d.L = p->L;
-    d.nch = pick_nch(Lg, M, N);
+    d.nch = pick_nch(Lg, M);
+    d.n_off = 0; d.Nt = N; d.red_c0 = 0; d.red_cn = d.nch;
     d.trace_cap = kTraceCap;
     d.rho = prm->rho; d.sigma = prm->sigma; d.lambda = prm->lambda;
     d.sh_csc = prm->sherman_csc; d.sh_cdl = prm->sherman_cdl;
@@ -633,7 +821,29 @@ cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const flo
     for (int m = 0; m < M; ++m)
         for (int l = p->fl[m]; l < Lg; ++l) D0m[(size_t)m * Lg + l] = 0.f;
 
-    const BufPlan b = buf_plan(L4, N, M, gamma, X0 != nullptr);
+    // streamed: the fewest chunks (a divisor of the mean chunks) whose window fits the budget;
+    // a problem that fits whole stays resident
+    int Nwin = N;
+    if (stream_budget && plan_total(buf_plan(L4, N, M, gamma, X0 != nullptr)) > stream_budget) {
+        const int nch = d.nch;
+        for (int k = 2; k <= nch && !p->nchunk; ++k) {
+            if (nch % k) continue;
+            int mx = 1;
+            for (int c = 0; c < k; ++c)
+                mx = std::max(mx, (int)((long)((c + 1) * (nch / k)) * N / nch) - (int)((long)(c * (nch / k)) * N / nch));
+            if (plan_total(buf_plan(L4, mx, M, gamma, X0 != nullptr, N)) <= stream_budget) {
+                p->nchunk = k;
+                Nwin = mx;
+            }
+        }
+        if (!p->nchunk)
+            return cdlm_fail(ctx, CDLM_ERR_OOM, "a device window of %d chunks still exceeds the budget of %llu bytes", nch,
+                             (unsigned long long)stream_budget);
+        p->win = Nwin;
+        for (int c = 0; c <= p->nchunk; ++c) p->cn0.push_back((int)((long)(c * (nch / p->nchunk)) * N / nch));
+        d.N = Nwin;
+    }
+    const BufPlan b = buf_plan(L4, Nwin, M, gamma, X0 != nullptr, N);
     cdlm_status st = CDLM_OK;
     const size_t total = plan_total(b);
     if (arena) {
@@ -683,7 +893,42 @@ cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const flo
         tw[t] = make_float2((float)cos(a), (float)sin(a));
     }
     hipError_t e = hipSuccess;
-    if (s) {
+    if (p->nchunk) {
+        // the per-sequence arrays of all N sequences in pinned host memory (data and initial
+        // codes in, state in and out, X_best slices out)
+        const size_t Hs = (size_t)half_bins(Lg), nslot = (size_t)Lg / 4;
+        auto host = [&](cdlm_problem::HostArr &h, void *dev, size_t per_seq) -> bool {
+            h.dev = static_cast<char *>(dev);
+            h.per_seq = per_seq;
+            void *q = nullptr;
+            if (hipHostMalloc(&q, per_seq * (size_t)N ? per_seq * (size_t)N : 4) != hipSuccess) {
+                (void)hipGetLastError();
+                return false;
+            }
+            p->pinned.push_back(q);
+            h.host = static_cast<char *>(q);
+            return true;
+        };
+        const size_t LMb = (size_t)Lg * M * sizeof(float), Lb = (size_t)Lg * sizeof(float);
+        struct { void *dev; size_t per; } arrs[] = {
+            {d.Xt, LMb}, {d.y2, LMb}, {d.y3, 3 * nslot * M * sizeof(float)}, {d.g1h, Hs * M * sizeof(float2)},
+            {d.y3s, (size_t)M * (Lg / 8 + 1) * sizeof(float2)}, {d.kap, Hs * sizeof(float2)}, {d.h, Lb}, {d.y1, Lb},
+            {d.Y1, Lb}, {d.DX, Lb}, {d.G2, Lb}, {d.g2, Lb}, {d.Td, Lb}};
+        bool ok = host(p->hs, s_dev, (size_t)L4 * sizeof(float)) && (!X0 || host(p->hx0, X0_dev, LMb)) &&
+                  host(p->hxs[0], d.xs[0], (size_t)p->L * M * sizeof(float)) &&
+                  host(p->hxs[1], d.xs[1], (size_t)p->L * M * sizeof(float));
+        for (auto &a : arrs) {
+            if (!ok) break;
+            p->hstate.emplace_back();
+            ok = host(p->hstate.back(), a.dev, a.per);
+        }
+        if (!ok) {
+            free_problem(p.get());
+            return cdlm_fail(ctx, CDLM_ERR_OOM, "pinned host memory for the state of %d sequences", N);
+        }
+        std::memcpy(p->hs.host, s, (size_t)L4 * N * sizeof(float));
+        if (X0) std::memcpy(p->hx0.host, X0, LMb * (size_t)N);
+    } else if (s) {
         e = hipMemcpyAsync(s_dev, s, b.s, hipMemcpyHostToDevice, hs);
     } else {
         // base codes: N*L bytes over the bus, the one-hot expansion on the device
@@ -700,7 +945,7 @@ cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const flo
         if (e == hipSuccess) e = hipStreamSynchronize(hs);
         if (cd) (void)hipFree(cd);
     }
-    if (e == hipSuccess && X0) e = hipMemcpyAsync(X0_dev, X0, b.Ex0, hipMemcpyHostToDevice, hs);
+    if (e == hipSuccess && X0 && !p->nchunk) e = hipMemcpyAsync(X0_dev, X0, b.Ex0, hipMemcpyHostToDevice, hs);
     if (e == hipSuccess) e = hipMemcpyAsync(D0_dev, D0m.data(), b.LM, hipMemcpyHostToDevice, hs);
     if (e == hipSuccess) e = hipMemcpyAsync(fl_dev, p->fl.data(), b.M, hipMemcpyHostToDevice, hs);
     if (e == hipSuccess) e = hipMemcpyAsync(tw_dev, tw.data(), Lg * sizeof(float2), hipMemcpyHostToDevice, hs);
@@ -711,7 +956,20 @@ cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const flo
     }
     cdlm_problem *pp = p.get();
     st = set_build(ctx, &pp, 1, &p->set);
-    if (st == CDLM_OK) st = run_reset(ctx, p->set);
+    if (st == CDLM_OK && p->nchunk) {
+        // chunk k's launch set: its sequences, their first index and its dictionary-mean chunks
+        p->csets.resize((size_t)p->nchunk);
+        const int per = d.nch / p->nchunk;
+        for (int k = 0; k < p->nchunk && st == CDLM_OK; ++k) {
+            d.N = p->cn0[k + 1] - p->cn0[k];
+            d.n_off = p->cn0[k];
+            d.red_c0 = k * per;
+            d.red_cn = per;
+            st = set_build(ctx, &pp, 1, &p->csets[(size_t)k]);
+        }
+        d.N = Nwin; d.n_off = 0; d.red_c0 = 0; d.red_cn = d.nch;
+    }
+    if (st == CDLM_OK) st = p->nchunk ? streamed_reset(p.get()) : run_reset(ctx, p->set);
     if (st == CDLM_OK) st = sync_ctx(ctx);
     if (st != CDLM_OK) {
         free_problem(p.get());
@@ -733,6 +991,7 @@ extern "C" cdlm_status cdlm_problem_destroy(cdlm_problem *p) {
 
 extern "C" cdlm_status cdlm_problem_reset(cdlm_problem *p) {
     if (!p) return CDLM_ERR_ARG;
+    if (p->nchunk) return streamed_reset(p);
     cdlm_status st = run_reset(p->ctx, p->set);
     if (st != CDLM_OK) return st;
     return sync_ctx(p->ctx);
@@ -754,6 +1013,8 @@ extern "C" size_t cdlm_problem_bytes(const cdlm_problem *p) { return p ? p->tota
 extern "C" cdlm_status cdlm_code_update(cdlm_problem *p, int32_t num_iter) {
     if (!p) return CDLM_ERR_ARG;
     if (num_iter < 0) return cdlm_fail(p->ctx, CDLM_ERR_ARG, "num_iter < 0");
+    if (p->nchunk)
+        return stream_pass(p, false, -1, [&](const LaunchSet &S, int) { return run_csc(p->ctx, S, &p->timer, num_iter); });
     cdlm_status st = run_csc(p->ctx, p->set, &p->timer, num_iter);
     if (st != CDLM_OK) return st;
     return sync_ctx(p->ctx);
@@ -763,7 +1024,7 @@ extern "C" cdlm_status cdlm_dict_update(cdlm_problem *p, int32_t num_iter) {
     if (!p) return CDLM_ERR_ARG;
     if (num_iter < 0) return cdlm_fail(p->ctx, CDLM_ERR_ARG, "num_iter < 0");
     for (int i = 0; i < num_iter; ++i) {
-        cdlm_status st = run_dict(p->ctx, p->set, &p->timer);
+        cdlm_status st = p->nchunk ? streamed_dict(p, &p->timer) : run_dict(p->ctx, p->set, &p->timer);
         if (st != CDLM_OK) return st;
     }
     return sync_ctx(p->ctx);
@@ -771,7 +1032,7 @@ extern "C" cdlm_status cdlm_dict_update(cdlm_problem *p, int32_t num_iter) {
 
 extern "C" cdlm_status cdlm_update_best(cdlm_problem *p, double *obj, int32_t *improved) {
     if (!p) return CDLM_ERR_ARG;
-    cdlm_status st = run_best(p->ctx, p->set, &p->timer, 0, 1, -1, nullptr);
+    cdlm_status st = p->nchunk ? streamed_best(p, 0, 1, -1, &p->timer) : run_best(p->ctx, p->set, &p->timer, 0, 1, -1, nullptr);
     if (st != CDLM_OK) return st;
     CdlScal sc;
     if ((st = read_scal(p, &sc)) != CDLM_OK) return st;
@@ -783,7 +1044,7 @@ extern "C" cdlm_status cdlm_update_best(cdlm_problem *p, double *obj, int32_t *i
 extern "C" cdlm_status cdlm_objective(cdlm_problem *p, double *obj) {
     if (!p || !obj) return CDLM_ERR_ARG;
     // the candidate X_best slice buffer is scratch here; no decision is taken
-    cdlm_status st = run_best(p->ctx, p->set, nullptr, 0, 0, -1, nullptr);
+    cdlm_status st = p->nchunk ? streamed_best(p, 0, 0, -1, nullptr) : run_best(p->ctx, p->set, nullptr, 0, 0, -1, nullptr);
     if (st != CDLM_OK) return st;
     CdlScal sc;
     if ((st = read_scal(p, &sc)) != CDLM_OK) return st;
@@ -796,7 +1057,8 @@ extern "C" cdlm_status cdlm_optimize(cdlm_problem *p, int32_t iter_num, int32_t 
     if (!p) return CDLM_ERR_ARG;
     if (iter_num < 0 || csc_iter < 0 || cdl_iter < 0) return cdlm_fail(p->ctx, CDLM_ERR_ARG, "negative iteration count");
     if (iter_num > kTraceCap) return cdlm_fail(p->ctx, CDLM_ERR_ARG, "iter_num > %d", kTraceCap);
-    cdlm_status st = run_schedule(p->ctx, p->set, &p->timer, iter_num, csc_iter, cdl_iter);
+    cdlm_status st = p->nchunk ? streamed_schedule(p, &p->timer, iter_num, csc_iter, cdl_iter)
+                               : run_schedule(p->ctx, p->set, &p->timer, iter_num, csc_iter, cdl_iter);
     if (st != CDLM_OK) return st;
     if (obj_trace && iter_num > 0)
         CDLM_HIP(p->ctx, hipMemcpyAsync(obj_trace, p->dv.trace, iter_num * sizeof(double), hipMemcpyDeviceToHost, p->ctx->stream));
@@ -822,8 +1084,99 @@ extern "C" cdlm_status cdlm_obj_best(cdlm_problem *p, float *obj_best) {
     return CDLM_OK;
 }
 
+namespace {
+// cdlm_get of a streamed problem: per-sequence arrays from the host state, d, g1, g3 and X through
+// the device window chunk by chunk, the per-filter arrays from the device.
+cdlm_status streamed_get(cdlm_problem *p, int32_t field, void *dst, size_t bytes) {
+    const size_t N = (size_t)p->N, E = (size_t)p->Lg * p->M * N, LN = (size_t)p->Lg * N;
+    hipStream_t st = p->ctx->stream;
+    auto host_copy = [&](int idx, size_t cnt) -> cdlm_status {
+        if (bytes != cnt * sizeof(float))
+            return cdlm_fail(p->ctx, CDLM_ERR_ARG, "field %d needs %zu bytes, got %zu", field, cnt * sizeof(float), bytes);
+        std::memcpy(dst, p->hstate[(size_t)idx].host, bytes);
+        return CDLM_OK;
+    };
+    switch (field) {
+    case CDLM_F_y2: return host_copy(1, E);
+    case CDLM_F_y1: return host_copy(7, LN);
+    case CDLM_F_Y1: return host_copy(8, LN);
+    case CDLM_F_DX: return host_copy(9, LN);
+    case CDLM_F_G2: return host_copy(10, LN);
+    case CDLM_F_g2: return host_copy(11, LN);
+    case CDLM_F_TD: return host_copy(12, LN);
+    case CDLM_F_y3: {
+        if (bytes != E * sizeof(float)) return cdlm_fail(p->ctx, CDLM_ERR_ARG, "field y3 needs %zu bytes", E * sizeof(float));
+        const size_t nslot = (size_t)p->Lg / 4;
+        const float *packed = reinterpret_cast<const float *>(p->hstate[2].host);
+        float *o = static_cast<float *>(dst);
+        for (size_t c = 0; c < (size_t)p->M * N; ++c)
+            for (size_t sl = 0; sl < nslot; ++sl) {
+                const float *q = &packed[(c * nslot + sl) * 3];
+                float *w = o + c * p->Lg + 4 * sl;
+                w[0] = 0.f; w[1] = q[0]; w[2] = q[1]; w[3] = q[2];
+            }
+        return CDLM_OK;
+    }
+    case CDLM_F_XBEST_SLICE: {
+        const size_t cnt = (size_t)p->L * p->M * N;
+        if (bytes != cnt * sizeof(float)) return cdlm_fail(p->ctx, CDLM_ERR_ARG, "field %d needs %zu bytes", field, cnt * 4);
+        CdlScal sc;
+        cdlm_status s = read_scal(p, &sc);
+        if (s != CDLM_OK) return s;
+        std::memcpy(dst, p->hxs[sc.best_idx & 1].host, bytes);
+        return CDLM_OK;
+    }
+    case CDLM_F_X:
+    case CDLM_F_D:
+    case CDLM_F_g1:
+    case CDLM_F_g3: {
+        if (bytes != E * sizeof(float)) return cdlm_fail(p->ctx, CDLM_ERR_ARG, "field %d needs %zu bytes", field, E * 4);
+        const size_t per = (size_t)p->Lg * p->M;
+        float *tmp = nullptr;
+        if (hipMalloc(&tmp, per * (size_t)p->win * sizeof(float)) != hipSuccess) {
+            (void)hipGetLastError();
+            return cdlm_fail(p->ctx, CDLM_ERR_OOM, "no memory for field %d", field);
+        }
+        char *o = static_cast<char *>(dst);
+        cdlm_status s = CDLM_OK;
+        const int what = field == CDLM_F_D ? 0 : field == CDLM_F_g1 ? 1 : 2;
+        for (int k = 0; k < p->nchunk && s == CDLM_OK; ++k) {
+            const size_t a = (size_t)p->cn0[k], n = (size_t)(p->cn0[k + 1] - p->cn0[k]);
+            s = stream_in(p, k, false);
+            if (s != CDLM_OK) break;
+            if (field == CDLM_F_X) {
+                // the candidate slice buffer is scratch here; X is written back materialised
+                s = run_best_seq(p->ctx, p->csets[(size_t)k], nullptr, tmp);
+                if (s == CDLM_OK) s = stream_out(p, k, -1);
+            } else if (n > 0) {
+                hipLaunchKernelGGL(k_dict_to_spatial, dim3((unsigned)(n * p->M)), dim3(kThreads), lds_bytes_half(p->Lg), st,
+                                   p->csets[(size_t)k].tab, what, tmp);
+                s = lcheck(p->ctx, "k_dict_to_spatial");
+            }
+            if (s == CDLM_OK && n > 0 &&
+                hipMemcpyAsync(o + a * per * sizeof(float), tmp, n * per * sizeof(float), hipMemcpyDeviceToHost, st) !=
+                    hipSuccess)
+                s = cdlm_fail(p->ctx, CDLM_ERR_HIP, "copy of field %d failed", field);
+        }
+        if (s == CDLM_OK && field == CDLM_F_X) s = run_best_global(p->ctx, p->set, 0, 0, -1);
+        if (s == CDLM_OK) s = sync_ctx(p->ctx);
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(tmp);
+        return s;
+    }
+    default:
+        break;
+    }
+    return CDLM_ERR_STATE;   // per-filter fields: read from the device as for a resident problem
+}
+}  // namespace
+
 extern "C" cdlm_status cdlm_get(cdlm_problem *p, int32_t field, void *dst, size_t bytes) {
     if (!p || !dst) return CDLM_ERR_ARG;
+    if (p->nchunk) {
+        cdlm_status s = streamed_get(p, field, dst, bytes);
+        if (s != CDLM_ERR_STATE) return s;
+    }
     const size_t E = (size_t)p->Lg * p->M * p->N, LN = (size_t)p->Lg * p->N, LM = (size_t)p->Lg * p->M;
     const CdlDev &d = p->dv;
     const void *src = nullptr;
@@ -905,6 +1258,8 @@ extern "C" cdlm_status cdlm_get(cdlm_problem *p, int32_t field, void *dst, size_
 }
 
 cdlm_status cdlm_problem_xbest_device(cdlm_problem *p, const float **slice, int *L, int *M, int *N) {
+    if (p->nchunk)
+        return cdlm_fail(p->ctx, CDLM_ERR_UNSUPPORTED, "a streamed problem keeps its X_best slices in host memory");
     CdlScal sc;
     cdlm_status st = read_scal(p, &sc);
     if (st != CDLM_OK) return st;
@@ -1099,6 +1454,8 @@ extern "C" cdlm_status cdlm_batch_create(cdlm_ctx *ctx, cdlm_problem *const *pro
     for (int i = 0; i < count; ++i) {
         if (!problems[i]) return cdlm_fail(ctx, CDLM_ERR_ARG, "null problem %d", i);
         if (problems[i]->ctx != ctx) return cdlm_fail(ctx, CDLM_ERR_ARG, "problem %d belongs to another context", i);
+        if (problems[i]->nchunk)
+            return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "problem %d is streamed through a device window", i);
         for (int j = 0; j < i; ++j)
             if (problems[j] == problems[i]) return cdlm_fail(ctx, CDLM_ERR_ARG, "problem %d appears twice", i);
     }
@@ -1348,25 +1705,35 @@ extern "C" cdlm_status cdlm_autotune(cdlm_ctx *ctx, int32_t L4, int32_t M, int32
     Tuning best;
     float best_t = 0.f;
     bool have = false;
-    for (int o = 0; o < kFftOrders && st == CDLM_OK; ++o) {
-        for (int ci = 0; ci < kCsc && st == CDLM_OK; ++ci) {
-            const int nsl = cand_nsl[ci];
-            best_csc[o][ci] = -1.f;
-            if (nsl < need || pick_csc_waves(Lg, gamma, nsl, cand_waves[ci]) != cand_waves[ci]) continue;
-            Tuning tu;
-            tu.order = o; tu.nsl = nsl; tu.waves = cand_waves[ci];
-            if ((st = apply(tu)) == CDLM_OK) st = timed(false, 20, &best_csc[o][ci]);
-        }
-        for (int o4 = 0; o4 < kFftOrders && st == CDLM_OK; ++o4)
-            for (int bi = 0; bi < kDict && st == CDLM_OK; ++bi) {
-                const int B = cand_B[bi];
-                best_dict[o][o4][bi] = -1.f;
-                if (lds_bytes_dict(Lg, gamma, B) > kLdsLimit - kThreads * sizeof(double)) continue;
-                Tuning tu;
-                tu.order = o; tu.order4 = o4; tu.B = B;
-                if ((st = apply(tu)) == CDLM_OK) st = timed(true, 2, &best_dict[o][o4][bi]);
-            }
+    // every candidate timed in each of kRounds interleaved rounds, its fastest round kept (the
+    // candidates differ by a few percent, about the spread of a single timing)
+    constexpr int kRounds = 3;
+    for (int o = 0; o < kFftOrders; ++o) {
+        for (int ci = 0; ci < kCsc; ++ci) best_csc[o][ci] = -1.f;
+        for (int o4 = 0; o4 < kFftOrders; ++o4)
+            for (int bi = 0; bi < kDict; ++bi) best_dict[o][o4][bi] = -1.f;
     }
+    auto keep_min = [](float &slot, float t) { if (slot < 0.f || t < slot) slot = t; };
+    for (int r = 0; r < kRounds && st == CDLM_OK; ++r)
+        for (int o = 0; o < kFftOrders && st == CDLM_OK; ++o) {
+            for (int ci = 0; ci < kCsc && st == CDLM_OK; ++ci) {
+                const int nsl = cand_nsl[ci];
+                if (nsl < need || pick_csc_waves(Lg, gamma, nsl, cand_waves[ci]) != cand_waves[ci]) continue;
+                Tuning tu;
+                tu.order = o; tu.nsl = nsl; tu.waves = cand_waves[ci];
+                float t = 0.f;
+                if ((st = apply(tu)) == CDLM_OK && (st = timed(false, 20, &t)) == CDLM_OK) keep_min(best_csc[o][ci], t);
+            }
+            for (int o4 = 0; o4 < kFftOrders && st == CDLM_OK; ++o4)
+                for (int bi = 0; bi < kDict && st == CDLM_OK; ++bi) {
+                    const int B = cand_B[bi];
+                    if (lds_bytes_dict(Lg, gamma, B) > kLdsLimit - kThreads * sizeof(double)) continue;
+                    Tuning tu;
+                    tu.order = o; tu.order4 = o4; tu.B = B;
+                    float t = 0.f;
+                    if ((st = apply(tu)) == CDLM_OK && (st = timed(true, 2, &t)) == CDLM_OK) keep_min(best_dict[o][o4][bi], t);
+                }
+        }
     for (int o = 0; o < kFftOrders && st == CDLM_OK; ++o)
         for (int ci = 0; ci < kCsc; ++ci) {
             if (best_csc[o][ci] < 0) continue;

@@ -72,6 +72,11 @@ struct CdlScal {
 // Descriptor of one problem (all device pointers).
 struct CdlDev {
     int L4, N, M, Lg, gamma, L, nch, trace_cap;
+    // N is the number of sequences the per-sequence kernels see; a problem streamed through a
+    // device window runs them chunk by chunk: the chunk's first sequence is n_off of Nt in all,
+    // and its dictionary-mean chunks are red_c0 .. red_c0 + red_cn - 1 of nch (resident
+    // problems: n_off = 0, Nt = N, all nch chunks)
+    int n_off, Nt, red_c0, red_cn;
     float rho, sigma, lambda, sh_csc, sh_cdl;
     const float *s;
     const int *fl;
@@ -326,7 +331,8 @@ __global__ __launch_bounds__(kThreads) void k_init(const CdlDev *__restrict__ ta
         float g[TAPS];
         load_taps<TAPS>(G1, mm, Lg, g);
         for (int s = lane; s < nslot; s += 64) {
-            const float x = P.X0 ? gp(P.X0)[col + 4 * s] : gauss_of(P.seed, col + 4 * (size_t)s);
+            const size_t colg = ((size_t)(P.n_off + n) * M + mm) * Lg;   // the element's index in the whole problem
+            const float x = P.X0 ? gp(P.X0)[col + 4 * s] : gauss_of(P.seed, colg + 4 * (size_t)s);
             const float4 xp = make_float4(x, 0.f, 0.f, 0.f);
             st4(P.Xt + col, s, xp);
             st4(P.y2 + col, s, zero4);
@@ -675,9 +681,16 @@ __global__ __launch_bounds__(kThreads) void k_best(const CdlDev *__restrict__ ta
             if (tid < st) red[tid] += red[tid + st];
             __syncthreads();
         }
-        if (tid == 0) gp(P.partials)[(size_t)n * kPartStride + f] = red[0];
+        if (tid == 0) gp(P.partials)[(size_t)(P.n_off + n) * kPartStride + f] = red[0];
         __syncthreads();
     }
+}
+
+// X is held materialised (what k_reduce_partials records after k_best), for a chunk of a
+// streamed problem whose k_best ran but whose partials are not all in yet; one thread per problem.
+__global__ void k_mark_materialised(const CdlDev *__restrict__ tab, int np) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < np) tab[p].sc->xmat = 1;
 }
 
 // Fixed-order sum of the per-sequence partials; one block per problem.
@@ -687,7 +700,7 @@ __global__ __launch_bounds__(kThreads) void k_reduce_partials(const CdlDev *__re
     const int tid = threadIdx.x;
     for (int f = 0; f < 5; ++f) {
         double a = 0.0;
-        for (int n = tid; n < P.N; n += kThreads) a += gp(P.partials)[(size_t)n * kPartStride + f];
+        for (int n = tid; n < P.Nt; n += kThreads) a += gp(P.partials)[(size_t)n * kPartStride + f];
         red[tid] = a;
         __syncthreads();
         for (int st = kThreads / 2; st > 0; st >>= 1) {
@@ -971,7 +984,10 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
 // Partial sums over a chunk of sequences, per (bin, m), of A = g1h_{t-1} and of
 // c = conj(Y3h_t) kappa_t (the same float product the next step forms), in double, real and
 // imaginary parts: the means over n of d, d + g1 and d + g3 (train.jl:76,83,95) follow from
-// them (k_dict_finalize).  Blocks of a problem: (tile, chunk).
+// them (k_dict_finalize).  Blocks of a problem: (tile, chunk), mapped so that all tiles of a
+// chunk share the block index modulo 8, i.e. one XCD: every tile of a chunk reads the same
+// Y3 half spectra and kappa of its sequences, which then come from HBM once into that XCD's L2
+// (the chunk count is a multiple of 8).
 __global__ __launch_bounds__(kThreads) void k_dict_reduce(const CdlDev *__restrict__ tab, const int *__restrict__ starts,
                                                           int np) {
     int b;
@@ -979,7 +995,16 @@ __global__ __launch_bounds__(kThreads) void k_dict_reduce(const CdlDev *__restri
     const int Lg = P.Lg, Hs = half_bins(Lg), H = Lg / 2 + 1, M = P.M, HM = Hs * M;
     const int Pq = Lg >> 2, Ph = Pq / 2 + 1;
     const int tiles = (HM + kThreads - 1) / kThreads;
-    const int c = b / tiles, tile = b - c * tiles;
+    int c, tile;
+    if ((P.red_cn & 7) == 0) {
+        const int g = b & 7, j = b >> 3;                  // XCD group, index within it
+        c = g + 8 * (j / tiles);
+        tile = j % tiles;
+    } else {
+        c = b / tiles;
+        tile = b - c * tiles;
+    }
+    c += P.red_c0;
     const int i = tile * kThreads + threadIdx.x;
     if (i >= HM) return;
     const int m = i / Hs, k = i - m * Hs;
@@ -987,8 +1012,9 @@ __global__ __launch_bounds__(kThreads) void k_dict_reduce(const CdlDev *__restri
     while (kq >= Pq) kq -= Pq;
     const bool cj = kq >= Ph;
     const int yi = cj ? Pq - kq : kq;
-    const int N = P.N, nch = P.nch;
-    const int n0 = (int)((long)c * N / nch), n1 = (int)((long)(c + 1) * N / nch);
+    const int Nt = P.Nt, nch = P.nch;
+    // the chunk's sequences, as indices into the sequences this launch sees
+    const int n0 = (int)((long)c * Nt / nch) - P.n_off, n1 = (int)((long)(c + 1) * Nt / nch) - P.n_off;
     double ax = 0.0, ay = 0.0, cx = 0.0, cy = 0.0;
     if (k < H) {
         for (int n = n0; n < n1; ++n) {
@@ -1028,7 +1054,7 @@ __global__ __launch_bounds__(kThreads) void k_dict_finalize(const CdlDev *__rest
     const int tid = threadIdx.x, Lg = P.Lg, H = Lg / 2 + 1, Hs = half_bins(Lg);
     const int HM = Hs * P.M;
     const int flm = gp(P.fl)[m];
-    const double invN = 1.0 / (double)P.N;
+    const double invN = 1.0 / (double)P.Nt;
     const float scale = 1.0f / (float)Lg;
     const int nxt = (P.sc->g1cur & 1) ^ 1;
     float *G1new = P.G1b[nxt];

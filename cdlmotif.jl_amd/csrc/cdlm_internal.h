@@ -34,6 +34,10 @@ cdlm_status cdlm_fail(cdlm_ctx *ctx, cdlm_status st, const char *fmt, ...);
 cdlm_status cdlm_problem_create_in(cdlm_ctx *ctx, const cdlm_params *prm, const float *s, int32_t L4, int32_t N,
                                    const float *D0, uint64_t seed, void *arena, size_t arena_bytes, cdlm_problem **out);
 
+// Device bytes of the smallest window a streamed problem of this shape can use (its sequences
+// in as many chunks as the dictionary means have); 0 for a bad shape.
+size_t cdlm_stream_min_bytes(int32_t L4, int32_t N, int32_t M, int32_t gamma);
+
 // Device pointer of a problem's current X_best slice (L, M, N) and its dimensions; reads the
 // slice index from the device (synchronises the context stream).
 cdlm_status cdlm_problem_xbest_device(cdlm_problem *p, const float **slice, int *L, int *M, int *N);

@@ -4,7 +4,10 @@
 // try_to_find_motif -> cdl_optimize, src/WRAPPER/wrap.jl:4-12,140-149).  Here a server
 // thread keeps many jobs resident on one device and advances all of them together:
 //   * submit copies a job's data to host memory and queues it (arrival time recorded);
-//     a job whose device footprint exceeds the whole budget is refused at once;
+//     a job whose device footprint exceeds the whole budget is streamed: admitted when nothing
+//     else is resident, it runs alone as a problem whose state lives in pinned host memory and
+//     whose sequences move through a device window of the budget in chunks (its ticks are
+//     cdlm_optimize of one outer iteration), with the results it gets resident;
 //   * between ticks, queued jobs are admitted in arrival order while their footprint
 //     (cdlm_footprint_bytes, the exact bytes a problem holds) fits the free budget and the
 //     resident jobs hold fewer sequences than fill_seqs (the sequences that occupy every
@@ -62,6 +65,7 @@ struct Job {
     double mult = 1.0;
     size_t footprint = 0;
     double work = 0.0;     // iter_num * N * Lg * M
+    bool streamed = false; // footprint beyond the budget: streamed through a device window, alone
     // state
     int32_t state = CDLM_JOB_QUEUED;
     int bypassed = 0;      // later arrivals admitted before this job
@@ -146,8 +150,26 @@ std::vector<Job *> pick_admissions(cdlm_server *srv, std::vector<void *> *to_fre
         return a->work < b->work;
     });
     bool blocked = false;   // a starving job does not fit: nothing passes it
+    for (Job *j : srv->active)
+        if (j->streamed) return out;   // a streamed job has the device to itself
     for (Job *j : cand) {
         if (slots <= 0 || (srv->fill > 0 && seqs >= srv->fill) || blocked) break;
+        if (j->streamed) {
+            // alone: admitted when nothing is resident, with the whole budget (pooled arenas freed)
+            if (out.empty() && srv->active.empty()) {
+                for (auto &a : srv->pool) to_free->push_back(a.first);
+                srv->pool.clear();
+                srv->pool_bytes = 0;
+                j->arena = nullptr;
+                j->arena_bytes = srv->budget;
+                srv->in_use += j->arena_bytes;
+                j->state = CDLM_JOB_RUNNING;
+                out.push_back(j);
+                break;
+            }
+            if (starving(j)) blocked = true;
+            continue;
+        }
         int best = -1;
         for (size_t i = 0; i < srv->pool.size(); ++i)
             if (srv->pool[i].second >= j->footprint && (best < 0 || srv->pool[i].second < srv->pool[(size_t)best].second))
@@ -226,7 +248,7 @@ void finish_job(cdlm_server *srv, Job *j) {
     const std::string err = st == CDLM_OK ? std::string() : ctx_error(srv);
     cdlm_problem_destroy(j->p);   // the arena is the server's: it goes back to the pool
     j->p = nullptr;
-    return_arena(srv, j, st == CDLM_OK);
+    return_arena(srv, j, st == CDLM_OK && !j->streamed);
     std::lock_guard<std::mutex> lk(srv->mu);
     j->t_done = Clock::now();
     if (st == CDLM_OK) {
@@ -259,6 +281,12 @@ bool tick(cdlm_server *srv) {
         act = srv->active;
     }
     if (act.empty()) return true;
+    std::vector<double> tr(act.size());
+    if (act.size() == 1 && act[0]->streamed) {
+        // one outer iteration, chunk by chunk through the device window
+        drop_batch(srv);
+        if (cdlm_optimize(act[0]->p, 1, srv->opts.csc_iter, srv->opts.cdl_iter, tr.data()) != CDLM_OK) return false;
+    } else {
     if (srv->membership_changed || !srv->batch) {
         if (srv->batch) cdlm_batch_destroy(srv->batch);
         srv->batch = nullptr;
@@ -268,8 +296,8 @@ bool tick(cdlm_server *srv) {
         cdlm_batch_set_graph(srv->batch, srv->opts.use_graph ? 1 : 0);
         srv->membership_changed = false;
     }
-    std::vector<double> tr(act.size());
     if (cdlm_batch_optimize(srv->batch, 1, srv->opts.csc_iter, srv->opts.cdl_iter, tr.data()) != CDLM_OK) return false;
+    }
     std::vector<Job *> done;
     for (size_t i = 0; i < act.size(); ++i) {
         act[i]->trace.push_back(tr[i]);
@@ -303,6 +331,31 @@ void server_loop(cdlm_server *srv) {
         }
         for (void *a : to_free) (void)hipFree(a);
         for (Job *j : adm) {
+            if (j->streamed) {
+                cdlm_problem *p = nullptr;
+                cdlm_status st = cdlm_problem_create_streamed(srv->ctx, &j->prm, j->s.data(), j->L4, j->N,
+                                                              j->D0.empty() ? nullptr : j->D0.data(), nullptr, j->seed,
+                                                              j->arena_bytes, &p);
+                std::lock_guard<std::mutex> lk(srv->mu);
+                if (st != CDLM_OK) {
+                    srv->in_use -= j->arena_bytes;
+                    j->arena_bytes = 0;
+                    j->state = CDLM_JOB_FAILED;
+                    j->error = ctx_error(srv);
+                    j->t_done = Clock::now();
+                    srv->cv_done.notify_all();
+                    continue;
+                }
+                int32_t Lg = 0, gamma = 0;
+                cdlm_problem_dims(p, &Lg, &gamma, nullptr, nullptr, nullptr);
+                j->p = p;
+                j->Lg = Lg;
+                j->gamma = gamma;
+                j->t_admit = Clock::now();
+                srv->active.push_back(j);
+                srv->membership_changed = true;
+                continue;
+            }
             if (!j->arena) {
                 if (hipMalloc(&j->arena, j->arena_bytes) != hipSuccess) {
                     (void)hipGetLastError();
@@ -481,10 +534,13 @@ extern "C" cdlm_status cdlm_server_submit(cdlm_server *srv, const cdlm_job_spec 
         return reject(CDLM_ERR_UNSUPPORTED, "job shape outside the kernels' range (filter length <= 32, Lg <= 2048)");
     j->footprint = cdlm_footprint_bytes(spec->L4, spec->N, prm.M, gamma, 0);
     j->work = (double)spec->iter_num * spec->N * (double)(spec->L4 + 2 * gamma) * prm.M;
-    if (j->footprint > srv->budget) {
+    // a job beyond the budget is streamed through a device window of the budget (alone on the
+    // device); its state then lives in pinned host memory
+    j->streamed = j->footprint > srv->budget;
+    if (j->streamed && cdlm_stream_min_bytes(spec->L4, spec->N, prm.M, gamma) > srv->budget) {
         char buf[256];
-        snprintf(buf, sizeof(buf), "job needs %zu device bytes, more than the server budget of %zu", j->footprint,
-                 srv->budget);
+        snprintf(buf, sizeof(buf), "job needs %zu device bytes and at least %zu streamed, more than the server budget of %zu",
+                 j->footprint, cdlm_stream_min_bytes(spec->L4, spec->N, prm.M, gamma), srv->budget);
         return reject(CDLM_ERR_OOM, buf);
     }
     j->prm = prm;

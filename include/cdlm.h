@@ -78,6 +78,22 @@ cdlm_status cdlm_problem_create(cdlm_ctx *ctx, const cdlm_params *params, const 
  * instead of 16*N*L. */
 cdlm_status cdlm_problem_create_from_bases(cdlm_ctx *ctx, const cdlm_params *params, const uint8_t *bases, int32_t L,
                                            int32_t N, const float *D0, const float *X0, uint64_t seed, cdlm_problem **out);
+/* As cdlm_problem_create, for problems that may not fit the device: when the problem's footprint
+ * (cdlm_footprint_bytes) exceeds device_budget bytes (0: 80% of the free device memory), the
+ * state of all N sequences lives in pinned host memory and the sequences move through a device
+ * window in chunks (the fewest, a divisor of the dictionary-mean chunk count, whose window fits
+ * the budget): every pass over the sequences (code_update, update_best, dict_update, one outer
+ * iteration of cdlm_optimize) moves each chunk in, runs it and moves it back.  The per-filter
+ * state and the fixed-order reductions stay on the device, so the results equal the resident
+ * problem's bit for bit; the price is the host<->device traffic of the whole state per pass.
+ * The reference sizes its batches from the free memory the same way (src/SEARCH/store.jl:109-119).
+ * A problem that fits stays resident.  A streamed problem cannot join a batch or an activation
+ * record (its X_best slices are in host memory: read them with cdlm_get). */
+cdlm_status cdlm_problem_create_streamed(cdlm_ctx *ctx, const cdlm_params *params, const float *s, int32_t L4,
+                                         int32_t N, const float *D0, const float *X0, uint64_t seed,
+                                         uint64_t device_budget, cdlm_problem **out);
+/* Chunks a problem moves through the device in (1: resident). */
+cdlm_status cdlm_problem_chunks(const cdlm_problem *p, int32_t *chunks);
 cdlm_status cdlm_problem_destroy(cdlm_problem *p);
 /* Re-runs the initialisation from the same D0 / X0 / seed without re-uploading s. */
 cdlm_status cdlm_problem_reset(cdlm_problem *p);
@@ -225,8 +241,10 @@ cdlm_status cdlm_server_destroy(cdlm_server *srv);
 /* Message of the last failed server call or job; the string is a copy owned by the calling
  * thread, valid until that thread calls cdlm_server_last_error again. */
 const char *cdlm_server_last_error(cdlm_server *srv);
-/* Queues a job; CDLM_ERR_OOM when its footprint exceeds the whole budget (it could never
- * be admitted), CDLM_ERR_UNSUPPORTED outside the kernels' shape range. */
+/* Queues a job; CDLM_ERR_UNSUPPORTED outside the kernels' shape range.  A job whose footprint
+ * exceeds the whole budget is streamed (cdlm_problem_create_streamed with the budget as its
+ * device window): it waits until nothing else is resident and then runs alone, its state in
+ * pinned host memory; CDLM_ERR_OOM when even its smallest window exceeds the budget. */
 cdlm_status cdlm_server_submit(cdlm_server *srv, const cdlm_job_spec *job, uint64_t *id);
 cdlm_status cdlm_server_poll(cdlm_server *srv, uint64_t id, int32_t *state);
 /* Waits until the job is done or failed, at most timeout_ms (< 0: no limit). */
@@ -261,7 +279,8 @@ typedef struct {
     int32_t csc_waves;
 } cdlm_tuning;
 /* Times every candidate on a scratch problem of n_seq sequences (<= 0: six per CU),
- * keeps the fastest for 15 CSC + 1 dictionary iterations, and returns it in *out (may be NULL). */
+ * each in three interleaved rounds (its fastest round counts), keeps the fastest for 15 CSC + 1
+ * dictionary iterations, and returns it in *out (may be NULL). */
 cdlm_status cdlm_autotune(cdlm_ctx *ctx, int32_t L4, int32_t M, int32_t gamma, int32_t n_seq, cdlm_tuning *out);
 /* Sets (t != NULL) or clears (t == NULL) the choices for a shape; get returns those in force. */
 cdlm_status cdlm_tuning_set(cdlm_ctx *ctx, int32_t L4, int32_t M, int32_t gamma, const cdlm_tuning *t);

@@ -61,7 +61,12 @@ def check(tag, xs, thresh, count, z, sfx="f64", tol=1e-4, log=print, max_flip_fr
     log("%s code threshold %.9g (fp64 oracle %.9g, rel %.2e); positive codes %d (%d)" % (
         tag, thresh, t64, abs(thresh - t64) / t64, count, c64))
     assert abs(thresh - t64) <= tol * t64, (thresh, t64)
-    assert abs(count - c64) <= max(1, tol * c64), (count, c64)
+    # the positive count moves with every code near zero: within 1e-4, or 1.5 x the FP32 oracle's
+    # own distance from FP64 where the fixture holds it
+    c_tol = max(1, tol * c64)
+    if "count_f32" in z:
+        c_tol = max(c_tol, 1.5 * abs(int(z["count_f32"]) - c64))
+    assert abs(count - c64) <= c_tol, (count, c64, c_tol)
     act = active(xs, thresh)
     # uinds: the filters with any active code
     uinds = (np.nonzero(act.any(axis=(0, 2)))[0] + 1).astype(np.int32)

@@ -1,5 +1,5 @@
 """The reference-structured GPU loop used as the plain per-dataset baseline of the bench
-(cdlm.refloop: the reference's code_update!/dict_update!/update_best! statement for statement
+(refloop.py at the repository root: the reference's code_update!/dict_update!/update_best! statement for statement
 on torch/hipFFT) computes the same iteration as the oracle: relative error <= 1e-4 after a
 short schedule, so the baseline it times is the reference's algorithm."""
 import os
@@ -18,7 +18,7 @@ def test_reference_structured_loop_matches_oracle():
     import sys
     sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
     from make_golden import make_cdl_inputs
-    from cdlm.refloop import ReferenceStructuredCDL
+    from refloop import ReferenceStructuredCDL
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     fl = [12, 8, 12, 16, 4]

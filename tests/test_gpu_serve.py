@@ -78,8 +78,8 @@ def test_served_jobs_equal_alone_in_the_mix():
 
 def test_admission_by_footprint():
     """A budget that holds only the largest job: jobs queue, run as memory frees up, never
-    exceed the budget, and still give their alone results; a job larger than the budget is
-    refused at submission."""
+    exceed the budget, and still give their alone results; a job whose smallest streamed window
+    (77 MB) is larger than the budget is refused at submission."""
     sizes = [cdlm.footprint_bytes(L, N, len(fl), max(fl)) for _, N, L, fl, _ in JOBS]
     budget = max(sizes) + min(sizes) // 2
     srv = cdlm.Server(mem_budget=budget, csc_iter=CSC, cdl_iter=CDL_IT)
