@@ -118,8 +118,10 @@ def cpu_baseline(sample_n: int, pwm_seqs: int):
     step (a 1000-sequence slice for all 200 iterations takes about 25 minutes on 8 cores, far
     beyond a benchmark run; the CSC work is per sequence, so seq*iters/s carries over).  c1:
     the CPU-runnable workload in full (N=500, L=100, M=10, filter_len 16, 100 ADMM iterations,
-    one training).  pwm: the PWM-scan workload's hit list on pwm_seqs sequences x 512 PWMs, the
-    oracle's scan in parallel threads over PWM chunks."""
+    one training).  c3_dataset: one dataset of the batched configuration at its real shape, 512 of
+    its sequences for one outer iteration (the served configuration's jobs are such datasets).
+    pwm: the PWM-scan workload's hit list on pwm_seqs sequences x 512 PWMs, the oracle's scan in
+    parallel threads over PWM chunks."""
     import numpy as np
     import oracle as O
     from cdlm import synth
@@ -162,6 +164,27 @@ def cpu_baseline(sample_n: int, pwm_seqs: int):
                  "sample": "the CPU-runnable workload in full: N=500 L=100 M=10 filter_len=16, 10 x (10 CSC + "
                            "update_best + 1 dict) = 100 ADMM iterations, one training, code threshold %.6f" % th1}
     log("bench: CPU baseline, CPU-runnable workload in full: %.2f s (%.0f seq*iters/s)" % (dt1, 500 * 100 / dt1))
+    # one dataset of the batched configuration (its real L, M, filter_len), 512 of its sequences,
+    # one outer iteration of its schedule
+    sp = synth.c3_specs(1)[0]
+    n3, fl3 = 512, [sp["filter_len"]] * sp["M"]
+    b3, _ = synth.planted_bases(sp["seed"], n3, sp["L"], sp["motif_lens"], sp["occupancy"], alpha=0.5)
+    Lg3 = 4 * sp["L"] + 2 * sp["filter_len"]
+    D03 = synth.rand_probvec_filters(sp["seed"], sp["M"], Lg3, fl3)
+    X03 = np.random.default_rng(9).standard_normal((n3, sp["M"], Lg3)).astype(np.float32)
+    o3 = O.OracleCDL(synth.onehot(b3), D03, X03, fl3, RHO, SIGMA, LAM, "f32")
+    t0 = time.perf_counter()
+    o3.code_update(C3_CSC_ITER)
+    o3.update_best()
+    o3.dict_update(C3_CDL_ITER)
+    dt3 = time.perf_counter() - t0
+    del o3, X03
+    out["c3_dataset"] = {"value": n3 * C3_CSC_ITER / dt3, "unit": "seq*iters/s", "seconds": dt3, "cores": threads,
+                         "sample": "the first dataset of the batched configuration (L=%d M=%d filter_len=%d), %d of its "
+                                   "sequences, one outer iteration (%d CSC + update_best + %d dict)"
+                                   % (sp["L"], sp["M"], sp["filter_len"], n3, C3_CSC_ITER, C3_CDL_ITER)}
+    log("bench: CPU baseline, one batched-configuration dataset (L=%d M=%d): %.1f seq*iters/s (%.1f s)"
+        % (sp["L"], sp["M"], n3 * C3_CSC_ITER / dt3, dt3))
     if pwm_seqs > 0:
         from concurrent.futures import ThreadPoolExecutor
         pb, pw, pl, pt = synth.pwm_workload(5, pwm_seqs, PWM_L, PWM_K)
@@ -373,7 +396,9 @@ def served_leg(rate: float, n_jobs: int, policy: str = "fifo"):
         "ticks": st["ticks"], "plain_loop_p50_latency_s": q(plat, 0.5), "plain_loop_p99_latency_s": q(plat, 0.99),
         "plain_loop_datasets_per_s": n / float(done), "results_identical_to_alone": bool(same),
         "note": "the configuration's 1000 jobs per rate were not run (about 10 minutes per rate); the 5% large jobs "
-                "(N=100000, L=500, M=128, about 630 GB of state) exceed the device and are refused",
+                "(N=100000, L=500, M=128: 475 GB of state, more than the device) are not in the sample: the server "
+                "streams such a job through a device window with its state in pinned host memory, and 475 GB of "
+                "pinned memory is more than the measuring machine allows a process",
     }
 
 

@@ -133,15 +133,19 @@ int pick_dict_batch(int Lg, int taps) {
 // Chunks of sequences in the dictionary mean: about 4096 blocks per problem and at least 32
 // chunks, a multiple of the 8 XCDs (k_dict_reduce keeps every tile of a chunk on one XCD, so the
 // chunk's stored Y3 half spectra and kappa, which all its tiles read, come from HBM once into that
-// XCD's L2); chunks may be empty when N is small.  A streamed problem moves through the device in
-// a divisor of this many chunks, so that its means are summed exactly as when it is resident.
+// XCD's L2), and no more than N rounded up to 8 (at least 32): then every chunk holds at most one
+// sequence, the sums are the same as with more chunks, and the partial sums (4 doubles per bin,
+// filter and chunk) stay small for small N.  A streamed problem moves through the device in a
+// divisor of this many chunks, so that its means are summed exactly as when it is resident.
 constexpr int kXcds = 8;
-int pick_nch(int Lg, int M) {
+int pick_nch(int Lg, int M, int N) {
     const size_t HM = (size_t)half_bins(Lg) * M;
     const size_t tiles = (HM + kThreads - 1) / kThreads;
     int nch = (int)((4096 + tiles - 1) / tiles);
     nch = (nch + kXcds - 1) / kXcds * kXcds;
-    return nch < 32 ? 32 : nch;
+    if (nch < 32) nch = 32;
+    const int cap = std::max(32, (N + kXcds - 1) / kXcds * kXcds);
+    return std::min(nch, cap);
 }
 
 // Dynamic LDS above the default needs hipFuncSetAttribute.  The attribute is per kernel
@@ -264,7 +268,7 @@ BufPlan buf_plan(int L4, int N, int M, int gamma, bool keep_x0, int Ntot = 0) {
     b.kap = (size_t)half_bins((int)Lg) * N * sizeof(float2);
     b.M = (size_t)M * sizeof(int);
     b.partials = (size_t)Ntot * kPartStride * sizeof(double);
-    b.dpart = (size_t)pick_nch((int)Lg, M) * kDpart * ((size_t)half_bins((int)Lg) * M) * sizeof(double);
+    b.dpart = (size_t)pick_nch((int)Lg, M, Ntot) * kDpart * ((size_t)half_bins((int)Lg) * M) * sizeof(double);
     b.trace = (size_t)kTraceCap * sizeof(double);
     b.sc = sizeof(CdlScal);
     return b;
@@ -597,7 +601,7 @@ extern "C" size_t cdlm_footprint_bytes(int32_t L4, int32_t N, int32_t M, int32_t
 
 size_t cdlm_stream_min_bytes(int32_t L4, int32_t N, int32_t M, int32_t gamma) {
     if (L4 <= 0 || N <= 0 || M <= 0 || gamma < 0) return 0;
-    const int nch = pick_nch(L4 + 2 * gamma, M);
+    const int nch = pick_nch(L4 + 2 * gamma, M, N);
     int mx = 1;
     for (int c = 0; c < nch; ++c) mx = std::max(mx, (int)((long)(c + 1) * N / nch) - (int)((long)c * N / nch));
     return plan_total(buf_plan(L4, mx, M, gamma, false, N));
@@ -803,7 +807,7 @@ cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const flo
     CdlDev &d = p->dv;
     memset(&d, 0, sizeof(d));
     d.L4 = L4; d.N = N; d.M = M; d.Lg = Lg; d.gamma = gamma; d.L = p->L;
-    d.nch = pick_nch(Lg, M);
+    d.nch = pick_nch(Lg, M, N);
     d.n_off = 0; d.Nt = N; d.red_c0 = 0; d.red_cn = d.nch;
     d.trace_cap = kTraceCap;
     d.rho = prm->rho; d.sigma = prm->sigma; d.lambda = prm->lambda;

@@ -172,7 +172,9 @@ def test_c2_slice_1000_vs_oracle_fixture(reps):
         b32, b64 = ref[f + "_f32"], ref[f + "_f64"]
         e_g64, e_3264, e_g32 = rel(a, b64), rel(b32, b64), rel(a, b32)
         flush("%s %s: gpu-f64 %.2e  f32oracle-f64 %.2e  gpu-f32oracle %.2e" % (tag, f, e_g64, e_3264, e_g32))
-        assert e_g64 <= max(TOL, 1.5 * e_3264), (f, e_g64, e_3264)
+        # at this shape the GPU holds 1e-4 against FP64 outright (measured ~1e-5), tighter than the
+        # FP32 oracle's own distance from FP64
+        assert e_g64 <= TOL, (f, e_g64, e_3264)
     if "thresh_f64" not in ref.files:
         return
     # after thresholding (cdl_optimize, wrap.jl:12; record_training_result!, store.jl:15-19): the

@@ -29,8 +29,9 @@ def _budget(L4, N, M, gamma, x0, chunks):
     """Device bytes for a window of about N / chunks sequences (the per-filter part and the
     per-sequence objective partials of all N come on top of the window)."""
     fp = cdlm.footprint_bytes
-    per_seq = (fp(L4 // 4, N, M, gamma, x0) - fp(L4 // 4, 1, M, gamma, x0)) / (N - 1)
-    fixed = fp(L4 // 4, 1, M, gamma, x0) - per_seq
+    # N and N - 1 sequences have the same dictionary-mean chunk count here (N % 8 != 1)
+    per_seq = fp(L4 // 4, N, M, gamma, x0) - fp(L4 // 4, N - 1, M, gamma, x0)
+    fixed = fp(L4 // 4, N, M, gamma, x0) - per_seq * N
     return int(fixed + per_seq * (-(-N // chunks) + 2) + 64 * 1024)
 
 
