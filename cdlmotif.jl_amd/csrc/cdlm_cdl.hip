@@ -101,11 +101,11 @@ int pick_csc_waves(int Lg, int taps, int nsl, int tuned) {
 size_t lds_bytes_init(int Lg, int taps) { return sizeof(float) * (kWaves * (size_t)(Lg + taps) + kWaves * (size_t)Lg); }
 size_t lds_bytes_best(int Lg, int taps) { return sizeof(float) * (size_t)(Lg + taps + 4); }
 // k_dict with W waves: twiddles, h with its tail, three spectrum accumulators and the stored
-// kappa, two length-Lg complex buffers, and per wave two of length Lg/4 and the stored Y3h.
+// kappa, two length-Lg complex buffers, and per wave two of length Lg/4 + 2 and the stored Y3h.
 size_t lds_bytes_dict(int Lg, int taps, int W) {
     const size_t Hp = (size_t)(((Lg / 2 + 1) + 3) & ~3);
     return sizeof(float) * (2 * (size_t)Lg + (size_t)((Lg + taps + 4 + 3) & ~3) + 7 * Hp + 4 * (size_t)Lg +
-                            (size_t)W * ((size_t)Lg + 2 * (size_t)dict_phx(Lg)));
+                            (size_t)W * ((size_t)Lg + 8 + 2 * (size_t)dict_phx(Lg)));
 }
 size_t lds_bytes_half(int Lg) { return sizeof(float2) * 3 * (size_t)Lg; }
 size_t lds_bytes_fin(int Lg) { return sizeof(float) * (((size_t)Lg + 3) / 4 * 4 + 6 * (size_t)Lg); }

@@ -761,7 +761,8 @@ __global__ __launch_bounds__(kThreads) void k_decide(const CdlDev *__restrict__ 
 // G1hp = F(D): dh = F(D), g1h = 0.
 // Step t, columns m = w, w+W, ... of wave w (W = blockDim/64):
 //   * Y3 = Pi4(X + y3) = Pi4(X) (y3 = 0 on the support) has one sample per slot, so Y3h is a
-//     length-Lg/4 DFT repeated 4 times: one wave-level transform per column;
+//     length-Lg/4 DFT repeated 4 times: one wave-level complex transform per pair of columns
+//     (m, m + W), whose real and imaginary inputs are the two columns' real Y3;
 //   * per bin: dh_{t-1} and g1h_{t-1} from (A, Y3h_{t-1}, kappa_{t-1}); Qh = (G1h - g1h_{t-1})
 //     + (G3h - g3h_{t-1}) (cdl_kernels.jl:8); |Y3h|^2, Y3h dh_{t-1} and Y3h Qh add into the
 //     wave's registers in ascending m (:52-59); g1h_{t-1} replaces A, Y3h_t replaces Y3h_{t-1};
@@ -774,13 +775,13 @@ __global__ __launch_bounds__(kThreads) void k_decide(const CdlDev *__restrict__ 
 // the float4 slots t = lane + 64 i of a spectral column (bins 2t, 2t+1) and s = lane + 64 i
 // of a spatial one.
 // LDS: twiddles | h with wrapped tail | accS, accT, accP | kappa_{t-1} | fa, fb (Lg complex,
-// the block transforms) | per wave: ya, yb (LgMax/4 complex each), yo (Y3h_{t-1}).
+// the block transforms) | per wave: ya, yb (LgMax/4 + 2 complex each: after the transform they
+// hold both columns' Y3h_t and the second column's Y3h_{t-1}), yo (the first column's Y3h_{t-1}).
 // Registers: at most 168 (3 waves per SIMD) for padded lengths up to 1020 and filters up to
 // 24; 256 otherwise.
 template <int TAPS, int NSL>
 __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
     const CdlDev *__restrict__ tab, const int *__restrict__ starts, int np, int LgMax) {
-    constexpr bool kPf = NSL <= 4;              // the column's A in flight across the Y3 transform
     constexpr int NQ = NSL <= 4 ? 4 : 9;        // spectral float4 slots per lane: Hs/2 <= 64 NQ
     constexpr int NY = NSL <= 4 ? 2 : 5;        // stored Y3h bins per lane: Lg/8 + 1 <= 64 NY
     extern __shared__ float4 smem4[];
@@ -798,7 +799,7 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
     float2 *accP = accT + HsX;
     float2 *kold = accP + HsX;
     float2 *fa = kold + HsX, *fb = fa + LgMax;
-    float2 *ya = fb + LgMax + w * (2 * PqX + PhX), *yb = ya + PqX, *yo = yb + PqX;
+    float2 *ya = fb + LgMax + w * (2 * (PqX + 2) + PhX), *yb = ya + PqX + 2, *yo = yb + PqX + 2;
     const size_t ro = (size_t)n * Lg;
     float2 *y3s = P.y3s + (size_t)n * M * Ph;
     float2 *kap = P.kap + (size_t)n * Hs;
@@ -827,30 +828,172 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
     float2 aT[2 * NQ], aP[2 * NQ];
 #pragma unroll
     for (int i = 0; i < 2 * NQ; ++i) { aS[i] = 0.f; aT[i] = make_float2(0.f, 0.f); aP[i] = make_float2(0.f, 0.f); }
+    // the support samples X[4s] of a column (all the transform needs once X is materialised)
+    float xa[NSL], xb[NSL];
+    auto loadx0 = [&](int m, float *x) {
+        const size_t col = ((size_t)n * M + m) * Lg;
+#pragma unroll
+        for (int i = 0; i < NSL; ++i) {
+            const int s = lane + 64 * i;
+            x[i] = (m < M && s < Pq) ? gp(P.Xt)[col + 4 * (size_t)s] : 0.f;
+        }
+    };
+    // X of column m rebuilt from (Xt, h) and written back materialised; its support samples go to
+    // part c (0: real, 1: imaginary) of the transform input
+    auto rebuild = [&](int m, int c) {
+        const size_t col = ((size_t)n * M + m) * Lg;
+        float g[TAPS];
+        load_taps<TAPS>(G1, m, Lg, g);
+        float *yf = reinterpret_cast<float *>(ya) + c;
+#pragma unroll
+        for (int i = 0; i < NSL; ++i) {
+            const int s = lane + 64 * i;
+            if (s < Pq) {
+                float X[4];
+                recon4<TAPS>(ld4(P.Xt + col, s), hb4, s, g, X);
+                st4(P.Xt + col, s, mk4(X));
+                yf[2 * s] = X[0];
+            }
+        }
+    };
+    // The per-bin step of column m: its A (in av, or loaded here when av is null), the stored
+    // Y3h_{t-1} (Yo) and the new Y3h (Yn)
+    auto column_bins = [&](int m, const float4 *av, const float2 *Yo, const float2 *Yn) {
+        const size_t colh = ((size_t)n * M + m) * Hs, fcol = (size_t)m * Hs;
+        float *Acol = reinterpret_cast<float *>(P.g1h + colh);
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            const int t = lane + 64 * i;
+            if (t < nq) {
+                const float4 A = av ? av[i] : ld4(Acol, t);
+                const float4 P1 = ld4(G1hp + 2 * fcol, t), PF = ld4(Fp + 2 * fcol, t);
+                const float4 F1 = ld4(G1h + 2 * fcol, t), F3 = ld4(G3h + 2 * fcol, t), DL = ld4(D13h + 2 * fcol, t);
+                float an[4];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int k = 2 * t + h, j = 2 * i + h;
+                    float2 Ac = make_float2(0.f, 0.f);
+                    if (k < H) {
+                        const float2 a = make_float2(f4(A, 2 * h), f4(A, 2 * h + 1));
+                        const float2 p1 = make_float2(f4(P1, 2 * h), f4(P1, 2 * h + 1));
+                        const float2 pf = make_float2(f4(PF, 2 * h), f4(PF, 2 * h + 1));
+                        const float2 co = cmul(cconj(y3h(Yo, k)), kold[k]);
+                        const float2 dho = make_float2(((p1.x - a.x) + (pf.x - a.x)) + co.x,
+                                                       ((p1.y - a.y) + (pf.y - a.y)) + co.y);   // dh_{t-1}
+                        Ac = make_float2((a.x + dho.x) - p1.x, (a.y + dho.y) - p1.y);            // g1h_{t-1}
+                        const float2 a3 = make_float2(Ac.x - f4(DL, 2 * h), Ac.y - f4(DL, 2 * h + 1));   // g3h_{t-1}
+                        const float2 Q = make_float2((f4(F1, 2 * h) - Ac.x) + (f4(F3, 2 * h) - a3.x),
+                                                     (f4(F1, 2 * h + 1) - Ac.y) + (f4(F3, 2 * h + 1) - a3.y));
+                        const float2 Yk = y3h(Yn, k);
+                        aS[j] += Yk.x * Yk.x + Yk.y * Yk.y;
+                        aT[j] = cadd(aT[j], cmul(Yk, dho));
+                        aP[j] = cadd(aP[j], cmul(Yk, Q));
+                    }
+                    an[2 * h] = Ac.x;
+                    an[2 * h + 1] = Ac.y;
+                }
+                st4(Acol, t, mk4(an));
+            }
+        }
+    };
+    auto loadA = [&](int m, float4 *av) {
+        const float *Acol = reinterpret_cast<const float *>(P.g1h + ((size_t)n * M + m) * Hs);
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            const int t = lane + 64 * i;
+            if (t < nq) av[i] = ld4(Acol, t);
+        }
+    };
+    // Columns in pairs (m, m + W) of the wave's columns w, w + W, ... (one at a time at NSL = 8):
+    // one length-Lg/4 complex transform of Y3_m + i Y3_m2 gives both columns' spectra (both
+    // inputs are real: Y3h_m(k) = (Z(k) + conj Z(-k)) / 2, Y3h_m2(k) = (Z(k) - conj Z(-k)) / 2i);
+    // the per-bin steps then run for m and for m2, so the sums over m keep their ascending order.
+    if constexpr (NSL <= 4) {
+    if (!mat && w < M) { loadx0(w, xa); loadx0(w + W, xb); }
+    for (int m = w; m < M; m += 2 * W) {
+        const int mm = __builtin_amdgcn_readfirstlane(m);
+        const int m2 = mm + W;
+        const bool two = m2 < M;
+        if (mat) {
+            rebuild(mm, 0);
+            if (two) {
+                rebuild(m2, 1);
+            } else {
+                for (int s = lane; s < Pq; s += 64) ya[s].y = 0.f;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NSL; ++i) {
+                const int s = lane + 64 * i;
+                if (s < Pq) ya[s] = make_float2(xa[i], xb[i]);
+            }
+        }
+        // the stored Y3h_{t-1} of both columns and the first column's A, in flight during the transform
+        float4 av[NQ];
+        float2 yv[NY], yv2[NY];
+        loadA(mm, av);
+#pragma unroll
+        for (int i = 0; i < NY; ++i) {
+            const int k = lane + 64 * i;
+            if (k < Ph) {
+                yv[i] = gp(y3s)[(size_t)mm * Ph + k];
+                yv2[i] = two ? gp(y3s)[(size_t)m2 * Ph + k] : make_float2(0.f, 0.f);
+            }
+        }
+        wave_lds_sync();
+        const float2 *Z = team_fft<false>(ya, yb, P.plan4, tw, lane, 64, WaveSync{}, 1);
+        float2 *Zw = (Z == ya) ? ya : yb;
+        float2 *O = (Z == ya) ? yb : ya;
+        float2 y1n[NY], y2n[NY];
+#pragma unroll
+        for (int i = 0; i < NY; ++i) {
+            const int k = lane + 64 * i;
+            if (k < Ph) {
+                const float2 a = Z[k], b = cconj(Z[k == 0 ? 0 : Pq - k]);
+                y1n[i] = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
+                y2n[i] = make_float2(0.5f * (a.y - b.y), -0.5f * (a.x - b.x));
+            }
+        }
+        wave_lds_sync();   // every lane has read Z before it is overwritten
+        // LDS: Y3h_t of m in O[0, Ph), of m2 in Zw[0, Ph); Y3h_{t-1} of m in yo, of m2 in O[Ph, 2 Ph)
+#pragma unroll
+        for (int i = 0; i < NY; ++i) {
+            const int k = lane + 64 * i;
+            if (k < Ph) {
+                O[k] = y1n[i];
+                Zw[k] = y2n[i];
+                yo[k] = yv[i];
+                O[Ph + k] = yv2[i];
+                gp(y3s)[(size_t)mm * Ph + k] = y1n[i];
+                if (two) gp(y3s)[(size_t)m2 * Ph + k] = y2n[i];
+            }
+        }
+        wave_lds_sync();
+        // the next pair's support samples, in flight during this pair's per-bin steps
+        if (!mat && mm + 2 * W < M) { loadx0(mm + 2 * W, xa); loadx0(mm + 3 * W, xb); }
+        column_bins(mm, av, yo, O);
+        if (two) {
+            loadA(m2, av);
+            column_bins(m2, av, O + Ph, Zw);
+        }
+        wave_lds_sync();
+    }
+    } else {
+    // NSL = 8: one column per transform (its per-bin step's registers leave no room for a second)
     float4 nx[NSL];
-    auto loadx = [&](size_t col) {
+    auto loadx = [&](int m) {
+        const size_t col = ((size_t)n * M + m) * Lg;
 #pragma unroll
         for (int i = 0; i < NSL; ++i) {
             const int s = lane + 64 * i;
             if (s < Pq) nx[i] = ld4(P.Xt + col, s);
         }
     };
-    if (w < M) loadx(((size_t)n * M + w) * Lg);
+    if (w < M) loadx(w);
     for (int m = w; m < M; m += W) {
         const int mm = __builtin_amdgcn_readfirstlane(m);
         const size_t col = ((size_t)n * M + mm) * Lg;
-        const size_t colh = ((size_t)n * M + mm) * Hs, fcol = (size_t)mm * Hs;
-        float *Acol = reinterpret_cast<float *>(P.g1h + colh);
-        // this column's stored Y3h_{t-1} (and with prefetch its A), in flight during the Y3 transform
-        float4 av[kPf ? NQ : 1];
         float2 yv[NY];
-        if constexpr (kPf) {
-#pragma unroll
-            for (int i = 0; i < NQ; ++i) {
-                const int t = lane + 64 * i;
-                if (t < nq) av[i] = ld4(Acol, t);
-            }
-        }
 #pragma unroll
         for (int i = 0; i < NY; ++i) {
             const int k = lane + 64 * i;
@@ -872,7 +1015,7 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
                 ya[s] = make_float2(x0, 0.f);
             }
         }
-        if (mm + W < M) loadx(col + (size_t)W * Lg);
+        if (mm + W < M) loadx(mm + W);
 #pragma unroll
         for (int i = 0; i < NY; ++i) {
             const int k = lane + 64 * i;
@@ -881,41 +1024,9 @@ __global__ __launch_bounds__(512, (NSL <= 4 && TAPS <= 24 ? 3 : 2)) void k_dict(
         wave_lds_sync();
         const float2 *Y = team_fft<false>(ya, yb, P.plan4, tw, lane, 64, WaveSync{}, 1);
         for (int k = lane; k < Ph; k += 64) gp(y3s)[(size_t)mm * Ph + k] = Y[k];   // the old values are in yo
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) {
-            const int t = lane + 64 * i;
-            if (t < nq) {
-                const float4 A = kPf ? av[kPf ? i : 0] : ld4(Acol, t);
-                const float4 P1 = ld4(G1hp + 2 * fcol, t), PF = ld4(Fp + 2 * fcol, t);
-                const float4 F1 = ld4(G1h + 2 * fcol, t), F3 = ld4(G3h + 2 * fcol, t), DL = ld4(D13h + 2 * fcol, t);
-                float an[4];
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int k = 2 * t + h, j = 2 * i + h;
-                    float2 Ac = make_float2(0.f, 0.f);
-                    if (k < H) {
-                        const float2 a = make_float2(f4(A, 2 * h), f4(A, 2 * h + 1));
-                        const float2 p1 = make_float2(f4(P1, 2 * h), f4(P1, 2 * h + 1));
-                        const float2 pf = make_float2(f4(PF, 2 * h), f4(PF, 2 * h + 1));
-                        const float2 co = cmul(cconj(y3h(yo, k)), kold[k]);
-                        const float2 dho = make_float2(((p1.x - a.x) + (pf.x - a.x)) + co.x,
-                                                       ((p1.y - a.y) + (pf.y - a.y)) + co.y);   // dh_{t-1}
-                        Ac = make_float2((a.x + dho.x) - p1.x, (a.y + dho.y) - p1.y);            // g1h_{t-1}
-                        const float2 a3 = make_float2(Ac.x - f4(DL, 2 * h), Ac.y - f4(DL, 2 * h + 1));   // g3h_{t-1}
-                        const float2 Q = make_float2((f4(F1, 2 * h) - Ac.x) + (f4(F3, 2 * h) - a3.x),
-                                                     (f4(F1, 2 * h + 1) - Ac.y) + (f4(F3, 2 * h + 1) - a3.y));
-                        const float2 Yk = y3h(Y, k);
-                        aS[j] += Yk.x * Yk.x + Yk.y * Yk.y;
-                        aT[j] = cadd(aT[j], cmul(Yk, dho));
-                        aP[j] = cadd(aP[j], cmul(Yk, Q));
-                    }
-                    an[2 * h] = Ac.x;
-                    an[2 * h + 1] = Ac.y;
-                }
-                st4(Acol, t, mk4(an));
-            }
-        }
+        column_bins(mm, nullptr, yo, Y);
         wave_lds_sync();
+    }
     }
     if (mat)
         for (int l = tid; l < Lg; l += THR) gp(P.h)[ro + l] = 0.f;
