@@ -40,7 +40,8 @@ class cdlm_params(C.Structure):
 
 class cdlm_tuning(C.Structure):
     _fields_ = [("fft_order", C.c_int32), ("fft4_order", C.c_int32), ("dict_batch", C.c_int32),
-                ("csc_slots", C.c_int32), ("csc_waves", C.c_int32)]
+                ("csc_slots", C.c_int32), ("csc_waves", C.c_int32),
+                ("csc_mfma", C.c_int32)]
 
 
 class cdlm_server_opts(C.Structure):

@@ -65,6 +65,7 @@ struct TimedLaunch {
 struct LaunchSet {
     int np = 0;
     int taps = 0, nsl = 4, B = 2, LgMax = 0, waves = 4;
+    int mfma = 1;   // CSC kernel: 1 = k_csc (vector ALUs), 2 = k_csc_tz (matrix cores)
     int blk_seq = 0, blk_filt = 0, blk_cs = 0, blk_red = 0;
     void *mem = nullptr;  // descriptors followed by the prefix arrays
     CdlDev *tab = nullptr;
@@ -80,12 +81,15 @@ struct LaunchSet {
 };
 
 constexpr size_t kLdsLimit = 160 * 1024;
+// Untuned shapes run k_csc_tz (where its LDS fits) instead of k_csc.
+constexpr bool kCscMfmaDefault = false;
 constexpr int kTraceCap = 4096;
 
 size_t lds_bytes_csc(int Lg, int taps, int waves = kWaves) {
     return sizeof(float) * (2 * (size_t)Lg + (size_t)(Lg + taps + 4) + waves * (size_t)(Lg + taps) + waves * (size_t)Lg +
                             4 * (size_t)Lg);
 }
+size_t lds_bytes_csctz(int Lg, int taps, int waves) { return sizeof(float) * csctz_lds_floats(Lg, taps, waves); }
 // CSC waves per work-group where the slots fit 4 per lane: 8 when two such work-groups' LDS
 // fits a CU (two sequences per CU, no register prefetch), else 16 when its LDS fits (one
 // sequence per CU), else 4; a tuning of 4, 8 or 16 overrides when it fits.
@@ -97,6 +101,27 @@ int pick_csc_waves(int Lg, int taps, int nsl, int tuned) {
     if (tuned == 16) return ok16 ? 16 : 4;
     if (tuned == 4) return 4;
     return two8 ? 8 : ok16 ? 16 : 4;
+}
+// CSC kernel and waves per work-group.  k_csc_tz (mfma = 2, or 0 where it is the default) runs
+// 8 waves where two work-groups' LDS fits a CU, else 16 (one per CU), or the tuned 8 or 16
+// where that fits; otherwise, and for mfma = 1, k_csc with pick_csc_waves.
+void pick_csc(int Lg, int taps, int nsl, int tuned_mfma, int tuned_waves, int *mfma, int *waves) {
+    if (tuned_mfma == 2 || (tuned_mfma == 0 && kCscMfmaDefault)) {
+        const size_t b8 = lds_bytes_csctz(Lg, taps, 8), b16 = lds_bytes_csctz(Lg, taps, 16);
+        int wv = 0;
+        if (tuned_waves == 8 && b8 <= kLdsLimit) wv = 8;
+        else if (tuned_waves == 16 && b16 <= kLdsLimit) wv = 16;
+        else if (2 * b8 <= kLdsLimit) wv = 8;
+        else if (b16 <= kLdsLimit) wv = 16;
+        else if (b8 <= kLdsLimit) wv = 8;
+        if (wv) {
+            *mfma = 2;
+            *waves = wv;
+            return;
+        }
+    }
+    *mfma = 1;
+    *waves = pick_csc_waves(Lg, taps, nsl, tuned_mfma == 2 ? 0 : tuned_waves);
 }
 size_t lds_bytes_init(int Lg, int taps) { return sizeof(float) * (kWaves * (size_t)(Lg + taps) + kWaves * (size_t)Lg); }
 size_t lds_bytes_best(int Lg, int taps) { return sizeof(float) * (size_t)(Lg + taps + 4); }
@@ -171,6 +196,7 @@ struct Tuning {
     int B = 0;                    // dictionary waves per work-group, one column each (0: by occupancy)
     int nsl = 0;                  // CSC float4 slots per lane (0: the fewest that cover Lg)
     int waves = 0;                // CSC waves per work-group (4 or 16; 0: the default for the shape)
+    int mfma = 0;                 // CSC kernel (1: k_csc, 2: k_csc_tz; 0: the default for the shape)
 };
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int, int>, Tuning> g_tune;
@@ -338,8 +364,8 @@ cdlm_status set_build(cdlm_ctx *ctx, cdlm_problem *const *ps, int np, LaunchSet 
     if ((tuned_B == 2 || tuned_B == 4 || tuned_B == 8) && lds_bytes_dict(Lg, taps, tuned_B) <= kLdsLimit - kThreads * sizeof(double))
         T.B = tuned_B;
     if (T.B == 0) return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "padded length %d: dictionary step exceeds LDS", Lg);
-    T.waves = pick_csc_waves(Lg, taps, T.nsl, ps[lead]->tune.waves);
-    T.lds_csc = lds_bytes_csc(Lg, taps, T.waves);
+    pick_csc(Lg, taps, T.nsl, ps[lead]->tune.mfma, ps[lead]->tune.waves, &T.mfma, &T.waves);
+    T.lds_csc = T.mfma == 2 ? lds_bytes_csctz(Lg, taps, T.waves) : lds_bytes_csc(Lg, taps, T.waves);
     T.lds_init = lds_bytes_init(Lg, taps);
     T.lds_best = lds_bytes_best(Lg, taps);
     T.lds_dict = lds_bytes_dict(Lg, taps, T.B);
@@ -358,6 +384,10 @@ cdlm_status set_build(cdlm_ctx *ctx, cdlm_problem *const *ps, int np, LaunchSet 
         if ((st = ensure_lds(ctx, (const void *)k_csc<TT, 8, 4>, T.lds_csc)) != CDLM_OK) return st;   \
         if ((st = ensure_lds(ctx, (const void *)k_csc<TT, 4, 16>, T.lds_csc)) != CDLM_OK) return st;  \
         if ((st = ensure_lds(ctx, (const void *)k_csc<TT, 4, 8>, T.lds_csc)) != CDLM_OK) return st;   \
+        if ((st = ensure_lds(ctx, (const void *)k_csc_tz<TT, 4, 8>, T.lds_csc)) != CDLM_OK) return st;   \
+        if ((st = ensure_lds(ctx, (const void *)k_csc_tz<TT, 4, 16>, T.lds_csc)) != CDLM_OK) return st;  \
+        if ((st = ensure_lds(ctx, (const void *)k_csc_tz<TT, 8, 8>, T.lds_csc)) != CDLM_OK) return st;   \
+        if ((st = ensure_lds(ctx, (const void *)k_csc_tz<TT, 8, 16>, T.lds_csc)) != CDLM_OK) return st;  \
         if ((st = ensure_lds(ctx, (const void *)k_best<TT>, T.lds_best)) != CDLM_OK) return st;    \
         if ((st = ensure_lds(ctx, (const void *)k_dict<TT, 4>, T.lds_dict)) != CDLM_OK) return st;         \
         if ((st = ensure_lds(ctx, (const void *)k_dict<TT, 8>, T.lds_dict)) != CDLM_OK) return st;         \
@@ -464,7 +494,19 @@ cdlm_status run_csc(cdlm_ctx *ctx, const LaunchSet &S, Timer *tm, int iters) {
     {
         TimedLaunch tl(tm, st, 0, iters);
 #define L_CSC(T)                                                                                                     \
-    if (S.waves == 16)                                                                                               \
+    if (S.mfma == 2 && S.LgMax <= 1024 && S.waves == 16)                                                             \
+        hipLaunchKernelGGL((k_csc_tz<T, 4, 16>), dim3(S.blk_seq), dim3(64 * 16), S.lds_csc, st, S.tab, S.st_seq, S.np, \
+                           iters);                                                                                   \
+    else if (S.mfma == 2 && S.LgMax <= 1024)                                                                         \
+        hipLaunchKernelGGL((k_csc_tz<T, 4, 8>), dim3(S.blk_seq), dim3(64 * 8), S.lds_csc, st, S.tab, S.st_seq, S.np,  \
+                           iters);                                                                                   \
+    else if (S.mfma == 2 && S.waves == 16)                                                                           \
+        hipLaunchKernelGGL((k_csc_tz<T, 8, 16>), dim3(S.blk_seq), dim3(64 * 16), S.lds_csc, st, S.tab, S.st_seq, S.np, \
+                           iters);                                                                                   \
+    else if (S.mfma == 2)                                                                                            \
+        hipLaunchKernelGGL((k_csc_tz<T, 8, 8>), dim3(S.blk_seq), dim3(64 * 8), S.lds_csc, st, S.tab, S.st_seq, S.np,  \
+                           iters);                                                                                   \
+    else if (S.waves == 16)                                                                                               \
         hipLaunchKernelGGL((k_csc<T, 4, 16>), dim3(S.blk_seq), dim3(64 * 16), S.lds_csc, st, S.tab, S.st_seq, S.np,  \
                            iters);                                                                                   \
     else if (S.waves == 8)                                                                                           \
@@ -1310,10 +1352,10 @@ namespace {
 // Shape class of a problem run alone: members of one class share a launch set without
 // lowering each other's occupancy.
 struct ShapeClass {
-    int nsl, B, taps, waves, occ_csc, occ_dict;
+    int nsl, B, taps, waves, mfma, occ_csc, occ_dict;
     bool operator<(const ShapeClass &o) const {
-        return std::tie(nsl, B, taps, waves, occ_csc, occ_dict) <
-               std::tie(o.nsl, o.B, o.taps, o.waves, o.occ_csc, o.occ_dict);
+        return std::tie(nsl, B, taps, waves, mfma, occ_csc, occ_dict) <
+               std::tie(o.nsl, o.B, o.taps, o.waves, o.mfma, o.occ_csc, o.occ_dict);
     }
 };
 
@@ -1325,9 +1367,12 @@ ShapeClass shape_class(const cdlm_problem *p) {
         lds_bytes_dict(p->Lg, p->gamma, p->tune.B) <= kLdsLimit - kThreads * sizeof(double))
         c.B = p->tune.B;
     c.taps = p->gamma;
-    c.waves = pick_csc_waves(p->Lg, p->gamma, c.nsl, p->tune.waves);
-    c.occ_csc = std::min<int>(c.waves == 16 ? 1 : c.waves == 8 ? 2 : (c.nsl == 4 && c.taps <= 20 ? 3 : 2),
-                              (int)(kLdsLimit / lds_bytes_csc(p->Lg, p->gamma, c.waves)));
+    pick_csc(p->Lg, p->gamma, c.nsl, p->tune.mfma, p->tune.waves, &c.mfma, &c.waves);
+    if (c.mfma == 2)
+        c.occ_csc = std::min<int>(c.waves == 8 ? 2 : 1, (int)(kLdsLimit / lds_bytes_csctz(p->Lg, p->gamma, c.waves)));
+    else
+        c.occ_csc = std::min<int>(c.waves == 16 ? 1 : c.waves == 8 ? 2 : (c.nsl == 4 && c.taps <= 20 ? 3 : 2),
+                                  (int)(kLdsLimit / lds_bytes_csc(p->Lg, p->gamma, c.waves)));
     c.occ_dict = c.B > 0 ? dict_wg_per_cu(p->Lg, p->gamma, c.B) : 0;
     return c;
 }
@@ -1620,7 +1665,8 @@ extern "C" cdlm_status cdlm_tuning_set(cdlm_ctx *ctx, int32_t L4, int32_t M, int
     if (t->fft_order < 0 || t->fft_order >= kFftOrders || t->fft4_order < 0 || t->fft4_order >= kFftOrders ||
         (t->dict_batch != 0 && t->dict_batch != 2 && t->dict_batch != 4 && t->dict_batch != 8) ||
         (t->csc_slots != 0 && t->csc_slots != 4 && t->csc_slots != 8) ||
-        (t->csc_waves != 0 && t->csc_waves != 4 && t->csc_waves != 8 && t->csc_waves != 16))
+        (t->csc_waves != 0 && t->csc_waves != 4 && t->csc_waves != 8 && t->csc_waves != 16) ||
+        t->csc_mfma < 0 || t->csc_mfma > 2 || (t->csc_mfma == 2 && t->csc_waves == 4))
         return cdlm_fail(ctx, CDLM_ERR_ARG, "bad tuning values");
     Tuning tu;
     tu.order = t->fft_order;
@@ -1628,6 +1674,7 @@ extern "C" cdlm_status cdlm_tuning_set(cdlm_ctx *ctx, int32_t L4, int32_t M, int
     tu.B = t->dict_batch;
     tu.nsl = t->csc_slots;
     tu.waves = t->csc_waves;
+    tu.mfma = t->csc_mfma;
     g_tune[key] = tu;
     return CDLM_OK;
 }
@@ -1642,6 +1689,7 @@ extern "C" cdlm_status cdlm_tuning_get(cdlm_ctx *ctx, int32_t L4, int32_t M, int
     out->dict_batch = tu.B;
     out->csc_slots = tu.nsl;
     out->csc_waves = tu.waves;
+    out->csc_mfma = tu.mfma;
     return CDLM_OK;
 }
 
@@ -1700,9 +1748,10 @@ extern "C" cdlm_status cdlm_autotune(cdlm_ctx *ctx, int32_t L4, int32_t M, int32
         return set_build(ctx, &p, 1, &p->set);
     };
     const int need = (Lg / 4 <= 256) ? 4 : 8;
-    // CSC candidates: (slots per lane, waves per work-group)
-    const int kCsc = 4;
-    const int cand_nsl[kCsc] = {4, 4, 4, 8}, cand_waves[kCsc] = {4, 8, 16, 4};
+    // CSC candidates: (kernel, slots per lane, waves per work-group)
+    const int kCsc = 6;
+    const int cand_mfma[kCsc] = {1, 1, 1, 1, 2, 2}, cand_nsl[kCsc] = {4, 4, 4, 8, 0, 0},
+              cand_waves[kCsc] = {4, 8, 16, 4, 8, 16};
     const int kDict = 3;
     const int cand_B[kDict] = {2, 4, 8};
     float best_csc[kFftOrders][kCsc], best_dict[kFftOrders][kFftOrders][kDict];
@@ -1722,9 +1771,12 @@ extern "C" cdlm_status cdlm_autotune(cdlm_ctx *ctx, int32_t L4, int32_t M, int32
         for (int o = 0; o < kFftOrders && st == CDLM_OK; ++o) {
             for (int ci = 0; ci < kCsc && st == CDLM_OK; ++ci) {
                 const int nsl = cand_nsl[ci];
-                if (nsl < need || pick_csc_waves(Lg, gamma, nsl, cand_waves[ci]) != cand_waves[ci]) continue;
+                if (cand_mfma[ci] == 1 && nsl < need) continue;
+                int pk = 0, pw = 0;
+                pick_csc(Lg, gamma, nsl < need ? need : nsl, cand_mfma[ci], cand_waves[ci], &pk, &pw);
+                if (pk != cand_mfma[ci] || pw != cand_waves[ci]) continue;
                 Tuning tu;
-                tu.order = o; tu.nsl = nsl; tu.waves = cand_waves[ci];
+                tu.order = o; tu.nsl = nsl; tu.waves = cand_waves[ci]; tu.mfma = cand_mfma[ci];
                 float t = 0.f;
                 if ((st = apply(tu)) == CDLM_OK && (st = timed(false, 20, &t)) == CDLM_OK) keep_min(best_csc[o][ci], t);
             }
@@ -1749,6 +1801,7 @@ extern "C" cdlm_status cdlm_autotune(cdlm_ctx *ctx, int32_t L4, int32_t M, int32
                         have = true;
                         best_t = t;
                         best.order = o; best.order4 = o4; best.nsl = cand_nsl[ci]; best.waves = cand_waves[ci];
+                        best.mfma = cand_mfma[ci];
                         best.B = cand_B[bi];
                     }
                 }
@@ -1768,6 +1821,7 @@ extern "C" cdlm_status cdlm_autotune(cdlm_ctx *ctx, int32_t L4, int32_t M, int32
         out->dict_batch = best.B;
         out->csc_slots = best.nsl;
         out->csc_waves = best.waves;
+        out->csc_mfma = best.mfma;
     }
     return CDLM_OK;
 }

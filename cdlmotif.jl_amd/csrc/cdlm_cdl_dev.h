@@ -613,6 +613,258 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
   }
 }
 
+// ---------------------------------------------------------------------------- CSC on the matrix cores
+// The same iterations as k_csc (train.jl:5-43), one wave per column as there (so every
+// load and store of the streamed state is one contiguous 1 KB run per wave), with the
+// filter_len-tap correlation that rebuilds X and the convolution that forms z on
+// v_mfma_f32_16x16x4_f32 instead of the vector ALUs.  Positions 256b .. 256b + 255 of a
+// column form a 16 x 16 block D[v][u] = position 256b + 16u + v; in the MFMA result layout
+// (register r of lane l is D[4(l/16) + r][l%16]) lane (a = l/16, c = l%16) holds positions
+// 256b + 16c + 4a .. + 3, which is float4 slot 64b + 4c + a of the column.
+//   rebuild:  D = Xt + T H with the Toeplitz T[v][k] = g[k - v] (16 x (TAPS + 16), zero off the
+//             filter) and the Hankel block H[k][u] = h[256b + 16u + k];
+//   convolve: z-block += T' V with T'[v][k] = g[v - k + TAPS - 1] and V[k][u] =
+//             v[256b + 16u + k - TAPS + 1] (v of the column in LDS with a wrapped head), summed
+//             over the wave's columns in the accumulator; lane l supplies A[l%16][l/16] and
+//             B[l/16][l%16], so T and T' are KS = (TAPS + 16)/4 registers per column, read
+//             from the filter in LDS with zero margins.
+// The B operands of a lane step 16 positions per lane across a block, so h and v sit in LDS
+// with 4 floats of padding after every 16 (word q at q + 4 floor(q/16)): the 64 lanes of one
+// operand read then fall in distinct banks (two-way at most).
+// The Toeplitz form spends TAPS + 16 products per output instead of TAPS; the matrix cores
+// have the room, and the vector ALUs keep only the shrink / dual updates.  The wave partials of
+// z are combined in a fixed order (waves 0 and 1 store, later waves add in rounds of two, then
+// the two slots in order), so results are deterministic and do not depend on the launch's other problems.
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ constexpr int csctz_ks(int taps) { return (taps + 16) / 4; }
+// LDS word of position-word q in the padded h and v buffers
+__host__ __device__ constexpr int csctz_pad(int q) { return q + 4 * (q >> 4); }
+// words of the column v buffer (and of h with its wrapped tail), and their padded floats
+__host__ __device__ constexpr int csctz_vl(int Lg, int taps) { return 256 * ((Lg + 255) / 256) + taps + 4; }
+__host__ __device__ constexpr int csctz_vlp(int Lg, int taps) { return csctz_pad(csctz_vl(Lg, taps) + 15); }
+__host__ __device__ constexpr int csctz_nzs(int waves) { return waves < 2 ? waves : 2; }
+// Dynamic LDS of k_csc_tz in floats: twiddles, h, the waves' filters, the waves' v columns
+// (also the FFT buffers), the z slots, y1, Y1, DX and the data window.
+__host__ __device__ constexpr size_t csctz_lds_floats(int Lg, int taps, int waves) {
+    return 2 * (size_t)Lg + (size_t)csctz_vlp(Lg, taps) + 64 * (size_t)waves + (size_t)waves * csctz_vlp(Lg, taps) +
+           (size_t)csctz_nzs(waves) * Lg + 4 * (size_t)Lg;
+}
+
+// Registers: two 8-wave work-groups per CU for padded lengths up to 1024 (four 256-position
+// blocks), one beyond that (the LDS of longer columns holds one), so 256 registers per lane.
+template <int TAPS, int NBT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES, (NBT > 4 && WAVES == 8 ? 2 : 4)) void k_csc_tz(
+    const CdlDev *__restrict__ tab, const int *__restrict__ starts, int np, int iters) {
+    constexpr int kThr = 64 * WAVES;
+    constexpr int kKS = csctz_ks(TAPS);
+    extern __shared__ float4 smem4[];
+    int n;
+    const CdlDev &P = tab[find_problem(starts, np, blockIdx.x, n)];
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int a = lane >> 4, c = lane & 15;
+    const int Lg = P.Lg, M = P.M, nslot = Lg >> 2;
+    const int nb = (Lg + 255) >> 8;
+    const int vl = csctz_vl(Lg, TAPS), vlp = csctz_vlp(Lg, TAPS);
+    constexpr int nzs = csctz_nzs(WAVES);
+    float *sm = reinterpret_cast<float *>(smem4);
+    float2 *tw = reinterpret_cast<float2 *>(sm);
+    float *hb = sm + 2 * Lg;                    // word q = h[q mod Lg], q < vl (padded)
+    float *gw = hb + vlp + 64 * w;              // this wave's filter: gw[16 + j] = g[j], zero margins
+    float *vbase = hb + vlp + 64 * WAVES;
+    float *vb = vbase + w * vlp;                // this wave's v column: word q = v[q - TAPS + 1] (padded)
+    float *zs = vbase + WAVES * vlp;            // nzs z slots of length Lg
+    float *y1s = zs + nzs * Lg;                 // y1, Y1, DX and the data window, kept in LDS
+    float *Y1s = y1s + Lg;
+    float *DXs = Y1s + Lg;
+    float *bs = DXs + Lg;
+    float2 *fa = reinterpret_cast<float2 *>(vbase);
+    float2 *fb = fa + Lg;
+    const size_t ro = (size_t)n * Lg;
+    for (int l = tid; l < Lg; l += kThr) {
+        tw[l] = gp(P.tw)[l];
+        y1s[l] = gp(P.y1)[ro + l];
+        Y1s[l] = gp(P.Y1)[ro + l];
+        DXs[l] = gp(P.DX)[ro + l];
+        const bool win = (l >= P.gamma) && (l < P.gamma + P.L4);
+        bs[l] = win ? gp(P.s)[(size_t)n * P.L4 + (l - P.gamma)] : 0.f;
+    }
+    for (int l = tid; l < vl; l += kThr) hb[csctz_pad(l)] = gp(P.h)[ro + (l < Lg ? l : l % Lg)];
+    if (n == 0 && tid == 0) P.sc->xmat = 0;   // this launch leaves h != 0
+    const float *G1 = g1cur_of(P);
+    const float lam = P.lambda;
+    const float inv_c0 = 1.0f / P.sh_csc;
+    const int nw = M < WAVES ? M : WAVES;     // waves that own columns
+    // streamed state of block b of column mc (packed: (u3, u2), y3 not read); lanes past the
+    // column keep what they hold (their rows of the result are never used).  Loading the next
+    // block ahead of the current one measured slower (3.61 vs 3.52 ms per iteration at
+    // Lg = 840, M = 64): the matrix-core chains already overlap the loads of other waves.
+    float *const Xtp = P.Xt, *const y2p = P.y2, *const y3p = P.y3;
+    auto load_blk = [&](int mc, int b, bool packed, float4 &x, float4 &y, float4 &z) {
+        const int s = 64 * b + 4 * c + a;
+        if (s < nslot) {
+            const size_t col = ((size_t)n * M + mc) * Lg;
+            x = ld4(Xtp + col, s);
+            y = ld4(y2p + col, s);
+            z = packed ? make_float4(0.f, 0.f, 0.f, 0.f) : ld3(y3p + col3_of(n, mc, M, nslot), s);
+        }
+    };
+#pragma nounroll
+  for (int it = 0; it < iters; ++it) {
+    const bool rdU = it > 0, wrU = it + 1 < iters;   // (u2, u3) read / written instead of (Xt, y2, y3)
+    __syncthreads();
+    // v past Lg is read only against zero Toeplitz entries; keep it finite (the region held
+    // FFT data or nothing)
+    for (int i = TAPS - 1 + Lg + lane; i < vl; i += 64) vb[csctz_pad(i)] = 0.f;
+    floatx4 zacc[NBT];
+#pragma unroll
+    for (int b = 0; b < NBT; ++b) zacc[b] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int m = w; m < M; m += WAVES) {
+        const int mm = __builtin_amdgcn_readfirstlane(m);
+        const size_t col = ((size_t)n * M + mm) * Lg, col3 = col3_of(n, mm, M, nslot);
+        {
+            const int j = lane - 16;
+            gw[lane] = (j >= 0 && j < TAPS && j < Lg) ? gp(G1)[(size_t)mm * Lg + j] : 0.f;
+        }
+        wave_lds_sync();
+        float tr[kKS], tc[kKS];
+#pragma unroll
+        for (int ks = 0; ks < kKS; ++ks) {
+            tr[ks] = gw[16 + 4 * ks + a - c];
+            tc[ks] = gw[16 + c - 4 * ks - a + TAPS - 1];
+        }
+        // X, the shrink / dual updates and v, block by block
+#pragma unroll
+        for (int b = 0; b < NBT; ++b) {
+            if (b < nb) {
+                const int s = 64 * b + 4 * c + a;
+                const bool ok = s < nslot;
+                float4 xv = make_float4(0.f, 0.f, 0.f, 0.f), b2 = xv, b3 = xv;
+                load_blk(mm, b, rdU, xv, b2, b3);
+                if (rdU) csc_unpack(xv, b2, b3, lam, inv_c0);
+                floatx4 acc = {xv.x, xv.y, xv.z, xv.w};
+                // word 256b + 16c + 4ks + a: (4ks + a) / 16 = ks / 4
+                const float *hp = hb + 320 * b + 20 * c + a;
+#pragma unroll
+                for (int ks = 0; ks < kKS; ++ks)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(tr[ks], hp[4 * ks + 4 * (ks >> 2)], acc, 0, 0, 0);
+                float u2[4], y2n[4], y3n[4], v[4], xn[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float x = acc[q];
+                    const float t = x + f4(b2, q);                      // shrink! (csc_kernels.jl:58-59)
+                    const float at = fabsf(t) - lam;
+                    const float Y2 = at > 0.f ? copysignf(at, t) : 0.f;
+                    const float Y3 = (q == 0) ? x : 0.f;                  // projectionX! (:73-77), y3 = 0 there
+                    u2[q] = t;
+                    y2n[q] = t - Y2;                                      // train.jl:13, (y2 + X) - Y2
+                    y3n[q] = (q == 0) ? 0.f : f4(b3, q) + x;              // train.jl:14
+                    v[q] = (Y2 - y2n[q]) + (Y3 - y3n[q]);
+                    xn[q] = __fmul_rn(v[q], inv_c0);
+                }
+                if (ok) {
+                    if (wrU) {
+                        const float u3[4] = {acc[0], y3n[1], y3n[2], y3n[3]};
+                        st4(y2p + col, s, mk4(u2));
+                        st4(Xtp + col, s, mk4(u3));
+                    } else {
+                        st4(y2p + col, s, mk4(y2n));
+                        st3(y3p + col3, s, y3n);
+                        st4(Xtp + col, s, mk4(xn));
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) vb[csctz_pad(4 * s + q + TAPS - 1)] = v[q];
+                }
+
+            }
+        }
+        wave_lds_sync();
+        if (lane < TAPS - 1) vb[csctz_pad(lane)] = vb[csctz_pad(lane + Lg)];   // wrapped head, v[lane - TAPS + 1]
+        wave_lds_sync();
+        // z-blocks += T' V
+#pragma unroll
+        for (int b = 0; b < NBT; ++b) {
+            if (b < nb) {
+                const float *vp = vb + 320 * b + 20 * c + a;      // word 256b + 16c + 4ks + a
+#pragma unroll
+                for (int ks = 0; ks < kKS; ++ks)
+                    zacc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(tc[ks], vp[4 * ks + 4 * (ks >> 2)], zacc[b], 0, 0, 0);
+            }
+        }
+        wave_lds_sync();   // the next column rewrites gw and vb
+    }
+    // z slots: waves 0 .. nzs-1 store their partial, later waves add theirs in rounds of nzs
+#pragma unroll
+    for (int k = 0; k * nzs < WAVES; ++k) {
+        if (w / nzs == k && w < nw) {
+            float *zp = zs + (w - k * nzs) * Lg;
+#pragma unroll
+            for (int b = 0; b < NBT; ++b) {
+                const int s = 64 * b + 4 * c + a;
+                if (b < nb && s < nslot) {
+                    float4 *q4 = reinterpret_cast<float4 *>(zp + 4 * s);
+                    const float4 zv = make_float4(zacc[b][0], zacc[b][1], zacc[b][2], zacc[b][3]);
+                    if (k == 0) {
+                        *q4 = zv;
+                    } else {
+                        const float4 o = *q4;
+                        *q4 = make_float4(o.x + zv.x, o.y + zv.y, o.z + zv.z, o.w + zv.w);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // z = the slots in order; u = Y1 - y1' with y1' = y1 + DX - Y1 (train.jl:12)
+    const int nzu = nw < nzs ? nw : nzs;
+    for (int l = tid; l < Lg; l += kThr) {
+        float z = zs[l];
+        for (int k = 1; k < nzu; ++k) z += zs[k * Lg + l];
+        const float Y1o = Y1s[l];
+        const float y1n = y1s[l] + DXs[l] - Y1o;
+        y1s[l] = y1n;
+        fa[l] = make_float2(z, Y1o - y1n);
+    }
+    __syncthreads();
+    float2 *C = team_fft<false>(fa, fb, P.plan, tw, tid, kThr, BlockSync{});
+    float2 *O = (C == fa) ? fb : fa;
+    const float c0 = P.sh_csc;
+    for (int k = tid; k < Lg; k += kThr) {
+        const float2 Ck = C[k], Cm = C[k == 0 ? 0 : Lg - k];
+        const float2 Z = make_float2(0.5f * (Ck.x + Cm.x), 0.5f * (Ck.y - Cm.y));   // F(z)
+        const float2 U = make_float2(0.5f * (Ck.y + Cm.y), -0.5f * (Ck.x - Cm.x));  // F(u)
+        const float ss = gp(P.sS)[k];
+        const float den = c0 * gp(P.cS)[k];
+        const float2 hh = make_float2((c0 * U.x - Z.x) / den, (c0 * U.y - Z.y) / den);
+        const float2 dx = make_float2(Z.x * inv_c0 + ss * hh.x, Z.y * inv_c0 + ss * hh.y);
+        O[k] = make_float2(hh.x - dx.y, hh.y + dx.x);   // h_hat + i DX_hat
+    }
+    __syncthreads();
+    float2 *Rr = team_fft<true>(O, C, P.plan, tw, tid, kThr, BlockSync{});
+    const float scale = 1.0f / (float)Lg;
+    const float rho = P.rho;
+    const bool last = it + 1 == iters;
+    for (int l = tid; l < Lg; l += kThr) {
+        const float hn = Rr[l].x * scale;
+        const float DXn = Rr[l].y * scale;
+        const bool win = (l >= P.gamma) && (l < P.gamma + P.L4);
+        const float wl = win ? 1.f : 0.f;
+        const float y1n = y1s[l];
+        const float Y1n = (bs[l] + rho * (DXn + y1n)) / (wl + rho);   // update_Y1! (csc_kernels.jl:91)
+        for (int t = l; t < vl; t += Lg) hb[csctz_pad(t)] = hn;      // h and its wrapped tail
+        DXs[l] = DXn;
+        Y1s[l] = Y1n;
+        if (last) {
+            gp(P.h)[ro + l] = hn;
+            gp(P.DX)[ro + l] = DXn;
+            gp(P.y1)[ro + l] = y1n;
+            gp(P.Y1)[ro + l] = Y1n;
+        }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------- best / objective
 // Per-sequence partials of objective() (CDL_setup.jl:286): sum (DX_window - s)^2, sum |X|,
 // and the positive-X statistics of update_best!/wrap.jl:12; writes the candidate

@@ -270,13 +270,18 @@ cdlm_status cdlm_server_stats(cdlm_server *srv, cdlm_server_stats_t *out);
  * 16: one sequence per CU; with 8 and 16 the resident sequences' state stays in the
  * last-level cache between the CSC iterations of one launch; 0 = 8 where two work-groups'
  * LDS fits a CU, else 16 where one fits, else 4;
- * it changes the order of the sum over waves, hence rounding). */
+ * it changes the order of the sum over waves, hence rounding);
+ * csc_mfma: CSC kernel (1: the filter correlation and convolution on the vector ALUs, as
+ * above; 2: on the matrix cores as Toeplitz x Hankel products of 16 x 256-position blocks, one
+ * column per wave, csc_waves 8 (two work-groups per CU where their LDS fits) or 16, csc_slots
+ * unused; 0 = the default for the shape; the choice changes rounding). */
 typedef struct {
     int32_t fft_order;
     int32_t fft4_order;
     int32_t dict_batch;
     int32_t csc_slots;
     int32_t csc_waves;
+    int32_t csc_mfma;
 } cdlm_tuning;
 /* Times every candidate on a scratch problem of n_seq sequences (<= 0: six per CU),
  * each in three interleaved rounds (its fastest round counts), keeps the fastest for 15 CSC + 1

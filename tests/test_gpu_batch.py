@@ -142,9 +142,11 @@ def test_tuned_launch_choices_keep_parity():
     o.update_best()
     o.dict_update(1)
     o.code_update(3)
-    choices = [dict(fft_order=order, fft4_order=order4, dict_batch=B, csc_slots=8, csc_waves=4)
+    choices = [dict(fft_order=order, fft4_order=order4, dict_batch=B, csc_slots=8, csc_waves=4, csc_mfma=1)
                for order in (0, 1) for order4 in (0, 1) for B in (2, 4, 8)]
-    choices += [dict(fft_order=0, fft4_order=0, dict_batch=2, csc_slots=4, csc_waves=wv) for wv in (4, 8, 16)]
+    choices += [dict(fft_order=0, fft4_order=0, dict_batch=2, csc_slots=4, csc_waves=wv, csc_mfma=1) for wv in (4, 8, 16)]
+    choices += [dict(fft_order=o, fft4_order=0, dict_batch=4, csc_slots=0, csc_waves=wv, csc_mfma=2)
+                for o in (0, 1) for wv in (8, 16)]
     try:
         for ch in choices:
             cdlm.tuning_set(L, M, 20, ch)

@@ -204,16 +204,18 @@ def test_csc_is_per_sequence_exact():
         np.testing.assert_array_equal(a.get(f)[5:9], b.get(f), err_msg=f)
 
 
-@pytest.mark.parametrize("waves", [4, 8, 16])
-def test_csc_launch_split_is_exact(waves):
+@pytest.mark.parametrize("mfma,waves", [(1, 4), (1, 8), (1, 16), (2, 8), (2, 16)])
+def test_csc_launch_split_is_exact(mfma, waves):
     """One launch of 12 CSC iterations carries the state between its iterations as
     (X + y2, X + y3) and rebuilds Xt, y2, y3 from it; 12 launches of one iteration each store
     (Xt, y2, y3) after every iteration.  Both run the same float operations, so every field
-    must agree bit for bit, for each CSC work-group shape, and so must the next dictionary step."""
+    must agree bit for bit, for each CSC kernel and work-group shape, and so must the next
+    dictionary step."""
     fl = [20] * 12
     s, D0, X0, _ = _synth(6, 24, 200, 12, fl)
     L, M = 200, len(fl)
-    cdlm.tuning_set(L, M, 20, {"fft_order": 0, "fft4_order": 0, "dict_batch": 2, "csc_slots": 4, "csc_waves": waves})
+    cdlm.tuning_set(L, M, 20, {"fft_order": 0, "fft4_order": 0, "dict_batch": 2, "csc_slots": 4, "csc_waves": waves,
+                               "csc_mfma": mfma})
     try:
         a = cdlm.CDL(s, fl, 1.0, 100.0, 0.05, D0=D0, X0=X0)
         b = cdlm.CDL(s, fl, 1.0, 100.0, 0.05, D0=D0, X0=X0)
@@ -284,17 +286,26 @@ def test_x_materialisation_paths_agree():
         np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
 
 
+@pytest.mark.parametrize("mfma", [0, 1, 2])
 @pytest.mark.parametrize("shape", [(1, 2, 1, [4]), (3, 1, 2, [4, 4]), (2, 5, 3, [8, 4, 12]), (5, 3, 17, [8] * 17),
-                                   (2, 252, 2, [8, 8]), (3, 260, 3, [8, 12, 8]), (1, 502, 2, [20, 20])])
-def test_tiny_shapes_vs_oracle(shape):
+                                   (2, 252, 2, [8, 8]), (3, 260, 3, [8, 12, 8]), (1, 502, 2, [20, 20]),
+                                   (2, 13, 40, [4] * 39 + [32])])
+def test_tiny_shapes_vs_oracle(shape, mfma):
     """Edge shapes: one sequence, one base, one filter, fewer filters than the CSC kernel has
-    waves (M < 16: waves without columns), an odd filter count across the waves; padded
-    lengths 1024 (the first with more than 256 float4 slots of half spectrum, the dictionary
-    kernel's wider variant), 1064 and 2048 (the largest); short schedule, every launch choice
-    at its default."""
+    waves (waves without columns), an odd filter count across the waves; padded lengths 1024
+    (the first with more than 256 float4 slots of half spectrum, the dictionary kernel's wider
+    variant, and the last with four 256-position blocks per column in the matrix-core CSC
+    kernel), 1064 and 2048 (the largest), 116 with 32 taps (blocks mostly past the padded
+    length); short schedule; each CSC kernel (1: vector ALUs, 2: matrix cores) and the default
+    choice (0)."""
     N, L, M, fl = shape
     s, D0, X0, _ = _synth(40 + N + M, N, L, M, fl)
-    g, o = _both(s, D0, X0, fl, 1.0, 100.0, 0.05)
+    if mfma:
+        cdlm.tuning_set(L, M, max(fl), {"csc_mfma": mfma})
+    try:
+        g, o = _both(s, D0, X0, fl, 1.0, 100.0, 0.05)
+    finally:
+        cdlm.tuning_set(L, M, max(fl), None)
     for _ in range(2):
         g.code_update(4); o.code_update(4)
         ob_g, imp_g = g.update_best(); ob_o, imp_o = o.update_best()
