@@ -47,6 +47,8 @@
 // support samples are identically zero and only l % 4 != 0 is kept, 3 floats per slot.
 #pragma once
 
+#include <type_traits>
+
 #include "cdlm_internal.h"
 
 namespace {
@@ -203,6 +205,10 @@ __device__ __forceinline__ void conv_acc4(const float4 *vb4, int s, const float 
         float4 t = vb4[s + u];
         vw[4 * u] = t.x; vw[4 * u + 1] = t.y; vw[4 * u + 2] = t.z; vw[4 * u + 3] = t.w;
     }
+    // vw[0] takes no part in the sums; keeping it live keeps the loads whole 16-byte LDS reads
+    // (without it the compiler reads the remaining 23 words as unaligned 4-byte pairs, whose
+    // 16-byte lane stride puts 4-8 lanes on one bank)
+    asm volatile("" ::"v"(vw[0]));
     float4 z = first ? make_float4(0.f, 0.f, 0.f, 0.f) : zacc4[s];
     float o[4] = {z.x, z.y, z.z, z.w};
 #pragma unroll
@@ -464,10 +470,12 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
     };
     if (kPf && w < M) load_col(w, nx, n2, n3, false);
     const int nw = M < WAVES ? M : WAVES;       // waves that own columns (and z partials)
-#pragma nounroll
-  for (int it = 0; it < iters; ++it) {
-    const bool rdU = it > 0, wrU = it + 1 < iters;   // (u2, u3) read / written instead of (Xt, y2, y3)
-    __syncthreads();
+    // The column pass of one iteration, compiled once per (rdU, wrU): (u2, u3) read / written
+    // instead of (Xt, y2, y3).  With the two flags as constants the slot code has no branches
+    // on them, so the compiler keeps every value in place instead of merging two paths with
+    // register copies.
+    auto columns = [&](auto rdU_t, auto wrU_t) {
+    constexpr bool rdU = decltype(rdU_t)::value, wrU = decltype(wrU_t)::value;
     for (int m = w; m < M; m += WAVES) {
         const int mm = __builtin_amdgcn_readfirstlane(m);
         const size_t col = ((size_t)n * M + mm) * Lg, col3 = col3_of(n, mm, M, nslot);
@@ -546,6 +554,18 @@ __global__ __launch_bounds__(64 * WAVES, (CscShape<WAVES, NSL, TAPS>::kWavesPerS
         }
         wave_lds_sync();
         if constexpr (!kPf) have_next = false;
+    }
+    };
+#pragma nounroll
+  for (int it = 0; it < iters; ++it) {
+    __syncthreads();
+    if (it == 0) {
+        if (iters == 1) columns(std::false_type{}, std::false_type{});
+        else columns(std::false_type{}, std::true_type{});
+    } else if (it + 1 < iters) {
+        columns(std::true_type{}, std::true_type{});
+    } else {
+        columns(std::true_type{}, std::false_type{});
     }
     // the next iteration's first column does not depend on what follows: load it now (all of
     // it with prefetch, its first two slots without)
