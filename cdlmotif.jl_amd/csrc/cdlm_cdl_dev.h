@@ -692,7 +692,7 @@ __global__ __launch_bounds__(64 * WAVES, (NBT > 4 && WAVES == 8 ? 2 : 4)) void k
     float *hb = sm + 2 * Lg;                    // word q = h[q mod Lg], q < vl (padded)
     float *gw = hb + vlp + 64 * w;              // this wave's filter: gw[16 + j] = g[j], zero margins
     float *vbase = hb + vlp + 64 * WAVES;
-    float *vb = vbase + w * vlp;                // this wave's v column: word q = v[q - TAPS + 1] (padded)
+    float *vb = vbase + w * vlp;                // this wave's v column: word q = v[q - TAPS] (padded)
     float *zs = vbase + WAVES * vlp;            // nzs z slots of length Lg
     float *y1s = zs + nzs * Lg;                 // y1, Y1, DX and the data window, kept in LDS
     float *Y1s = y1s + Lg;
@@ -729,16 +729,14 @@ __global__ __launch_bounds__(64 * WAVES, (NBT > 4 && WAVES == 8 ? 2 : 4)) void k
             z = packed ? make_float4(0.f, 0.f, 0.f, 0.f) : ld3(y3p + col3_of(n, mc, M, nslot), s);
         }
     };
-#pragma nounroll
-  for (int it = 0; it < iters; ++it) {
-    const bool rdU = it > 0, wrU = it + 1 < iters;   // (u2, u3) read / written instead of (Xt, y2, y3)
-    __syncthreads();
-    // v past Lg is read only against zero Toeplitz entries; keep it finite (the region held
-    // FFT data or nothing)
-    for (int i = TAPS - 1 + Lg + lane; i < vl; i += 64) vb[csctz_pad(i)] = 0.f;
+    // V operand words 256b + 16c + 4ks + a + 1, padded: 320b + 20c + a + 1 + 4ks + 4(ks/4), plus
+    // 4 where the word crosses into the next row of 16 (a = 3 and ks = 3 mod 4)
+    const int vbase0 = 20 * c + a + 1, vbase3 = vbase0 + (a == 3 ? 4 : 0);
+    // The column pass of one iteration, compiled once per (rdU, wrU): (u2, u3) read / written
+    // instead of (Xt, y2, y3); with the flags constant the block code has no branches on them.
     floatx4 zacc[NBT];
-#pragma unroll
-    for (int b = 0; b < NBT; ++b) zacc[b] = floatx4{0.f, 0.f, 0.f, 0.f};
+    auto columns = [&](auto rdU_t, auto wrU_t) {
+    constexpr bool rdU = decltype(rdU_t)::value, wrU = decltype(wrU_t)::value;
     for (int m = w; m < M; m += WAVES) {
         const int mm = __builtin_amdgcn_readfirstlane(m);
         const size_t col = ((size_t)n * M + mm) * Lg, col3 = col3_of(n, mm, M, nslot);
@@ -792,26 +790,42 @@ __global__ __launch_bounds__(64 * WAVES, (NBT > 4 && WAVES == 8 ? 2 : 4)) void k
                         st3(y3p + col3, s, y3n);
                         st4(Xtp + col, s, mk4(xn));
                     }
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) vb[csctz_pad(4 * s + q + TAPS - 1)] = v[q];
+                    *reinterpret_cast<float4 *>(vb + csctz_pad(4 * s + TAPS)) = mk4(v);
                 }
-
             }
         }
         wave_lds_sync();
-        if (lane < TAPS - 1) vb[csctz_pad(lane)] = vb[csctz_pad(lane + Lg)];   // wrapped head, v[lane - TAPS + 1]
+        if (lane < TAPS) vb[csctz_pad(lane)] = vb[csctz_pad(lane + Lg)];   // wrapped head, v[lane - TAPS]
         wave_lds_sync();
         // z-blocks += T' V
 #pragma unroll
         for (int b = 0; b < NBT; ++b) {
             if (b < nb) {
-                const float *vp = vb + 320 * b + 20 * c + a;      // word 256b + 16c + 4ks + a
+                const float *vp0 = vb + 320 * b + vbase0, *vp3 = vb + 320 * b + vbase3;
 #pragma unroll
                 for (int ks = 0; ks < kKS; ++ks)
-                    zacc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(tc[ks], vp[4 * ks + 4 * (ks >> 2)], zacc[b], 0, 0, 0);
+                    zacc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                        tc[ks], ((ks & 3) == 3 ? vp3 : vp0)[4 * ks + 4 * (ks >> 2)], zacc[b], 0, 0, 0);
             }
         }
         wave_lds_sync();   // the next column rewrites gw and vb
+    }
+    };
+#pragma nounroll
+  for (int it = 0; it < iters; ++it) {
+    __syncthreads();
+    // v past Lg is read only against zero Toeplitz entries; keep it finite (the region held
+    // FFT data or nothing)
+    for (int i = TAPS + Lg + lane; i < vl; i += 64) vb[csctz_pad(i)] = 0.f;
+#pragma unroll
+    for (int b = 0; b < NBT; ++b) zacc[b] = floatx4{0.f, 0.f, 0.f, 0.f};
+    if (it == 0) {
+        if (iters == 1) columns(std::false_type{}, std::false_type{});
+        else columns(std::false_type{}, std::true_type{});
+    } else if (it + 1 < iters) {
+        columns(std::true_type{}, std::true_type{});
+    } else {
+        columns(std::true_type{}, std::false_type{});
     }
     // z slots: waves 0 .. nzs-1 store their partial, later waves add theirs in rounds of nzs
 #pragma unroll
