@@ -46,7 +46,8 @@ class cdlm_tuning(C.Structure):
 
 class cdlm_server_opts(C.Structure):
     _fields_ = [("mem_budget", C.c_uint64), ("max_active", C.c_int32), ("csc_iter", C.c_int32), ("cdl_iter", C.c_int32),
-                ("use_graph", C.c_int32), ("fill_seqs", C.c_int32), ("policy", C.c_int32)]
+                ("use_graph", C.c_int32), ("fill_seqs", C.c_int32), ("policy", C.c_int32),
+                ("lockstep", C.c_int32)]
 
 
 class cdlm_job_spec(C.Structure):

@@ -3,8 +3,9 @@
 The reference runs one motif-discovery dataset at a time (find_motifs_fasta ->
 try_to_find_motif -> cdl_optimize, src/WRAPPER/wrap.jl:4-12,140-149).  A Server keeps
 many CDL training jobs resident on one device: jobs are admitted by device footprint
-as memory frees up, and every tick advances all resident jobs by one outer iteration
-of the reference schedule in one batch.
+as memory frees up and run concurrently, each at its own pace (or, with lockstep=True,
+every tick advances all resident jobs by one outer iteration of the reference schedule in
+one batch).
 """
 from __future__ import annotations
 
@@ -21,12 +22,17 @@ QUEUED, RUNNING, DONE, FAILED = range(4)
 
 class Server:
     def __init__(self, device: int = 0, mem_budget: int = 0, max_active: int = 0, csc_iter: int = 15,
-                 cdl_iter: int = 1, graph: bool = True, fill_seqs: int = 0, policy: str = "fifo"):
+                 cdl_iter: int = 1, graph: bool = True, fill_seqs: int = 0, policy: str = "fifo",
+                 lockstep: bool = False):
         """fill_seqs: admit no job while the resident jobs hold this many sequences (0: twelve
         per CU, < 0: no limit besides memory).  policy: "fifo" (arrival order) or "shortest"
-        (least work first); a job passed by 4 later arrivals is admitted first under either."""
+        (least work first); a job passed by 4 later arrivals is admitted first under either.
+        lockstep: advance all resident jobs together, one batched outer iteration per tick
+        (graph: replay the ticks from captured HIP graphs); else (default) up to four jobs, or
+        max_active, run concurrently on their own streams, each at its own pace."""
         pol = {"fifo": 0, "shortest": 1}[policy]
-        o = cdlm_server_opts(mem_budget, max_active, csc_iter, cdl_iter, 1 if graph else 0, fill_seqs, pol)
+        o = cdlm_server_opts(mem_budget, max_active, csc_iter, cdl_iter, 1 if graph else 0, fill_seqs, pol,
+                             1 if lockstep else 0)
         h = C.c_void_p()
         st = load().cdlm_server_create(device, C.byref(o), C.byref(h))
         if st != 0:

@@ -1,8 +1,14 @@
 // cdlm_serve.hip -- continuous-batching server of CDL training jobs (host code over the C ABI).
 //
 // The reference trains datasets one at a time from the host (find_motifs_fasta ->
-// try_to_find_motif -> cdl_optimize, src/WRAPPER/wrap.jl:4-12,140-149).  Here a server
-// thread keeps many jobs resident on one device and advances all of them together:
+// try_to_find_motif -> cdl_optimize, src/WRAPPER/wrap.jl:4-12,140-149).  Here a server keeps
+// many jobs resident on one device.  By default they run concurrently, each at its own pace:
+// a dispatcher thread admits jobs (as below) and hands each to one of a few worker threads,
+// each with its own context and stream, which creates the job's problem in its arena and runs
+// its outer iterations back to back; the device interleaves the work-groups of the concurrent
+// jobs' kernels, so a small job beside a large one is not held to the large one's iteration
+// time, and a job's upload and initialisation overlap the others' iterations.  In lockstep
+// mode (opts.lockstep) one server thread advances all resident jobs together instead:
 //   * submit copies a job's data to host memory and queues it (arrival time recorded);
 //     a job whose device footprint exceeds the whole budget is streamed: admitted when nothing
 //     else is resident, it runs alone as a problem whose state lives in pinned host memory and
@@ -105,9 +111,16 @@ struct cdlm_server {
     std::vector<double> latencies;    // completed jobs, seconds
     int64_t ticks = 0;
     std::string last_error;
-    // server thread only
+    // server thread only (lockstep)
     cdlm_batch *batch = nullptr;
     bool membership_changed = true;
+    // concurrent mode: workers, jobs handed to them, idle workers, and a counter of events that
+    // can make another admission possible (a submission, a job leaving the device)
+    std::vector<std::thread> workers;
+    std::deque<Job *> ready;
+    std::condition_variable cv_ready;
+    int idle = 0;
+    uint64_t gen = 0;
 };
 
 namespace {
@@ -137,6 +150,7 @@ void fail_job(cdlm_server *srv, Job *j, const std::string &why) {
 std::vector<Job *> pick_admissions(cdlm_server *srv, std::vector<void *> *to_free) {
     std::vector<Job *> out;
     int slots = srv->opts.max_active > 0 ? srv->opts.max_active - (int)srv->active.size() : 1 << 30;
+    if (!srv->opts.lockstep) slots = std::min(slots, srv->idle);   // one job per idle worker
     long seqs = 0;
     for (Job *j : srv->active) seqs += j->N;
     // candidate order: jobs passed kMaxBypass times first (arrival order), then by policy
@@ -234,7 +248,7 @@ void return_arena(cdlm_server *srv, Job *j, bool reuse) {
     if (a) (void)hipFree(a);
 }
 
-void finish_job(cdlm_server *srv, Job *j) {
+void finish_job(cdlm_server *srv, Job *j, cdlm_ctx *pctx = nullptr) {
     double th = 0.0;
     int64_t cnt = 0;
     float ob = 0.f;
@@ -245,7 +259,11 @@ void finish_job(cdlm_server *srv, Job *j) {
         codes.resize((size_t)(j->L4 / 4) * j->prm.M * j->N);
         st = cdlm_get(j->p, CDLM_F_XBEST_SLICE, codes.data(), codes.size() * sizeof(float));
     }
-    const std::string err = st == CDLM_OK ? std::string() : ctx_error(srv);
+    std::string err;
+    if (st != CDLM_OK) {
+        const char *e = cdlm_last_error(pctx ? pctx : srv->ctx);
+        err = e ? e : "";
+    }
     cdlm_problem_destroy(j->p);   // the arena is the server's: it goes back to the pool
     j->p = nullptr;
     return_arena(srv, j, st == CDLM_OK && !j->streamed);
@@ -446,6 +464,131 @@ void server_loop(cdlm_server *srv) {
     srv->cv_done.notify_all();
 }
 
+// ---- concurrent mode
+// One job on a worker's context: its problem in its arena (allocated here when not recycled),
+// its outer iterations one call each (the server may stop between two), then its results.
+void run_job(cdlm_server *srv, cdlm_ctx *ctx, Job *j) {
+    auto fail = [&](const std::string &why, bool reuse) {
+        if (j->p) cdlm_problem_destroy(j->p);
+        j->p = nullptr;
+        return_arena(srv, j, reuse);
+        fail_job(srv, j, why);
+    };
+    auto stopping = [&] {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        return srv->stop;
+    };
+    if (!ctx) return fail("no device context for the job", false);
+    if (stopping()) return fail("server stopped", !j->streamed);
+    cdlm_problem *p = nullptr;
+    cdlm_status st;
+    if (j->streamed) {
+        st = cdlm_problem_create_streamed(ctx, &j->prm, j->s.data(), j->L4, j->N, j->D0.empty() ? nullptr : j->D0.data(),
+                                          nullptr, j->seed, j->arena_bytes, &p);
+    } else {
+        if (!j->arena && hipMalloc(&j->arena, j->arena_bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            j->arena = nullptr;
+            return fail("no device memory for the job's arena", false);
+        }
+        st = cdlm_problem_create_in(ctx, &j->prm, j->s.data(), j->L4, j->N, j->D0.empty() ? nullptr : j->D0.data(),
+                                    j->seed, j->arena, j->arena_bytes, &p);
+    }
+    if (st != CDLM_OK) {
+        const char *e = cdlm_last_error(ctx);
+        return fail(e ? e : "problem creation failed", !j->streamed);
+    }
+    int32_t Lg = 0, gamma = 0;
+    cdlm_problem_dims(p, &Lg, &gamma, nullptr, nullptr, nullptr);
+    j->p = p;
+    j->Lg = Lg;
+    j->gamma = gamma;
+    for (int it = 0; it < j->iter_num; ++it) {
+        if (stopping()) return fail("server stopped", false);
+        double tr = 0.0;
+        if (cdlm_optimize(p, 1, srv->opts.csc_iter, srv->opts.cdl_iter, &tr) != CDLM_OK) {
+            const char *e = cdlm_last_error(ctx);
+            return fail(e ? e : "device error", false);
+        }
+        j->trace.push_back(tr);
+        std::lock_guard<std::mutex> lk(srv->mu);
+        ++srv->ticks;
+    }
+    finish_job(srv, j, ctx);
+}
+
+void worker_loop(cdlm_server *srv) {
+    cdlm_ctx *ctx = nullptr;
+    if (cdlm_create(&ctx, srv->ctx->device) != CDLM_OK) ctx = nullptr;   // its jobs then fail
+    for (;;) {
+        Job *j = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(srv->mu);
+            srv->cv_ready.wait(lk, [&] { return srv->stop || !srv->ready.empty(); });
+            if (srv->ready.empty()) break;   // stopping, nothing handed over
+            j = srv->ready.front();
+            srv->ready.pop_front();
+        }
+        run_job(srv, ctx, j);
+        {
+            std::lock_guard<std::mutex> lk(srv->mu);
+            srv->active.erase(std::remove(srv->active.begin(), srv->active.end(), j), srv->active.end());
+            ++srv->idle;
+            ++srv->gen;
+        }
+        srv->cv_work.notify_all();
+    }
+    if (ctx) cdlm_destroy(ctx);
+}
+
+// Admits queued jobs whenever a worker is idle and a submission or a job leaving the device may
+// have made room; the workers do the rest.
+void dispatch_loop(cdlm_server *srv) {
+    (void)hipSetDevice(srv->ctx->device);   // pooled arenas are freed from this thread
+    uint64_t seen = ~0ull;
+    for (;;) {
+        std::vector<Job *> adm;
+        std::vector<void *> to_free;
+        {
+            std::unique_lock<std::mutex> lk(srv->mu);
+            srv->cv_work.wait(lk, [&] { return srv->stop || (!srv->queue.empty() && srv->idle > 0 && srv->gen != seen); });
+            if (srv->stop) break;
+            seen = srv->gen;
+            adm = pick_admissions(srv, &to_free);
+            for (Job *j : adm) {
+                j->t_admit = Clock::now();
+                srv->active.push_back(j);
+                --srv->idle;
+            }
+        }
+        for (void *a : to_free) (void)hipFree(a);   // before the workers allocate in their place
+        if (!adm.empty()) {
+            std::lock_guard<std::mutex> lk(srv->mu);
+            for (Job *j : adm) srv->ready.push_back(j);
+            ++srv->gen;   // more may fit once these are placed
+        }
+        srv->cv_ready.notify_all();
+    }
+    srv->cv_ready.notify_all();
+    for (std::thread &t : srv->workers)
+        if (t.joinable()) t.join();
+    std::vector<std::pair<void *, size_t>> pool;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        pool.swap(srv->pool);
+        srv->pool_bytes = 0;
+    }
+    for (auto &a : pool) (void)hipFree(a.first);
+    std::lock_guard<std::mutex> lk(srv->mu);
+    for (uint64_t id : srv->queue) {
+        Job *j = srv->jobs[id].get();
+        j->state = CDLM_JOB_FAILED;
+        j->error = "server stopped";
+    }
+    srv->queue.clear();
+    srv->cv_done.notify_all();
+}
+
 double quantile(std::vector<double> v, double q) {
     if (v.empty()) return 0.0;
     std::sort(v.begin(), v.end());
@@ -486,7 +629,15 @@ extern "C" cdlm_status cdlm_server_create(int32_t device, const cdlm_server_opts
         srv->fill = 0;   // no limit
     }
     cdlm_server *raw = srv.get();
-    raw->th = std::thread(server_loop, raw);
+    if (raw->opts.lockstep) {
+        raw->th = std::thread(server_loop, raw);
+    } else {
+        // four workers (the device's hardware queues per process), or max_active when smaller
+        const int nw = raw->opts.max_active > 0 ? std::min(raw->opts.max_active, 4) : 4;
+        raw->idle = nw;
+        for (int i = 0; i < nw; ++i) raw->workers.emplace_back(worker_loop, raw);
+        raw->th = std::thread(dispatch_loop, raw);
+    }
     *out = srv.release();
     return CDLM_OK;
 }
@@ -498,7 +649,8 @@ extern "C" cdlm_status cdlm_server_destroy(cdlm_server *srv) {
         srv->stop = true;
     }
     srv->cv_work.notify_all();
-    if (srv->th.joinable()) srv->th.join();
+    srv->cv_ready.notify_all();
+    if (srv->th.joinable()) srv->th.join();   // the dispatcher joins the workers
     cdlm_destroy(srv->ctx);
     delete srv;
     return CDLM_OK;
@@ -561,6 +713,7 @@ extern "C" cdlm_status cdlm_server_submit(cdlm_server *srv, const cdlm_job_spec 
     *id = j->id;
     srv->queue.push_back(j->id);
     srv->jobs[j->id] = std::move(j);
+    ++srv->gen;
     srv->cv_work.notify_all();
     return CDLM_OK;
 }

@@ -186,12 +186,14 @@ cdlm_status cdlm_batch_timing_read(cdlm_batch *b, int32_t kernel, double *total_
 /* ---- serving --------------------------------------------------------------------------- */
 /* A continuous-batching server of CDL training jobs on one device (the reference runs one
  * dataset at a time from find_motifs_fasta -> cdl_optimize, src/WRAPPER/wrap.jl:4-12,140-149).
- * The server owns its own context and a host thread.  Jobs are admitted in arrival order
+ * The server owns its own context and host threads.  Jobs are admitted in arrival order
  * (smaller later jobs may pass a waiting one) while their device footprint
  * (cdlm_footprint_bytes) fits the memory budget and the resident jobs do not yet fill the
- * device (fill_seqs sequences); every tick advances all resident jobs by
- * one outer iteration (csc_iter CSC iterations, update_best, cdl_iter dictionary
- * iterations) in one batch; a finished job's results move to host memory and its device
+ * device (fill_seqs sequences).  By default resident jobs run concurrently, each on a worker
+ * thread with its own stream, one outer iteration (csc_iter CSC iterations, update_best,
+ * cdl_iter dictionary iterations) after another at its own pace, up to four at a time (or
+ * max_active); with lockstep set, every tick advances all resident jobs by one outer iteration
+ * in one batch instead.  A finished job's results move to host memory and its device
  * memory (one arena per job) is kept for later admissions, freed only when a new arena must be
  * allocated within the budget.  A job's results do not depend on the jobs it shares the device
  * with.  The calls below are thread-safe. */
@@ -201,12 +203,14 @@ typedef struct {
     int32_t max_active;    /* resident jobs at most (0: no limit besides memory) */
     int32_t csc_iter;      /* CSC iterations per outer iteration, all jobs (0: 15, WRAPPER_setup.jl:8) */
     int32_t cdl_iter;      /* dictionary iterations per outer iteration (0: 1, WRAPPER_setup.jl:9) */
-    int32_t use_graph;     /* replay ticks from captured HIP graphs */
+    int32_t use_graph;     /* lockstep: replay ticks from captured HIP graphs */
     int32_t fill_seqs;     /* admit no job while resident jobs hold this many sequences
                             * (0: twelve per CU; < 0: no limit) */
     int32_t policy;        /* admission order: CDLM_POLICY_FIFO or CDLM_POLICY_SHORTEST (least work
                             * iter_num*N*Lg*M first); a job passed by 4 later arrivals goes first
                             * either way */
+    int32_t lockstep;      /* 1: all resident jobs advance together, one batched outer iteration
+                            * per tick; 0: concurrent jobs, each at its own pace */
 } cdlm_server_opts;
 enum { CDLM_POLICY_FIFO = 0, CDLM_POLICY_SHORTEST = 1 };
 typedef struct {
@@ -231,7 +235,7 @@ typedef struct {
     int32_t iter_num, L, M, N;
 } cdlm_job_result;
 typedef struct {
-    int64_t completed, active, queued, ticks;
+    int64_t completed, active, queued, ticks;   /* ticks: lockstep ticks, or outer iterations run */
     double p50_s, p99_s;   /* latency quantiles (nearest rank) of the completed jobs */
     uint64_t bytes_in_use, budget_bytes;
 } cdlm_server_stats_t;

@@ -1,8 +1,8 @@
-"""Served CDL: jobs admitted by device footprint while others run, advanced together one
-outer iteration per tick.  Each job's result (objective trace, code threshold, positive
-count, X_best slice) must be bit-identical to the same training run alone through the
-problem API, whatever else shares the device (the reference trains one dataset per
-cdl_optimize call, src/WRAPPER/wrap.jl:4-12)."""
+"""Served CDL: jobs admitted by device footprint while others run, concurrently at their own
+pace or advanced together one outer iteration per tick (lockstep).  Each job's result
+(objective trace, code threshold, positive count, X_best slice) must be bit-identical to the
+same training run alone through the problem API, whatever else shares the device (the
+reference trains one dataset per cdl_optimize call, src/WRAPPER/wrap.jl:4-12)."""
 import threading
 import time
 
@@ -50,10 +50,11 @@ def _same(r, ref, tag):
     assert np.array_equal(r["codes"], codes), tag
 
 
-def test_served_jobs_equal_alone_in_the_mix():
+@pytest.mark.parametrize("lockstep", [False, True])
+def test_served_jobs_equal_alone_in_the_mix(lockstep):
     refs = [_alone(j) for j in JOBS]
     # alone through the server
-    srv = cdlm.Server(csc_iter=CSC, cdl_iter=CDL_IT)
+    srv = cdlm.Server(csc_iter=CSC, cdl_iter=CDL_IT, lockstep=lockstep)
     sd, N, L, fl, it = JOBS[0]
     jid = srv.submit(_data(sd, N, L), fl, RHO, SIGMA, LAM, it, seed=sd, keep_codes=True)
     assert srv.wait(jid, 120) == cdlm.serve.DONE
@@ -76,13 +77,14 @@ def test_served_jobs_equal_alone_in_the_mix():
     srv.close()
 
 
-def test_admission_by_footprint():
+@pytest.mark.parametrize("lockstep", [False, True])
+def test_admission_by_footprint(lockstep):
     """A budget that holds only the largest job: jobs queue, run as memory frees up, never
     exceed the budget, and still give their alone results; a job whose smallest streamed window
     (77 MB) is larger than the budget is refused at submission."""
     sizes = [cdlm.footprint_bytes(L, N, len(fl), max(fl)) for _, N, L, fl, _ in JOBS]
     budget = max(sizes) + min(sizes) // 2
-    srv = cdlm.Server(mem_budget=budget, csc_iter=CSC, cdl_iter=CDL_IT)
+    srv = cdlm.Server(mem_budget=budget, csc_iter=CSC, cdl_iter=CDL_IT, lockstep=lockstep)
     seen = []
     stop = threading.Event()
 
@@ -109,19 +111,21 @@ def test_admission_by_footprint():
     srv.close()
 
 
-def test_destroy_with_pending_jobs_fails_them_cleanly():
-    srv = cdlm.Server(csc_iter=CSC, cdl_iter=CDL_IT, max_active=1)
+@pytest.mark.parametrize("lockstep", [False, True])
+def test_destroy_with_pending_jobs_fails_them_cleanly(lockstep):
+    srv = cdlm.Server(csc_iter=CSC, cdl_iter=CDL_IT, max_active=1, lockstep=lockstep)
     ids = [srv.submit(_data(sd, N, L), fl, RHO, SIGMA, LAM, 50, seed=sd) for sd, N, L, fl, _ in JOBS[:3]]
     time.sleep(0.05)
     srv.close()   # must not hang; queued and resident jobs end as failed
     assert ids
 
 
+@pytest.mark.parametrize("lockstep", [False, True])
 @pytest.mark.parametrize("policy", ["fifo", "shortest"])
-def test_admission_order_policy(policy):
+def test_admission_order_policy(policy, lockstep):
     """One resident job at a time: a long job runs while a big and then a small job queue.
     FIFO admits the big one first; shortest-work-first admits the small one first."""
-    srv = cdlm.Server(csc_iter=CSC, cdl_iter=CDL_IT, max_active=1, policy=policy)
+    srv = cdlm.Server(csc_iter=CSC, cdl_iter=CDL_IT, max_active=1, policy=policy, lockstep=lockstep)
     a = srv.submit(_data(301, 60, 200), [16] * 8, RHO, SIGMA, LAM, 400, seed=1)
     time.sleep(0.01)
     db, ds = _data(302, 60, 200), _data(303, 10, 20)
