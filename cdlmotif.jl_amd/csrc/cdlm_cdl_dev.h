@@ -751,47 +751,62 @@ __global__ __launch_bounds__(64 * WAVES, (NBT > 4 && WAVES == 8 ? 2 : 4)) void k
             tr[ks] = gw[16 + 4 * ks + a - c];
             tc[ks] = gw[16 + c - 4 * ks - a + TAPS - 1];
         }
-        // X, the shrink / dual updates and v, block by block
+        // X, the shrink / dual updates and v, two blocks at a time: both blocks' loads are issued
+        // first, then g * h of both (it does not depend on them: two independent MFMA chains per
+        // block, K-steps ks even / odd), then the elementwise updates; X = Xt + the chains' sum
+        auto update = [&](int b, float4 xv, float4 b2, float4 b3, const floatx4 &c0, const floatx4 &c1) {
+            const int s = 64 * b + 4 * c + a;
+            if (rdU) csc_unpack(xv, b2, b3, lam, inv_c0);
+            float x4[4], u2[4], y2n[4], y3n[4], v[4], xn[4];
 #pragma unroll
-        for (int b = 0; b < NBT; ++b) {
+            for (int q = 0; q < 4; ++q) {
+                const float x = f4(xv, q) + (c0[q] + c1[q]);
+                x4[q] = x;
+                const float t = x + f4(b2, q);                      // shrink! (csc_kernels.jl:58-59)
+                const float at = fabsf(t) - lam;
+                const float Y2 = at > 0.f ? copysignf(at, t) : 0.f;
+                const float Y3 = (q == 0) ? x : 0.f;                  // projectionX! (:73-77), y3 = 0 there
+                u2[q] = t;
+                y2n[q] = t - Y2;                                      // train.jl:13, (y2 + X) - Y2
+                y3n[q] = (q == 0) ? 0.f : f4(b3, q) + x;              // train.jl:14
+                v[q] = (Y2 - y2n[q]) + (Y3 - y3n[q]);
+                xn[q] = __fmul_rn(v[q], inv_c0);
+            }
+            if (s < nslot) {
+                if (wrU) {
+                    const float u3[4] = {x4[0], y3n[1], y3n[2], y3n[3]};
+                    st4(y2p + col, s, mk4(u2));
+                    st4(Xtp + col, s, mk4(u3));
+                } else {
+                    st4(y2p + col, s, mk4(y2n));
+                    st3(y3p + col3, s, y3n);
+                    st4(Xtp + col, s, mk4(xn));
+                }
+                *reinterpret_cast<float4 *>(vb + csctz_pad(4 * s + TAPS)) = mk4(v);
+            }
+        };
+        auto corr = [&](int b, floatx4 &c0, floatx4 &c1) {
+            // word 256b + 16c + 4ks + a: (4ks + a) / 16 = ks / 4
+            const float *hp = hb + 320 * b + 20 * c + a;
+            c0 = c1 = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < kKS; ++ks) {
+                if (ks & 1) c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(tr[ks], hp[4 * ks + 4 * (ks >> 2)], c1, 0, 0, 0);
+                else c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(tr[ks], hp[4 * ks + 4 * (ks >> 2)], c0, 0, 0, 0);
+            }
+        };
+#pragma unroll
+        for (int b = 0; b < NBT; b += 2) {
             if (b < nb) {
-                const int s = 64 * b + 4 * c + a;
-                const bool ok = s < nslot;
-                float4 xv = make_float4(0.f, 0.f, 0.f, 0.f), b2 = xv, b3 = xv;
-                load_blk(mm, b, rdU, xv, b2, b3);
-                if (rdU) csc_unpack(xv, b2, b3, lam, inv_c0);
-                floatx4 acc = {xv.x, xv.y, xv.z, xv.w};
-                // word 256b + 16c + 4ks + a: (4ks + a) / 16 = ks / 4
-                const float *hp = hb + 320 * b + 20 * c + a;
-#pragma unroll
-                for (int ks = 0; ks < kKS; ++ks)
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(tr[ks], hp[4 * ks + 4 * (ks >> 2)], acc, 0, 0, 0);
-                float u2[4], y2n[4], y3n[4], v[4], xn[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const float x = acc[q];
-                    const float t = x + f4(b2, q);                      // shrink! (csc_kernels.jl:58-59)
-                    const float at = fabsf(t) - lam;
-                    const float Y2 = at > 0.f ? copysignf(at, t) : 0.f;
-                    const float Y3 = (q == 0) ? x : 0.f;                  // projectionX! (:73-77), y3 = 0 there
-                    u2[q] = t;
-                    y2n[q] = t - Y2;                                      // train.jl:13, (y2 + X) - Y2
-                    y3n[q] = (q == 0) ? 0.f : f4(b3, q) + x;              // train.jl:14
-                    v[q] = (Y2 - y2n[q]) + (Y3 - y3n[q]);
-                    xn[q] = __fmul_rn(v[q], inv_c0);
-                }
-                if (ok) {
-                    if (wrU) {
-                        const float u3[4] = {acc[0], y3n[1], y3n[2], y3n[3]};
-                        st4(y2p + col, s, mk4(u2));
-                        st4(Xtp + col, s, mk4(u3));
-                    } else {
-                        st4(y2p + col, s, mk4(y2n));
-                        st3(y3p + col3, s, y3n);
-                        st4(Xtp + col, s, mk4(xn));
-                    }
-                    *reinterpret_cast<float4 *>(vb + csctz_pad(4 * s + TAPS)) = mk4(v);
-                }
+                const bool two = b + 1 < nb;
+                float4 xa = make_float4(0.f, 0.f, 0.f, 0.f), a2 = xa, a3 = xa, xb = xa, bb2 = xa, bb3 = xa;
+                load_blk(mm, b, rdU, xa, a2, a3);
+                if (two) load_blk(mm, b + 1, rdU, xb, bb2, bb3);
+                floatx4 ca0, ca1, cb0, cb1;
+                corr(b, ca0, ca1);
+                if (two) corr(b + 1, cb0, cb1);
+                update(b, xa, a2, a3, ca0, ca1);
+                if (two) update(b + 1, xb, bb2, bb3, cb0, cb1);
             }
         }
         wave_lds_sync();
