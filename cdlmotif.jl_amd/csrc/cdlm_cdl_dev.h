@@ -125,10 +125,15 @@ __device__ __forceinline__ int find_problem(const int *__restrict__ starts, int 
 // prove, so every access goes through gp(), which makes it a global-memory access
 // (global_load/global_store instead of flat instructions that also count against the
 // LDS wait counter).
+// Filter taps are read through the constant address space: the same word for every lane, and
+// no kernel that reads them writes them, so they come through the scalar cache into scalar
+// registers instead of taking vector-memory instructions beside the streamed state.
 #if defined(__HIP_DEVICE_COMPILE__)
 #define CDLM_GLOBAL __attribute__((address_space(1)))
+#define CDLM_CONST __attribute__((address_space(4)))
 #else
 #define CDLM_GLOBAL   // the host pass only parses device code
+#define CDLM_CONST
 #endif
 template <typename T>
 __device__ __forceinline__ CDLM_GLOBAL T *gp(T *p) { return (CDLM_GLOBAL T *)p; }
@@ -223,8 +228,14 @@ __device__ __forceinline__ void conv_acc4(const float4 *vb4, int s, const float 
 
 template <int TAPS>
 __device__ __forceinline__ void load_taps(const float *G, int m, int Lg, float *g) {
+    // the column's address made visibly uniform (it is: one problem per work-group, m per wave)
+    const uint64_t a = (uint64_t)(G + (size_t)m * Lg);
+    const uint64_t au = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32)) << 32) |
+                        (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
+    const CDLM_CONST float *Gc = (const CDLM_CONST float *)au;
+    const int nl = __builtin_amdgcn_readfirstlane(Lg);
 #pragma unroll
-    for (int j = 0; j < TAPS; ++j) g[j] = (j < Lg) ? gp(G)[(size_t)m * Lg + j] : 0.f;
+    for (int j = 0; j < TAPS; ++j) g[j] = (j < nl) ? Gc[j] : 0.f;
 }
 
 // h of sequence n into LDS with its circular tail: hb[Lg + t] = h[t mod Lg], t < TAPS + 4.
