@@ -235,11 +235,15 @@ struct cdlm_problem {
     // Streamed: the state of all N sequences lives in pinned host memory and the sequences move
     // through a device window of `win` sequences in nchunk chunks (0: a resident problem).
     struct HostArr {
-        char *dev = nullptr;    // the window's buffer
-        char *host = nullptr;   // N * per_seq bytes
+        char *dev[2] = {nullptr, nullptr};   // the windows' buffers (dev[1]: with two windows)
+        char *host = nullptr;                // N * per_seq bytes
         size_t per_seq = 0;
     };
-    int nchunk = 0, win = 0;
+    // With two windows, chunk k uses window k % 2: chunk k+1 moves in and chunk k-1 out (on two
+    // copy streams, the two directions of the bus at once) while chunk k's kernels run.
+    int nchunk = 0, win = 0, nwin = 1;
+    hipStream_t cin = nullptr, cout = nullptr;
+    std::vector<hipEvent_t> ev;        // per chunk: in, kernels, out
     std::vector<int> cn0;              // chunk k: sequences [cn0[k], cn0[k+1])
     std::vector<LaunchSet> csets;      // chunk k's launch set: its N, n_off and mean chunks
     std::vector<HostArr> hstate;       // moved in and out around every chunk pass
@@ -303,11 +307,16 @@ BufPlan buf_plan(int L4, int N, int M, int gamma, bool keep_x0, int Ntot = 0) {
 constexpr size_t kBufAlign = 256;
 size_t al(size_t x) { return (x + kBufAlign - 1) & ~(kBufAlign - 1); }
 
-// Arena bytes of a plan: every buffer rounded up to kBufAlign, in allocation order.
-size_t plan_total(const BufPlan &b) {
-    return al(b.s) + 2 * al(b.E) + al(b.Eh) + al(b.E3) + (b.Ex0 ? al(b.Ex0) : 0) + 7 * al(b.LN) + 7 * al(b.LM) +
-           6 * al(b.LMh) + 2 * al(b.Lg) /* cS, sS */ + al(2 * b.Lg) /* tw */ + 2 * al(b.xs) + al(b.y3s) + al(b.kap) +
-           al(b.M) + al(b.partials) + al(b.dpart) + al(b.trace) + al(b.sc);
+// Bytes of one window of per-sequence buffers (data, state, X_best slices).
+size_t plan_window(const BufPlan &b) {
+    return al(b.s) + 2 * al(b.E) + al(b.Eh) + al(b.E3) + (b.Ex0 ? al(b.Ex0) : 0) + 7 * al(b.LN) + 2 * al(b.xs) +
+           al(b.y3s) + al(b.kap);
+}
+// Arena bytes of a plan: every buffer rounded up to kBufAlign, in allocation order; a streamed
+// problem with two windows holds a second set of the per-sequence buffers.
+size_t plan_total(const BufPlan &b, int wins = 1) {
+    return (size_t)wins * plan_window(b) + 7 * al(b.LM) + 6 * al(b.LMh) + 2 * al(b.Lg) /* cS, sS */ +
+           al(2 * b.Lg) /* tw */ + al(b.M) + al(b.partials) + al(b.dpart) + al(b.trace) + al(b.sc);
 }
 
 cdlm_status palloc(cdlm_problem *p, void **out, size_t bytes) {
@@ -329,6 +338,13 @@ cdlm_status palloc_t(cdlm_problem *p, T **out, size_t bytes) {
 }
 
 void free_problem(cdlm_problem *p) {
+    if (p->cin) (void)hipStreamSynchronize(p->cin);
+    if (p->cout) (void)hipStreamSynchronize(p->cout);
+    for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
+    p->ev.clear();
+    if (p->cin) (void)hipStreamDestroy(p->cin);
+    if (p->cout) (void)hipStreamDestroy(p->cout);
+    p->cin = p->cout = nullptr;
     p->set.release();
     for (auto &cs : p->csets) cs.release();
     p->csets.clear();
@@ -718,12 +734,12 @@ namespace {
 // reductions see exactly what they see for a resident problem: a streamed training gives the
 // resident one's results bit for bit.  Chunks are unions of the dictionary-mean chunks.
 
-// Moves chunk k's state (and the data, and with x0 the initial codes) into the window.
-cdlm_status stream_in(cdlm_problem *p, int k, bool x0) {
-    hipStream_t st = p->ctx->stream;
+// Moves chunk k's state (and the data, and with x0 the initial codes) into its window, on stream st.
+cdlm_status stream_in(cdlm_problem *p, int k, bool x0, hipStream_t st) {
+    const int w = k % p->nwin;
     const size_t a = (size_t)p->cn0[k], n = (size_t)(p->cn0[k + 1] - p->cn0[k]);
     auto in = [&](const cdlm_problem::HostArr &h) {
-        return hipMemcpyAsync(h.dev, h.host + a * h.per_seq, n * h.per_seq, hipMemcpyHostToDevice, st);
+        return hipMemcpyAsync(h.dev[w], h.host + a * h.per_seq, n * h.per_seq, hipMemcpyHostToDevice, st);
     };
     hipError_t e = in(p->hs);
     for (size_t i = 0; i < p->hstate.size() && e == hipSuccess; ++i) e = in(p->hstate[i]);
@@ -732,12 +748,13 @@ cdlm_status stream_in(cdlm_problem *p, int k, bool x0) {
     return CDLM_OK;
 }
 
-// Moves chunk k's state back; cand >= 0: also the candidate X_best slice buffer cand.
-cdlm_status stream_out(cdlm_problem *p, int k, int cand) {
-    hipStream_t st = p->ctx->stream;
+// Moves chunk k's state back from its window, on stream st; cand >= 0: also the candidate X_best
+// slice buffer cand.
+cdlm_status stream_out(cdlm_problem *p, int k, int cand, hipStream_t st) {
+    const int w = k % p->nwin;
     const size_t a = (size_t)p->cn0[k], n = (size_t)(p->cn0[k + 1] - p->cn0[k]);
     auto out = [&](const cdlm_problem::HostArr &h) {
-        return hipMemcpyAsync(h.host + a * h.per_seq, h.dev, n * h.per_seq, hipMemcpyDeviceToHost, st);
+        return hipMemcpyAsync(h.host + a * h.per_seq, h.dev[w], n * h.per_seq, hipMemcpyDeviceToHost, st);
     };
     hipError_t e = hipSuccess;
     for (size_t i = 0; i < p->hstate.size() && e == hipSuccess; ++i) e = out(p->hstate[i]);
@@ -746,15 +763,50 @@ cdlm_status stream_out(cdlm_problem *p, int k, int cand) {
     return CDLM_OK;
 }
 
-// One pass over the chunks: in, fn(chunk set), out.
+// One pass over the chunks: in, fn(chunk set), out.  With one window each chunk waits for the
+// last; with two, chunk k's upload (copy stream cin) waits for chunk k-2's download from the same
+// window (copy stream cout), its kernels (the context stream) for its upload, and its download for
+// its kernels, so uploads, kernels and downloads of three consecutive chunks overlap.
 template <class F>
 cdlm_status stream_pass(cdlm_problem *p, bool x0, int cand, F fn) {
-    for (int k = 0; k < p->nchunk; ++k) {
-        cdlm_status st = stream_in(p, k, x0);
-        if (st == CDLM_OK) st = fn(p->csets[(size_t)k], k);
-        if (st == CDLM_OK) st = stream_out(p, k, cand);
-        if (st != CDLM_OK) return st;
+    hipStream_t cs = p->ctx->stream;
+    const int K = p->nchunk;
+    if (p->nwin < 2) {
+        for (int k = 0; k < K; ++k) {
+            cdlm_status st = stream_in(p, k, x0, cs);
+            if (st == CDLM_OK) st = fn(p->csets[(size_t)k], k);
+            if (st == CDLM_OK) st = stream_out(p, k, cand, cs);
+            if (st != CDLM_OK) return st;
+        }
+        return sync_ctx(p->ctx);
     }
+    hipEvent_t *ev_in = p->ev.data(), *ev_run = ev_in + K, *ev_out = ev_run + K;
+    auto put = [&](int k) -> cdlm_status {
+        if (k >= 2) CDLM_HIP(p->ctx, hipStreamWaitEvent(p->cin, ev_out[k - 2], 0));
+        cdlm_status st = stream_in(p, k, x0, p->cin);
+        if (st != CDLM_OK) return st;
+        CDLM_HIP(p->ctx, hipEventRecord(ev_in[k], p->cin));
+        return CDLM_OK;
+    };
+    // the previous pass's kernels and reductions (on cs) come before anything this pass moves
+    CDLM_HIP(p->ctx, hipEventRecord(ev_run[0], cs));
+    CDLM_HIP(p->ctx, hipStreamWaitEvent(p->cin, ev_run[0], 0));
+    cdlm_status st = put(0);
+    if (st == CDLM_OK && K > 1) st = put(1);
+    for (int k = 0; k < K && st == CDLM_OK; ++k) {
+        CDLM_HIP(p->ctx, hipStreamWaitEvent(cs, ev_in[k], 0));
+        if ((st = fn(p->csets[(size_t)k], k)) != CDLM_OK) break;
+        CDLM_HIP(p->ctx, hipEventRecord(ev_run[k], cs));
+        CDLM_HIP(p->ctx, hipStreamWaitEvent(p->cout, ev_run[k], 0));
+        if ((st = stream_out(p, k, cand, p->cout)) != CDLM_OK) break;
+        CDLM_HIP(p->ctx, hipEventRecord(ev_out[k], p->cout));
+        if (k + 2 < K) st = put(k + 2);
+    }
+    // everything this pass moved is back before the caller reads host memory or runs the next pass
+    hipError_t e = hipStreamSynchronize(p->cin);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->cout);
+    if (st != CDLM_OK) return st;
+    if (e != hipSuccess) return cdlm_fail(p->ctx, CDLM_ERR_HIP, "chunk copies: %s", hipGetErrorString(e));
     return sync_ctx(p->ctx);
 }
 
@@ -867,21 +919,24 @@ cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const flo
     for (int m = 0; m < M; ++m)
         for (int l = p->fl[m]; l < Lg; ++l) D0m[(size_t)m * Lg + l] = 0.f;
 
-    // streamed: the fewest chunks (a divisor of the mean chunks) whose window fits the budget;
-    // a problem that fits whole stays resident
+    // streamed: the fewest chunks (a divisor of the mean chunks) whose two windows fit the budget
+    // (copies overlapping the kernels), else the fewest whose one window does; a problem that fits
+    // whole stays resident
     int Nwin = N;
     if (stream_budget && plan_total(buf_plan(L4, N, M, gamma, X0 != nullptr)) > stream_budget) {
         const int nch = d.nch;
-        for (int k = 2; k <= nch && !p->nchunk; ++k) {
-            if (nch % k) continue;
-            int mx = 1;
-            for (int c = 0; c < k; ++c)
-                mx = std::max(mx, (int)((long)((c + 1) * (nch / k)) * N / nch) - (int)((long)(c * (nch / k)) * N / nch));
-            if (plan_total(buf_plan(L4, mx, M, gamma, X0 != nullptr, N)) <= stream_budget) {
-                p->nchunk = k;
-                Nwin = mx;
+        for (int wins = 2; wins >= 1 && !p->nchunk; --wins)
+            for (int k = 2; k <= nch && !p->nchunk; ++k) {
+                if (nch % k) continue;
+                int mx = 1;
+                for (int c = 0; c < k; ++c)
+                    mx = std::max(mx, (int)((long)((c + 1) * (nch / k)) * N / nch) - (int)((long)(c * (nch / k)) * N / nch));
+                if (plan_total(buf_plan(L4, mx, M, gamma, X0 != nullptr, N), wins) <= stream_budget) {
+                    p->nchunk = k;
+                    p->nwin = wins;
+                    Nwin = mx;
+                }
             }
-        }
         if (!p->nchunk)
             return cdlm_fail(ctx, CDLM_ERR_OOM, "a device window of %d chunks still exceeds the budget of %llu bytes", nch,
                              (unsigned long long)stream_budget);
@@ -891,7 +946,7 @@ cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const flo
     }
     const BufPlan b = buf_plan(L4, Nwin, M, gamma, X0 != nullptr, N);
     cdlm_status st = CDLM_OK;
-    const size_t total = plan_total(b);
+    const size_t total = plan_total(b, p->nwin);
     if (arena) {
         if (arena_bytes < total)
             return cdlm_fail(ctx, CDLM_ERR_ARG, "arena of %zu bytes is smaller than the problem's %zu", arena_bytes, total);
@@ -930,8 +985,21 @@ cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const flo
     A(d.dpart, b.dpart);
     A(d.trace, b.trace);
     A(d.sc, b.sc);
+    // the second window's per-sequence buffers (descriptor d2: d with these in place)
+    CdlDev d2 = d;
+    float *s_dev2 = nullptr, *X0_dev2 = nullptr;
+    if (p->nwin == 2) {
+        A(s_dev2, b.s);
+        A(d2.Xt, b.E); A(d2.y2, b.E); A(d2.y3, b.E3); A(d2.g1h, b.Eh);
+        if (X0) A(X0_dev2, b.Ex0);
+        A(d2.h, b.LN); A(d2.y1, b.LN); A(d2.Y1, b.LN); A(d2.DX, b.LN); A(d2.G2, b.LN); A(d2.g2, b.LN); A(d2.Td, b.LN);
+        A(d2.xs[0], b.xs); A(d2.xs[1], b.xs);
+        A(d2.y3s, b.y3s);
+        A(d2.kap, b.kap);
+    }
 #undef A
     d.s = s_dev; d.X0 = X0_dev; d.D0 = D0_dev; d.fl = fl_dev; d.tw = tw_dev;
+    d2.s = s_dev2; d2.X0 = X0_dev2; d2.D0 = D0_dev; d2.fl = fl_dev; d2.tw = tw_dev;
     hipStream_t hs = ctx->stream;
     std::vector<float2> tw(Lg);
     for (int t = 0; t < Lg; ++t) {
@@ -943,8 +1011,9 @@ cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const flo
         // the per-sequence arrays of all N sequences in pinned host memory (data and initial
         // codes in, state in and out, X_best slices out)
         const size_t Hs = (size_t)half_bins(Lg), nslot = (size_t)Lg / 4;
-        auto host = [&](cdlm_problem::HostArr &h, void *dev, size_t per_seq) -> bool {
-            h.dev = static_cast<char *>(dev);
+        auto host = [&](cdlm_problem::HostArr &h, void *dev, void *dev2, size_t per_seq) -> bool {
+            h.dev[0] = static_cast<char *>(dev);
+            h.dev[1] = static_cast<char *>(dev2);
             h.per_seq = per_seq;
             void *q = nullptr;
             if (hipHostMalloc(&q, per_seq * (size_t)N ? per_seq * (size_t)N : 4) != hipSuccess) {
@@ -956,17 +1025,33 @@ cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const flo
             return true;
         };
         const size_t LMb = (size_t)Lg * M * sizeof(float), Lb = (size_t)Lg * sizeof(float);
-        struct { void *dev; size_t per; } arrs[] = {
-            {d.Xt, LMb}, {d.y2, LMb}, {d.y3, 3 * nslot * M * sizeof(float)}, {d.g1h, Hs * M * sizeof(float2)},
-            {d.y3s, (size_t)M * (Lg / 8 + 1) * sizeof(float2)}, {d.kap, Hs * sizeof(float2)}, {d.h, Lb}, {d.y1, Lb},
-            {d.Y1, Lb}, {d.DX, Lb}, {d.G2, Lb}, {d.g2, Lb}, {d.Td, Lb}};
-        bool ok = host(p->hs, s_dev, (size_t)L4 * sizeof(float)) && (!X0 || host(p->hx0, X0_dev, LMb)) &&
-                  host(p->hxs[0], d.xs[0], (size_t)p->L * M * sizeof(float)) &&
-                  host(p->hxs[1], d.xs[1], (size_t)p->L * M * sizeof(float));
+        struct { void *dev, *dev2; size_t per; } arrs[] = {
+            {d.Xt, d2.Xt, LMb}, {d.y2, d2.y2, LMb}, {d.y3, d2.y3, 3 * nslot * M * sizeof(float)},
+            {d.g1h, d2.g1h, Hs * M * sizeof(float2)}, {d.y3s, d2.y3s, (size_t)M * (Lg / 8 + 1) * sizeof(float2)},
+            {d.kap, d2.kap, Hs * sizeof(float2)}, {d.h, d2.h, Lb}, {d.y1, d2.y1, Lb}, {d.Y1, d2.Y1, Lb},
+            {d.DX, d2.DX, Lb}, {d.G2, d2.G2, Lb}, {d.g2, d2.g2, Lb}, {d.Td, d2.Td, Lb}};
+        bool ok = host(p->hs, s_dev, s_dev2, (size_t)L4 * sizeof(float)) && (!X0 || host(p->hx0, X0_dev, X0_dev2, LMb)) &&
+                  host(p->hxs[0], d.xs[0], d2.xs[0], (size_t)p->L * M * sizeof(float)) &&
+                  host(p->hxs[1], d.xs[1], d2.xs[1], (size_t)p->L * M * sizeof(float));
         for (auto &a : arrs) {
             if (!ok) break;
             p->hstate.emplace_back();
-            ok = host(p->hstate.back(), a.dev, a.per);
+            ok = host(p->hstate.back(), a.dev, a.dev2, a.per);
+        }
+        if (ok && p->nwin == 2) {
+            // two copy streams (default priority) and three events per chunk
+            ok = hipStreamCreateWithFlags(&p->cin, hipStreamNonBlocking) == hipSuccess &&
+                 hipStreamCreateWithFlags(&p->cout, hipStreamNonBlocking) == hipSuccess;
+            for (int i = 0; ok && i < 3 * p->nchunk; ++i) {
+                hipEvent_t ev = nullptr;
+                ok = hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+                if (ok) p->ev.push_back(ev);
+            }
+            if (!ok) {
+                (void)hipGetLastError();
+                free_problem(p.get());
+                return cdlm_fail(ctx, CDLM_ERR_HIP, "copy streams for a streamed problem");
+            }
         }
         if (!ok) {
             free_problem(p.get());
@@ -1006,14 +1091,16 @@ cdlm_status problem_create_impl(cdlm_ctx *ctx, const cdlm_params *prm, const flo
         // chunk k's launch set: its sequences, their first index and its dictionary-mean chunks
         p->csets.resize((size_t)p->nchunk);
         const int per = d.nch / p->nchunk;
+        const CdlDev d1 = d;
         for (int k = 0; k < p->nchunk && st == CDLM_OK; ++k) {
+            d = (k % p->nwin) ? d2 : d1;   // chunk k's window
             d.N = p->cn0[k + 1] - p->cn0[k];
             d.n_off = p->cn0[k];
             d.red_c0 = k * per;
             d.red_cn = per;
             st = set_build(ctx, &pp, 1, &p->csets[(size_t)k]);
         }
-        d.N = Nwin; d.n_off = 0; d.red_c0 = 0; d.red_cn = d.nch;
+        d = d1;
     }
     if (st == CDLM_OK) st = p->nchunk ? streamed_reset(p.get()) : run_reset(ctx, p->set);
     if (st == CDLM_OK) st = sync_ctx(ctx);
@@ -1188,12 +1275,12 @@ cdlm_status streamed_get(cdlm_problem *p, int32_t field, void *dst, size_t bytes
         const int what = field == CDLM_F_D ? 0 : field == CDLM_F_g1 ? 1 : 2;
         for (int k = 0; k < p->nchunk && s == CDLM_OK; ++k) {
             const size_t a = (size_t)p->cn0[k], n = (size_t)(p->cn0[k + 1] - p->cn0[k]);
-            s = stream_in(p, k, false);
+            s = stream_in(p, k, false, st);
             if (s != CDLM_OK) break;
             if (field == CDLM_F_X) {
                 // the candidate slice buffer is scratch here; X is written back materialised
                 s = run_best_seq(p->ctx, p->csets[(size_t)k], nullptr, tmp);
-                if (s == CDLM_OK) s = stream_out(p, k, -1);
+                if (s == CDLM_OK) s = stream_out(p, k, -1, st);
             } else if (n > 0) {
                 hipLaunchKernelGGL(k_dict_to_spatial, dim3((unsigned)(n * p->M)), dim3(kThreads), lds_bytes_half(p->Lg), st,
                                    p->csets[(size_t)k].tab, what, tmp);

@@ -83,9 +83,11 @@ cdlm_status cdlm_problem_create_from_bases(cdlm_ctx *ctx, const cdlm_params *par
  * state of all N sequences lives in pinned host memory and the sequences move through a device
  * window in chunks (the fewest, a divisor of the dictionary-mean chunk count, whose window fits
  * the budget): every pass over the sequences (code_update, update_best, dict_update, one outer
- * iteration of cdlm_optimize) moves each chunk in, runs it and moves it back.  The per-filter
- * state and the fixed-order reductions stay on the device, so the results equal the resident
- * problem's bit for bit; the price is the host<->device traffic of the whole state per pass.
+ * iteration of cdlm_optimize) moves each chunk in, runs it and moves it back (with two windows,
+ * where they fit the budget, the next chunk's upload and the last one's download overlap the
+ * current chunk's kernels).  The per-filter state and the fixed-order reductions stay on the
+ * device, so the results equal the resident problem's bit for bit; the price is the
+ * host<->device traffic of the whole state per pass.
  * The reference sizes its batches from the free memory the same way (src/SEARCH/store.jl:109-119).
  * A problem that fits stays resident.  A streamed problem cannot join a batch or an activation
  * record (its X_best slices are in host memory: read them with cdlm_get). */

@@ -1,6 +1,7 @@
 """Streamed problems (cdlm_problem_create_streamed): a problem whose footprint exceeds its device
-budget keeps its per-sequence state in pinned host memory and moves the sequences through a
-device window in chunks.  The per-filter state and the fixed-order reductions stay on the
+budget keeps its per-sequence state in pinned host memory and moves the sequences through two
+device windows in chunks (the next chunk's upload and the last one's download overlapping the
+current chunk's kernels), or through one window when two do not fit.  The per-filter state and the fixed-order reductions stay on the
 device, so every result must equal the resident problem's bit for bit: traces, objectives,
 thresholds, every field, after single steps, whole schedules and a reset.  The server streams a
 job larger than its budget the same way."""
@@ -76,6 +77,27 @@ def test_streamed_equals_resident_bit_for_bit(chunks, x0):
     _same(g, r, "after reset")
     with pytest.raises(cdlm.CdlmError):
         cdlm.Batch([g])
+    g.close(); r.close()
+
+
+def test_streamed_one_window_when_two_do_not_fit():
+    """A budget that holds one window of a single sequence but not two: the problem streams
+    through one window (its chunks one after another, no copy overlap), one sequence per chunk,
+    and still gives the resident results bit for bit."""
+    N, L, M, fl = 64, 100, 16, [8] * 16
+    s, D0, X0, _ = _synth(17, N, L, M, fl)
+    fp = cdlm.footprint_bytes
+    per_seq = fp(L, N, M, 8, False) - fp(L, N - 1, M, 8, False)   # ~0.1 MB; 256-byte alignment aside
+    fixed = fp(L, N, M, 8, False) - per_seq * N
+    bud = int(fixed + 1.5 * per_seq)
+    r = cdlm.CDL(s, fl, 1.0, 100.0, 0.05, D0=D0, seed=5)
+    g = cdlm.CDL(s, fl, 1.0, 100.0, 0.05, D0=D0, seed=5, stream_budget=bud)
+    flush("one-window stream: budget %d (one sequence %d), chunks %d, device bytes %d" % (
+        bud, per_seq, g.chunks, g.device_bytes))
+    assert g.chunks >= N // 2 and g.device_bytes <= bud < fixed + 2 * per_seq
+    np.testing.assert_array_equal(g.optimize(2, 3, 1), r.optimize(2, 3, 1))
+    assert g.code_thresh(1.0) == r.code_thresh(1.0)
+    _same(g, r, "one window")
     g.close(); r.close()
 
 
