@@ -755,6 +755,9 @@ def main():
                      "algorithmic_bytes_per_launch": b_launch,
                      "algorithmic_basis": "4 N Lg (6M + 7) bytes per CSC iteration (X, y2, y3 read and written, y1, "
                                           "Y1, DX read and written, the data read) x %d iterations" % CSC_ITER,
+                     "note": "frac above 1 means the kernel moves fewer bytes than that compulsory count: it carries "
+                             "its state between the iterations of a launch as (X + y2, X + y3) and keeps y1, Y1, DX "
+                             "on chip; carry_floor and traffic give the bytes it does move",
                      "avg_launch_ms": launch_ms, "launches": csc_launches, "iterations_per_launch": CSC_ITER,
                      "ms_per_iteration": avg_csc,
                      "carry_floor": {"bytes_per_launch": b_carry, "achieved": achieved_carry,
