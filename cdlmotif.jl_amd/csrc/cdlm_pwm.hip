@@ -20,6 +20,8 @@
 
 #include "cdlm_internal.h"
 
+#include <type_traits>
+
 namespace {
 
 constexpr int kPwmThreads = 256;
@@ -92,12 +94,32 @@ __host__ __device__ constexpr int pwm_lpad(int L, int MAXJ, int CH) {
     return ((L + 16 * CH - 1) / (16 * CH)) * (16 * CH) + MAXJ + 16;
 }
 
+// Sequence prefetch: a wave's next sequence is loaded into registers (position lane + 64 k in
+// slot k, base 4 past the end) while the current one is scanned, so its loads are in flight
+// together and never wait one after another.  Covers the 64 * kPwmPre positions of any
+// one-hot LDS image that fits the scans' LDS budgets.
+constexpr int kPwmPre = 16;
+static_assert(kPwmLdsMax / (4 * sizeof(float4)) <= 64 * kPwmPre, "one-hot LDS image longer than the prefetch");
+struct SeqPre {
+    int b[kPwmPre];
+    // src: the wave's sequence (wave-uniform), L >= 1 bases, read through a buffer resource of
+    // L bytes: loads past L return 0 without touching memory
+    __device__ __forceinline__ void load(const uint8_t *src, int L, int lane) {
+        const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void *)src, (short)0, L, 0x00020000);
+#pragma unroll
+        for (int k = 0; k < kPwmPre; ++k) {
+            const int v = __builtin_amdgcn_raw_buffer_load_b8(r, lane + 64 * k, 0, 0);
+            b[k] = lane + 64 * k < L ? v : 4;
+        }
+    }
+};
+
 template <int G, int MAXJ, int CH, bool OH>
 __global__ __launch_bounds__(256) void k_pwm_hits_mfma(const uint8_t *__restrict__ bases, int L, long N, PwmTiles P,
                                                        int *hk, long long *hn, int *hp, signed char *hs, long long cap,
                                                        unsigned long long *count) {
     extern __shared__ uint8_t seqs_lds[];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int Lpad = pwm_lpad(L, MAXJ, CH);
     uint8_t *sq = seqs_lds + w * Lpad;
     float4 *oh = reinterpret_cast<float4 *>(seqs_lds) + (size_t)w * Lpad;
@@ -125,13 +147,19 @@ __global__ __launch_bounds__(256) void k_pwm_hits_mfma(const uint8_t *__restrict
     const int arow = lane & 15, abase = lane >> 4;
     const int jsteps = gmax;   // uniform: the tiles of this work-group
     const long nstride = (long)gridDim.y * 4;
+    SeqPre pre;
+    if (OH && (long)blockIdx.y * 4 + w < N) pre.load(bases + ((long)blockIdx.y * 4 + w) * (long)L, L, lane);
     for (long n = (long)blockIdx.y * 4 + w; n < N; n += nstride) {
         const uint8_t *src = bases + n * (long)L;
         if constexpr (OH) {
-            for (int i = lane; i < Lpad; i += 64) {
-                const int b = i < L ? src[i] : 4;
-                oh[i] = make_float4(b == 0 ? 1.f : 0.f, b == 1 ? 1.f : 0.f, b == 2 ? 1.f : 0.f, b == 3 ? 1.f : 0.f);
+#pragma unroll
+            for (int k = 0; k < kPwmPre; ++k) {
+                const int i = lane + 64 * k;
+                const int b = pre.b[k];
+                if (64 * k < Lpad && i < Lpad)
+                    oh[i] = make_float4(b == 0 ? 1.f : 0.f, b == 1 ? 1.f : 0.f, b == 2 ? 1.f : 0.f, b == 3 ? 1.f : 0.f);
             }
+            if (n + nstride < N) pre.load(src + nstride * (long)L, L, lane);
         } else {
             for (int i = lane; i < Lpad; i += 64) sq[i] = i < L ? src[i] : (uint8_t)4;
         }
@@ -232,7 +260,7 @@ template <int MAXJ, int CH, bool OH, bool NUC>
 __global__ __launch_bounds__(256) void k_pwm_best_mfma(const uint8_t *__restrict__ bases, int L, long N, PwmTiles P,
                                                        float *__restrict__ bv, int *__restrict__ bp) {
     extern __shared__ uint8_t seqs_lds[];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int Lpad = pwm_lpad(L, MAXJ, CH);
     uint8_t *sq = seqs_lds + w * Lpad;
     float4 *oh = reinterpret_cast<float4 *>(seqs_lds) + (size_t)w * Lpad;
@@ -248,13 +276,19 @@ __global__ __launch_bounds__(256) void k_pwm_best_mfma(const uint8_t *__restrict
     const float sth = ok ? P.slot_th[sl] : 0.f;
     const int arow = lane & 15, abase = lane >> 4;
     const long nstride = (long)gridDim.y * 4;
+    SeqPre pre;
+    if (OH && (long)blockIdx.y * 4 + w < N) pre.load(bases + ((long)blockIdx.y * 4 + w) * (long)L, L, lane);
     for (long n = (long)blockIdx.y * 4 + w; n < N; n += nstride) {
         const uint8_t *src = bases + n * (long)L;
         if constexpr (OH) {
-            for (int i = lane; i < Lpad; i += 64) {
-                const int b = i < L ? src[i] : 4;
-                oh[i] = make_float4(b == 0 ? 1.f : 0.f, b == 1 ? 1.f : 0.f, b == 2 ? 1.f : 0.f, b == 3 ? 1.f : 0.f);
+#pragma unroll
+            for (int k = 0; k < kPwmPre; ++k) {
+                const int i = lane + 64 * k;
+                const int b = pre.b[k];
+                if (64 * k < Lpad && i < Lpad)
+                    oh[i] = make_float4(b == 0 ? 1.f : 0.f, b == 1 ? 1.f : 0.f, b == 2 ? 1.f : 0.f, b == 3 ? 1.f : 0.f);
             }
+            if (n + nstride < N) pre.load(src + nstride * (long)L, L, lane);
         } else {
             for (int i = lane; i < Lpad; i += 64) sq[i] = i < L ? src[i] : (uint8_t)4;
         }
@@ -301,10 +335,12 @@ __global__ __launch_bounds__(256) void k_pwm_best_mfma(const uint8_t *__restrict
                 for (int r = 0; r < 4; ++r) {
                     const int l = w0 + 16 * c + 4 * abase + r;
                     const bool fits = l + sm <= L;
-                    if (sk < 0 || l >= L || (NUC && !fits)) continue;
+                    const bool valid = sk >= 0 && l < L && (!NUC || fits);
                     float v = acc[c][r];
                     if (!NUC) v = (fits && v > sth) ? v : 0.f;
-                    if (bi == INT_MAX || v > best) { best = v; bi = l; }
+                    const bool take = valid && (bi == INT_MAX || v > best);
+                    best = take ? v : best;
+                    bi = take ? l : bi;
                 }
         }
         // the four lanes of a slot: larger value, then smaller window
@@ -328,19 +364,22 @@ __global__ __launch_bounds__(256) void k_pwm_best_mfma(const uint8_t *__restrict
 // cost what one position-major pass of the K=4 form costs.  Layout (measured): block b uses A and
 // B lanes 32b..32b+31; result register r of lane l is block r/16, row 8((r%16)/4) + 4(l/32) + r%4,
 // column l%32.  Both blocks take the tile's 32 strand-PWM slots, block b the windows
-// w0 + 32b + row.  LDS: the tile's B values [a][j][32 slots] | per wave the sequence as four
-// one-hot planes of LP floats.  Same outputs as k_pwm_best_mfma (slot s = 32 t + column).
+// w0 + 32b + row; a wave runs two such 64-window chunks (w0, w0 + 64) as two independent MFMA
+// chains sharing each B load, base by base (a outer, j inner, so the accumulators stay in
+// place).  LDS: the tile's B values [a][j][32 slots] | per wave the sequence as four one-hot
+// planes of LP floats.  Same outputs as k_pwm_best_mfma (slot s = 32 t + column).
 constexpr int kBest32Lds = 64 * 1024;
-__host__ __device__ constexpr int best32_lpad(int L, int MAXJ) { return ((L + 63) / 64) * 64 + MAXJ + 16; }
+static_assert(kBest32Lds / (4 * 4 * sizeof(float)) <= 64 * kPwmPre, "one-hot planes longer than the prefetch");
+__host__ __device__ constexpr int best32_lpad(int L, int MAXJ) { return ((L + 127) / 128) * 128 + MAXJ + 16; }
 
 template <int MAXJ>
-__global__ __launch_bounds__(256) void k_pwm_best32_mfma(const uint8_t *__restrict__ bases, int L, long N,
+__global__ __launch_bounds__(256, MAXJ > 32 ? 2 : 3) void k_pwm_best32_mfma(const uint8_t *__restrict__ bases, int L, long N,
                                                          const float *__restrict__ B32, const int *__restrict__ tlen,
                                                          const int *__restrict__ slot_k, const int *__restrict__ slot_m,
                                                          int T, float *__restrict__ bv, int *__restrict__ bp) {
     typedef float floatx32 __attribute__((ext_vector_type(32)));
     extern __shared__ float lds32[];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int LP = best32_lpad(L, MAXJ);
     float *bt = lds32;                                   // 4 * MAXJ * 32
     float *oh = lds32 + 4 * MAXJ * 32 + (size_t)w * 4 * LP;
@@ -355,53 +394,70 @@ __global__ __launch_bounds__(256) void k_pwm_best32_mfma(const uint8_t *__restri
     const int sl = t * 32 + col;
     const int sk = t < T ? slot_k[sl] : -1, sm = t < T ? slot_m[sl] : 0;
     const long nstride = (long)gridDim.y * 4;
+    SeqPre pre;
+    if ((long)blockIdx.y * 4 + w < N) pre.load(bases + ((long)blockIdx.y * 4 + w) * (long)L, L, lane);
     for (long n = (long)blockIdx.y * 4 + w; n < N; n += nstride) {
         const uint8_t *src = bases + n * (long)L;
-        for (int i = lane; i < LP; i += 64) {
-            const int b = i < L ? src[i] : 4;
 #pragma unroll
-            for (int a = 0; a < 4; ++a) oh[a * LP + i] = b == a ? 1.f : 0.f;
+        for (int k = 0; k < kPwmPre; ++k) {
+            const int i = lane + 64 * k;
+            const int b = pre.b[k];
+            if (64 * k < LP && i < LP) {
+#pragma unroll
+                for (int a = 0; a < 4; ++a) oh[a * LP + i] = b == a ? 1.f : 0.f;
+            }
         }
+        if (n + nstride < N) pre.load(src + nstride * (long)L, L, lane);
         wave_lds_sync();
         float best = 0.f;
         int bi = INT_MAX;
-        for (int w0 = 0; w0 < L; w0 += 64) {
-            floatx32 acc;
+        for (int w0 = 0; w0 < L; w0 += 128) {
+            // two chunks of 64 windows (w0 and w0 + 64) share each B load: two independent chains
+            floatx32 acc0, acc1;
 #pragma unroll
-            for (int r = 0; r < 32; ++r) acc[r] = 0.f;
-            for (int a = 0; a < 4; ++a) {
-                const float *pa = oh + a * LP + w0 + lane;     // window w0 + lane (row lane%32 of block lane/32)
-                const float *ba = bt + a * MAXJ * 32 + col;
-#define CDLM_B32_STEPS(JS)                                                                               \
-    case JS:                                                                                             \
-        _Pragma("unroll") for (int j = 0; j < JS; ++j)                                                   \
-            acc = __builtin_amdgcn_mfma_f32_32x32x1f32(pa[j], ba[32 * j], acc, 0, 0, 0);                 \
-        break;
+            for (int r = 0; r < 32; ++r) { acc0[r] = 0.f; acc1[r] = 0.f; }
+            auto chain = [&](auto js_c) {
+                constexpr int JS = decltype(js_c)::value;
+#pragma unroll 1
+                for (int a = 0; a < 4; ++a) {
+                    const float *pa = oh + a * LP + w0 + lane;     // window w0 + lane (row lane%32 of block lane/32)
+                    const float *ba = bt + a * MAXJ * 32 + col;
+#pragma unroll
+                    for (int j = 0; j < JS; ++j) {
+                        const float bj = ba[32 * j];
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x1f32(pa[j], bj, acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x1f32(pa[64 + j], bj, acc1, 0, 0, 0);
+                    }
+                }
+            };
+#define CDLM_B32_STEPS(JS) case JS: chain(std::integral_constant<int, JS>{}); break;
 #define CDLM_B32_STEPS8(J0)                                                                              \
     CDLM_B32_STEPS(J0 + 1) CDLM_B32_STEPS(J0 + 2) CDLM_B32_STEPS(J0 + 3) CDLM_B32_STEPS(J0 + 4)          \
     CDLM_B32_STEPS(J0 + 5) CDLM_B32_STEPS(J0 + 6) CDLM_B32_STEPS(J0 + 7) CDLM_B32_STEPS(J0 + 8)
-                switch (jsteps) {
-                    CDLM_B32_STEPS8(0) CDLM_B32_STEPS8(8) CDLM_B32_STEPS8(16) CDLM_B32_STEPS8(24)
-                    default:
-                        if constexpr (MAXJ > 32) {
-                            switch (jsteps) {
-                                CDLM_B32_STEPS8(32) CDLM_B32_STEPS8(40) CDLM_B32_STEPS8(48) CDLM_B32_STEPS8(56)
-                                default: break;
-                            }
+            switch (jsteps) {
+                CDLM_B32_STEPS8(0) CDLM_B32_STEPS8(8) CDLM_B32_STEPS8(16) CDLM_B32_STEPS8(24)
+                default:
+                    if constexpr (MAXJ > 32) {
+                        switch (jsteps) {
+                            CDLM_B32_STEPS8(32) CDLM_B32_STEPS8(40) CDLM_B32_STEPS8(48) CDLM_B32_STEPS8(56)
+                            default: break;
                         }
-                        break;
-                }
+                    }
+                    break;
+            }
 #undef CDLM_B32_STEPS8
 #undef CDLM_B32_STEPS
-            }
-            // this lane's windows in increasing order: block r/16, row 8((r%16)/4) + 4 half + r%4
+            // this lane's windows in increasing order: chunk, block r/16, row 8((r%16)/4) + 4 half + r%4
 #pragma unroll
-            for (int r = 0; r < 32; ++r) {
-                const int l = w0 + 32 * (r >> 4) + 8 * ((r & 15) >> 2) + 4 * half + (r & 3);
-                if (sk < 0 || l + sm > L) continue;
-                const float v = acc[r];
-                if (bi == INT_MAX || v > best) { best = v; bi = l; }
-            }
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int r = 0; r < 32; ++r) {
+                    const int l = w0 + 64 * c + 32 * (r >> 4) + 8 * ((r & 15) >> 2) + 4 * half + (r & 3);
+                    const float v = c ? acc1[r] : acc0[r];
+                    const bool take = sk >= 0 && l + sm <= L && (bi == INT_MAX || v > best);
+                    best = take ? v : best;
+                    bi = take ? l : bi;
+                }
         }
         // the two lanes of a slot: larger value, then smaller window
         {
