@@ -201,6 +201,19 @@ __global__ __launch_bounds__(256) void k_pwm_hits_mfma(const uint8_t *__restrict
             }
 #undef CDLM_MFMA_STEPS8
 #undef CDLM_MFMA_STEPS
+            // hits are rare: one comparison of each group's largest score first, the per-window
+            // tests only when some lane of the wave has a score above its threshold
+            bool any = false;
+#pragma unroll
+            for (int gi = 0; gi < G; ++gi) {
+                float mx = acc[0][gi][0];
+#pragma unroll
+                for (int c = 0; c < CH; ++c)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) mx = fmaxf(mx, acc[c][gi][r]);
+                any = any || mx > sth[gi];
+            }
+            if (!__ballot(any)) continue;
             int myc = 0;
             unsigned hitmask = 0;
 #pragma unroll
