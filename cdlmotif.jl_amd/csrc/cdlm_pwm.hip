@@ -341,6 +341,23 @@ __global__ __launch_bounds__(256) void k_pwm_best_mfma(const uint8_t *__restrict
 #undef CDLM_BEST_STEPS8
 #undef CDLM_BEST_STEPS
             }
+            if (!NUC) {
+                // thresholded scores are rare: when no lane of the wave has a score above its
+                // threshold, every window's value is 0 and only a lane's first window of the chunk
+                // (w0 + 4 abase, the smallest) can be taken, as the per-window rule below would
+                float mx = acc[0][0];
+#pragma unroll
+                for (int c = 0; c < CH; ++c)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) mx = fmaxf(mx, acc[c][r]);
+                if (!__ballot(mx > sth)) {
+                    const int lf = w0 + 4 * abase;
+                    const bool take = sk >= 0 && lf < L && (bi == INT_MAX || 0.f > best);
+                    best = take ? 0.f : best;
+                    bi = take ? lf : bi;
+                    continue;
+                }
+            }
             // this lane's windows of its slot, in increasing order: l = w0 + 16 c + 4 abase + r
 #pragma unroll
             for (int c = 0; c < CH; ++c)

@@ -69,6 +69,35 @@ def test_argmax_and_best_random_vs_oracle():
     np.testing.assert_array_equal(sc.view(np.uint32), osc.view(np.uint32))
 
 
+def test_rare_and_negative_thresholds_vs_oracle():
+    """Thresholds in the upper tail of strongly negative scores: windows above the threshold are
+    rare (most 32-window chunks of a tile have none, the others some), and many of them score
+    below 0, so the argmax's running best can be negative and a later window of value 0 (below
+    its threshold) replaces it; hit list, argmax and best hit bit for bit against the oracle."""
+    rng = np.random.default_rng(31)
+    K, N, L = 20, 120, 333
+    lens = rng.integers(6, 17, K).astype(np.int32)
+    pwms = np.zeros((K, 4, int(lens.max())), np.float32)
+    for k in range(K):
+        pwms[k, :, :lens[k]] = (rng.standard_normal((4, lens[k])) * 0.5 - 1.0).astype(np.float32)
+    bases = rng.integers(0, 4, (N, L)).astype(np.uint8)
+    # per PWM, the score exceeded by about 0.3% of random windows (mean -m, sd ~0.5 sqrt(m))
+    thresh = np.array([np.float32(-lens[k] + 2.75 * 0.5 * np.sqrt(lens[k])) for k in range(K)], np.float32)
+    k_, n_, p_, st_, sc_ = O.pwm_hits(bases, pwms, lens, thresh, both_strands=True)
+    flush("rare thresholds: %d hits, %d with a negative score" % (len(k_), int((np.asarray(sc_) < 0).sum())))
+    assert 0 < len(k_) < N * L * K * 2 // 50
+    assert (np.asarray(sc_) < 0).any()
+    exp = np.stack([k_, n_, p_, st_], 1).astype(np.int64)
+    exp = exp[np.lexsort((exp[:, 3], exp[:, 2], exp[:, 1], exp[:, 0]))]
+    np.testing.assert_array_equal(cdlm.scan_hits(bases, pwms, lens, thresh, both_strands=True), exp)
+    np.testing.assert_array_equal(cdlm.scan_w_gpu(bases, pwms, lens, thresh), O.pwm_gpu_argmax(bases, pwms, lens, thresh))
+    pos, comp, sc = cdlm.overlapping_scan(bases, pwms, lens, thresh)
+    opos, ocomp, osc = O.pwm_best(bases, pwms, lens, thresh)
+    np.testing.assert_array_equal(pos, opos)
+    np.testing.assert_array_equal(comp.astype(np.int8), ocomp)
+    np.testing.assert_array_equal(sc.view(np.uint32), osc.view(np.uint32))
+
+
 def test_edge_cases():
     rng = np.random.default_rng(9)
     pwms = np.zeros((2, 4, 12), np.float32)
