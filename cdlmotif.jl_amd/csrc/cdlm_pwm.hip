@@ -478,16 +478,43 @@ __global__ __launch_bounds__(256, MAXJ > 32 ? 2 : 3) void k_pwm_best32_mfma(cons
 #undef CDLM_B32_STEPS8
 #undef CDLM_B32_STEPS
             // this lane's windows in increasing order: chunk, block r/16, row 8((r%16)/4) + 4 half + r%4
+            if (w0 + 127 + jsteps <= L) {
+                // every window of both chunks fits every slot of the tile: the first largest of a
+                // chunk by a pairwise tree (the later of two is taken only when strictly larger,
+                // as the sequential rule below), then the chunks in order
 #pragma unroll
-            for (int c = 0; c < 2; ++c)
+                for (int c = 0; c < 2; ++c) {
+                    float tv[32];
+                    int to[32];
 #pragma unroll
-                for (int r = 0; r < 32; ++r) {
-                    const int l = w0 + 64 * c + 32 * (r >> 4) + 8 * ((r & 15) >> 2) + 4 * half + (r & 3);
-                    const float v = c ? acc1[r] : acc0[r];
-                    const bool take = sk >= 0 && l + sm <= L && (bi == INT_MAX || v > best);
-                    best = take ? v : best;
-                    bi = take ? l : bi;
+                    for (int r = 0; r < 32; ++r) {
+                        tv[r] = c ? acc1[r] : acc0[r];
+                        to[r] = 32 * (r >> 4) + 8 * ((r & 15) >> 2) + (r & 3);
+                    }
+#pragma unroll
+                    for (int st = 1; st < 32; st <<= 1)
+#pragma unroll
+                        for (int i = 0; i < 32; i += 2 * st) {
+                            const bool t = tv[i + st] > tv[i];
+                            tv[i] = t ? tv[i + st] : tv[i];
+                            to[i] = t ? to[i + st] : to[i];
+                        }
+                    const bool take = sk >= 0 && (bi == INT_MAX || tv[0] > best);
+                    best = take ? tv[0] : best;
+                    bi = take ? w0 + 64 * c + 4 * half + to[0] : bi;
                 }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int r = 0; r < 32; ++r) {
+                        const int l = w0 + 64 * c + 32 * (r >> 4) + 8 * ((r & 15) >> 2) + 4 * half + (r & 3);
+                        const float v = c ? acc1[r] : acc0[r];
+                        const bool take = sk >= 0 && l + sm <= L && (bi == INT_MAX || v > best);
+                        best = take ? v : best;
+                        bi = take ? l : bi;
+                    }
+            }
         }
         // the two lanes of a slot: larger value, then smaller window
         {
