@@ -826,6 +826,24 @@ cdlm_status up(cdlm_ctx *ctx, T **dst, const std::vector<T> &src) {
     return CDLM_OK;
 }
 
+// Sequence blocks per tile group for the scan kernels, which step through the sequences with
+// a fixed stride: 64 rounds of the work-groups the device holds at once.  The work-groups have
+// equal work, and the finer the grain the less the device idles while the last ones finish
+// (4, 8, 16, 32 and 64 rounds measured; the gain levels off at 32-64).
+long scan_blocks(cdlm_ctx *ctx, const void *kfn, size_t lds, long groups, long N) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 256, lds) != hipSuccess || per_cu < 1) {
+        (void)hipGetLastError();
+        per_cu = 1;
+    }
+    long yb = (long)ctx->num_cu * per_cu * 64 / (groups > 0 ? groups : 1);
+    const long most = (N + 3) / 4;
+    if (yb > most) yb = most;
+    if (yb > 65535) yb = 65535;
+    if (yb < 1) yb = 1;
+    return yb;
+}
+
 // Launches k_pwm_hits_mfma over strand-PWM tiles already on the device (P, T tiles of up to
 // maxlen columns) with the hit counter already zeroed.  timed: HIP events around the launch
 // give ctx->last_scan_ms (waits for the kernel); otherwise the launch is only enqueued.
@@ -840,11 +858,11 @@ cdlm_status run_hits_kernel(cdlm_ctx *ctx, const uint8_t *d_bases, int L, long N
     const bool oh = 4 * (size_t)Lpad * sizeof(float4) <= kPwmLdsMax;
     const size_t lds = 4 * (size_t)Lpad * (oh ? sizeof(float4) : 1);
     if (lds > kPwmLdsMax) return cdlm_fail(ctx, CDLM_ERR_UNSUPPORTED, "sequence length %d too long for the scan", L);
-    long yb = (N + 3) / 4;
-    const long target = (long)ctx->num_cu * 16 / (P.groups > 0 ? P.groups : 1) + 1;
-    if (yb > target) yb = target;
-    if (yb > 65535) yb = 65535;
-    if (yb < 1) yb = 1;
+    const void *kfn = maxlen <= 32 ? (oh ? reinterpret_cast<const void *>(&k_pwm_hits_mfma<kPwmG, 32, kPwmCH, true>)
+                                         : reinterpret_cast<const void *>(&k_pwm_hits_mfma<kPwmG, 32, kPwmCH, false>))
+                                   : (oh ? reinterpret_cast<const void *>(&k_pwm_hits_mfma<kPwmG, 64, kPwmCH, true>)
+                                         : reinterpret_cast<const void *>(&k_pwm_hits_mfma<kPwmG, 64, kPwmCH, false>));
+    const long yb = scan_blocks(ctx, kfn, lds, P.groups, N);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timed && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) {
         (void)hipGetLastError();
@@ -968,11 +986,9 @@ cdlm_status launch_best(cdlm_ctx *ctx, const uint8_t *d_bases, int L, long N, co
             }
         }
         if (s == CDLM_OK) {
-            long yb = (N + 3) / 4;
-            const long target = (long)ctx->num_cu * 16 / (h32.T > 0 ? h32.T : 1) + 1;
-            if (yb > target) yb = target;
-            if (yb > 65535) yb = 65535;
-            if (yb < 1) yb = 1;
+            const long yb = scan_blocks(ctx, MAXJ == 32 ? reinterpret_cast<const void *>(&k_pwm_best32_mfma<32>)
+                                                    : reinterpret_cast<const void *>(&k_pwm_best32_mfma<64>),
+                                        lds32, h32.T, N);
             const long KN = (long)K * N;
             ScanTimer tm(ctx);
             if (MAXJ == 32)
@@ -1013,11 +1029,16 @@ cdlm_status launch_best(cdlm_ctx *ctx, const uint8_t *d_bases, int L, long N, co
         P.T = ht.T; P.B = dt.B; P.boff = dt.boff; P.tlen = dt.tlen;
         P.slot_k = dt.sk; P.slot_m = dt.sm; P.slot_st = dt.sst; P.slot_th = dt.sth;
         P.groups = ht.T;
-        long yb = (N + 3) / 4;
-        const long target = (long)ctx->num_cu * 16 / (P.groups > 0 ? P.groups : 1) + 1;
-        if (yb > target) yb = target;
-        if (yb > 65535) yb = 65535;
-        if (yb < 1) yb = 1;
+        const void *kfn =
+            nuc ? (MAXJ == 32 ? (oh ? reinterpret_cast<const void *>(&k_pwm_best_mfma<32, kPwmCH, true, true>)
+                                    : reinterpret_cast<const void *>(&k_pwm_best_mfma<32, kPwmCH, false, true>))
+                              : (oh ? reinterpret_cast<const void *>(&k_pwm_best_mfma<64, kPwmCH, true, true>)
+                                    : reinterpret_cast<const void *>(&k_pwm_best_mfma<64, kPwmCH, false, true>)))
+                : (MAXJ == 32 ? (oh ? reinterpret_cast<const void *>(&k_pwm_best_mfma<32, kPwmCH, true, false>)
+                                    : reinterpret_cast<const void *>(&k_pwm_best_mfma<32, kPwmCH, false, false>))
+                              : (oh ? reinterpret_cast<const void *>(&k_pwm_best_mfma<64, kPwmCH, true, false>)
+                                    : reinterpret_cast<const void *>(&k_pwm_best_mfma<64, kPwmCH, false, false>)));
+        const long yb = scan_blocks(ctx, kfn, lds, P.groups, N);
         const long KN = (long)K * N;
         const unsigned ob = (unsigned)((KN + 255) / 256);
         ScanTimer tm(ctx);
