@@ -49,14 +49,14 @@ def log(msg):
 
 
 # ------------------------------------------------------------------------- byte counts
-def survey_csc_bytes(N, Lg, M):
+def compulsory_csc_bytes(N, Lg, M):
     """Compulsory HBM bytes of one CSC iteration, 4 N Lg (6M + 7): X, y2 and y3 read and
     written (6M), y1, Y1 and DX read and written and the data read (7); persistent state only,
     every spectrum and auxiliary array recomputed on chip."""
     return 4.0 * N * Lg * (6 * M + 7)
 
 
-def survey_dict_bytes(N, Lg, M):
+def compulsory_dict_bytes(N, Lg, M):
     """Compulsory HBM bytes of one dictionary iteration, 4 N Lg (8M + 4): X, y3, d, g1 and g3
     read, d, g1 and g3 written (8M); g2 read and written, the data and Td (4)."""
     return 4.0 * N * Lg * (8 * M + 4)
@@ -710,13 +710,13 @@ def main():
     csc_launches = max(csc_n // CSC_ITER, 1)          # one launch per outer iteration
     launch_ms = csc_ms / csc_launches
     avg_dict = dict_ms / max(dict_n, 1)
-    b_launch = survey_csc_bytes(N, Lg, M_FILT) * CSC_ITER
+    b_launch = compulsory_csc_bytes(N, Lg, M_FILT) * CSC_ITER
     b_carry = carry_csc_bytes(N, Lg, M_FILT, CSC_ITER)
     achieved = b_launch / (launch_ms * 1e-3) / 1e9
     achieved_carry = b_carry / (launch_ms * 1e-3) / 1e9
-    b_step = args.outer * (CSC_ITER * survey_csc_bytes(N, Lg, M_FILT) + CDL_ITER * survey_dict_bytes(N, Lg, M_FILT))
+    b_step = args.outer * (CSC_ITER * compulsory_csc_bytes(N, Lg, M_FILT) + CDL_ITER * compulsory_dict_bytes(N, Lg, M_FILT))
     b_step_floor = args.outer * (b_carry + CDL_ITER * dict_floor_bytes(N, Lg, M_FILT))
-    b_dict = survey_dict_bytes(N, Lg, M_FILT)
+    b_dict = compulsory_dict_bytes(N, Lg, M_FILT)
     b_dict_floor = dict_floor_bytes(N, Lg, M_FILT)
     traffic, tsrc = None, None
     tpath = os.path.join(ROOT, "profiles", "csc_c2_traffic.json")
